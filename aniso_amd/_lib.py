@@ -1,0 +1,75 @@
+"""ctypes binding of libaniso_amd.so, the C ABI in include/aniso_amd.h.
+
+The library is built in-tree by ``python -m aniso_amd.build`` (or ``__graft_entry__.build()``).
+There is no fallback: if the library is missing, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libaniso_amd.so")
+
+P = ctypes.c_void_p
+
+
+class Camera_(ctypes.Structure):
+    _fields_ = [("R", ctypes.c_double * 9), ("t", ctypes.c_double * 3), ("fx", ctypes.c_double),
+                ("fy", ctypes.c_double), ("cx", ctypes.c_double), ("cy", ctypes.c_double),
+                ("width", ctypes.c_int32), ("height", ctypes.c_int32)]
+
+
+class Options_(ctypes.Structure):
+    _fields_ = [("sh_degree", ctypes.c_int32), ("splat_scale", ctypes.c_float), ("tile_w", ctypes.c_int32),
+                ("tile_h", ctypes.c_int32), ("batch_depth", ctypes.c_int32)]
+
+
+class Stats_(ctypes.Structure):
+    _fields_ = [("visible", ctypes.c_int64), ("tile_pairs", ctypes.c_int64), ("project_ms", ctypes.c_double),
+                ("sort_ms", ctypes.c_double), ("blend_ms", ctypes.c_double), ("pair_capacity", ctypes.c_int64),
+                ("retries", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+# Every symbol include/aniso_amd.h declares, with (restype, argtypes).
+SIGNATURES = {
+    "ans_available": (ctypes.c_int, []),
+    "ans_last_error": (ctypes.c_char_p, []),
+    "ans_renderer_create": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, P, P, P, P, P, ctypes.POINTER(P)]),
+    "ans_renderer_create_raw": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, P, P, P, P, P, ctypes.POINTER(P)]),
+    "ans_renderer_destroy": (None, [P]),
+    "ans_render": (ctypes.c_int, [P, ctypes.POINTER(Camera_), ctypes.POINTER(Options_), P, ctypes.c_size_t,
+                                  ctypes.POINTER(Stats_)]),
+    "ans_render_float": (ctypes.c_int, [P, ctypes.POINTER(Camera_), ctypes.POINTER(Options_), P, P,
+                                        ctypes.c_size_t, ctypes.POINTER(Stats_)]),
+    "ans_render_async": (ctypes.c_int, [P, ctypes.POINTER(Camera_), ctypes.POINTER(Options_), P, P]),
+    "ans_render_wait": (ctypes.c_int, [P, ctypes.POINTER(Stats_)]),
+    "ans_frame_counts": (ctypes.c_int, [P, P, P, P, P]),
+    "ans_frame_dump": (ctypes.c_int, [P, P, P, P, P, P]),
+    "ans_activate": (None, [ctypes.c_int64, P, P, P, P, P, P]),
+}
+
+_lib = None
+
+
+def load():
+    """Loads libaniso_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m aniso_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        msg = load().ans_last_error().decode(errors="replace")
+        if rc == 1:
+            raise ValueError(msg)
+        raise RuntimeError(msg)

@@ -1,0 +1,502 @@
+// Host side of the rasterizer: device arena, the per-frame launch sequence, and the C ABI
+// declared in include/aniso_amd.h.
+#include "aniso_amd.h"
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace ans {
+
+thread_local std::string g_lastError;
+
+void hipCheck(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string("HIP ") + what + ": " + hipGetErrorString(e));
+}
+
+int bitsFor(uint32_t v) {
+    int b = 0;
+    while (b < 32 && (1u << b) <= v) ++b;
+    return b;
+}
+
+CamParams makeCamParams(const ans_camera& c, const ans_options& o, int sceneShDegree) {
+    CamParams p{};
+    for (int i = 0; i < 9; ++i) p.R[i] = static_cast<float>(c.R[i]);
+    for (int i = 0; i < 3; ++i) p.t[i] = static_cast<float>(c.t[i]);
+    // Camera centre -R^T t in double, then cast (src/camera.cpp:19-23).
+    for (int j = 0; j < 3; ++j)
+        p.eye[j] = static_cast<float>(-(c.R[0 * 3 + j] * c.t[0] + c.R[1 * 3 + j] * c.t[1] + c.R[2 * 3 + j] * c.t[2]));
+    p.fx = static_cast<float>(c.fx);
+    p.fy = static_cast<float>(c.fy);
+    p.cx = static_cast<float>(c.cx);
+    p.cy = static_cast<float>(c.cy);
+    // The Jacobian clamp as src/render.cpp:58-59 derives it.
+    p.limX = 1.3f * static_cast<float>(0.5 * c.width / c.fx);
+    p.limY = 1.3f * static_cast<float>(0.5 * c.height / c.fy);
+    p.width = c.width;
+    p.height = c.height;
+    p.tileW = o.tile_w;
+    p.tileH = o.tile_h;
+    p.tilesX = (c.width + o.tile_w - 1) / o.tile_w;
+    p.tilesY = (c.height + o.tile_h - 1) / o.tile_h;
+    p.shDegree = std::max(0, std::min(o.sh_degree, sceneShDegree));
+    p.shCoeffs = (sceneShDegree + 1) * (sceneShDegree + 1);
+    p.splatScale = o.splat_scale;
+    return p;
+}
+
+ans_options resolveOptions(const ans_options* o) {
+    ans_options r{3, 1.0f, 16, 16, 256};
+    if (o) {
+        r.sh_degree = o->sh_degree;
+        r.splat_scale = o->splat_scale;
+        if (o->tile_w) r.tile_w = o->tile_w;
+        if (o->tile_h) r.tile_h = o->tile_h;
+        if (o->batch_depth) r.batch_depth = o->batch_depth;
+    }
+    const bool tileOk = (r.tile_w == 8 && r.tile_h == 8) || (r.tile_w == 16 && r.tile_h == 16) ||
+                        (r.tile_w == 32 && r.tile_h == 16);
+    if (!tileOk) throw std::invalid_argument("unsupported tile shape (8x8, 16x16 and 32x16 are supported)");
+    if (r.batch_depth != 64 && r.batch_depth != 128 && r.batch_depth != 256)
+        throw std::invalid_argument("batch_depth must be 64, 128 or 256");
+    if (!(r.splat_scale > 0.0f) || !std::isfinite(r.splat_scale))
+        throw std::invalid_argument("splat_scale must be positive and finite");
+    return r;
+}
+
+void checkCamera(const ans_camera* cam) {
+    if (!cam) throw std::invalid_argument("camera is null");
+    if (cam->width <= 0 || cam->height <= 0 || cam->width > 16384 || cam->height > 16384)
+        throw std::invalid_argument("image size must be in 1..16384");
+    if (!(cam->fx > 0) || !(cam->fy > 0)) throw std::invalid_argument("focal lengths must be positive");
+}
+
+// ---------------------------------------------------------------------------------------------
+
+Engine::Engine() {
+    hipCheck(hipGetDevice(&device), "hipGetDevice");
+    hipCheck(hipStreamCreateWithFlags(&ownStream, hipStreamNonBlocking), "hipStreamCreate");
+    stream = ownStream;
+    for (auto& e : ev) hipCheck(hipEventCreate(&e), "hipEventCreate");
+    ctl.reserve(1);
+    hipCheck(hipMemsetAsync(ctl.p, 0, sizeof(FrameCtl), stream), "hipMemsetAsync ctl");
+    hipCheck(hipHostMalloc(reinterpret_cast<void**>(&ctlHost), sizeof(FrameCtl), hipHostMallocDefault),
+             "hipHostMalloc");
+    hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
+}
+
+Engine::~Engine() {
+    if (ctlHost) (void)hipHostFree(ctlHost);
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ownStream) (void)hipStreamDestroy(ownStream);
+}
+
+void Engine::reserveGaussians(size_t n) {
+    const size_t m = std::max<size_t>(n, 1);
+    rec0.reserve(m); rec1.reserve(m); rec2.reserve(m);
+    rect.reserve(m); tilesTouched.reserve(m); depthKey.reserve(m); pairStart.reserve(m);
+    sortKA.reserve(m); sortVA.reserve(m); sortKB.reserve(m); sortVB.reserve(m);
+}
+
+void Engine::reservePairs(size_t cap) {
+    if (cap <= pairCap) return;
+    tileA.reserve(cap); gaussA.reserve(cap); tileB.reserve(cap); gaussB.reserve(cap);
+    pairCap = cap;
+}
+
+void Engine::reserveStatus(size_t n) {
+    const size_t partsDepth = (n + kSortTile - 1) / kSortTile;
+    const size_t partsDup = (n + kDupTile - 1) / kDupTile;
+    const size_t partsPair = (pairCap + kSortTile - 1) / kSortTile;
+    const size_t need = std::max({partsDepth * kRadix, partsDup, partsPair * kRadix, size_t(1)});
+    if (need > status.cap) {
+        // Fresh status words must not carry a tag a look-back could mistake for this frame's.
+        status.reserve(need);
+        hipCheck(hipMemsetAsync(status.p, 0, need * sizeof(uint64_t), stream), "hipMemsetAsync status");
+    }
+}
+
+// Enqueues one frame on the engine's stream. Outputs (either may be null) are device pointers.
+void Engine::enqueueFrame(const SceneDev& s, const CamParams& cp, const ans_options& o, uint8_t* out8,
+                          float* outf) {
+    const uint32_t n = static_cast<uint32_t>(s.n);
+    const uint32_t tiles = static_cast<uint32_t>(cp.tilesX) * static_cast<uint32_t>(cp.tilesY);
+    reserveGaussians(n);
+    ranges.reserve(tiles);
+    reserveStatus(n);
+    last.tiles = tiles;
+    last.n = n;
+    last.cp = cp;
+
+    hipCheck(hipEventRecord(ev[0], stream), "hipEventRecord");
+    const uint32_t initThreads = std::max<uint32_t>(tiles, (sizeof(FrameCtl) - 32) / 4);
+    k_initFrame<<<(initThreads + 255) / 256, 256, 0, stream>>>(ctl.p, ranges.p, tiles);
+    if (n > 0) {
+        k_preprocess<<<(n + 255) / 256, 256, 0, stream>>>(static_cast<int>(n), s.pos, s.scale, s.rot, s.opacity,
+                                                          s.sh, cp, rec0.p, rec1.p, rec2.p, rect.p,
+                                                          tilesTouched.p, depthKey.p, ctl.p);
+        const uint32_t histBlocks = std::min<uint32_t>(512, (n + 255) / 256);
+        k_histogram4<<<histBlocks, 256, 0, stream>>>(depthKey.p, n, &ctl.p->depthHist[0][0]);
+        // Depth sort: four stable 8-bit passes, ties keep ascending Gaussian index.
+        const uint32_t partsDepth = (n + kSortTile - 1) / kSortTile;
+        const uint32_t* kin = depthKey.p;
+        const uint32_t* vin = nullptr;
+        uint32_t* kout[4] = {sortKA.p, sortKB.p, sortKA.p, nullptr};
+        uint32_t* vout[4] = {sortVA.p, sortVB.p, sortVA.p, sortVB.p};
+        for (int pass = 0; pass < 4; ++pass) {
+            k_sortPass<<<partsDepth, kSortBlock, 0, stream>>>(kin, vin, kout[pass], vout[pass], nullptr, n, n,
+                                                              8 * pass, 8, &ctl.p->depthHist[pass][0],
+                                                              &ctl.p->partCounter[pass], status.p, ctl.p, pass);
+            kin = kout[pass];
+            vin = vout[pass];
+        }
+    }
+    hipCheck(hipEventRecord(ev[1], stream), "hipEventRecord");
+    const int tileBits = std::max(1, bitsFor(tiles - 1));
+    const int passes = (tileBits + 7) / 8;
+    const int digitBits = (tileBits + passes - 1) / passes;
+    last.sortedTile = tileA.p;
+    last.sortedGauss = gaussA.p;
+    if (n > 0) {
+        const uint32_t partsDup = (n + kDupTile - 1) / kDupTile;
+        k_expandPairs<<<partsDup, kDupBlock, 0, stream>>>(sortVB.p, tilesTouched.p, rect.p, n,
+                                                          static_cast<uint32_t>(pairCap), cp.tilesX, digitBits,
+                                                          passes, pairStart.p, tileA.p, gaussA.p,
+                                                          &ctl.p->partCounter[4], status.p, ctl.p, 4);
+        const uint32_t partsPair = static_cast<uint32_t>((pairCap + kSortTile - 1) / kSortTile);
+        uint32_t* tk[2] = {tileA.p, tileB.p};
+        uint32_t* tv[2] = {gaussA.p, gaussB.p};
+        for (int pass = 0; pass < passes; ++pass) {
+            const int src = pass & 1, dst = src ^ 1;
+            k_sortPass<<<partsPair, kSortBlock, 0, stream>>>(tk[src], tv[src], tk[dst], tv[dst], &ctl.p->pairs, 0,
+                                                             static_cast<uint32_t>(pairCap), pass * digitBits,
+                                                             digitBits, &ctl.p->pairHist[pass][0],
+                                                             &ctl.p->partCounter[5 + pass], status.p, ctl.p,
+                                                             5 + pass);
+        }
+        last.sortedTile = tk[passes & 1];
+        last.sortedGauss = tv[passes & 1];
+        k_tileRanges<<<static_cast<uint32_t>((pairCap + 255) / 256), 256, 0, stream>>>(last.sortedTile, ctl.p,
+                                                                                        ranges.p);
+    }
+    hipCheck(hipEventRecord(ev[2], stream), "hipEventRecord");
+    launchBlend(cp, o, out8, outf);
+    hipCheck(hipGetLastError(), "kernel launch");
+    hipCheck(hipEventRecord(ev[3], stream), "hipEventRecord");
+}
+
+template <int TW, int TH>
+void Engine::launchBlendT(const CamParams& cp, int batch, uint8_t* out8, float* outf) {
+    const uint32_t tiles = static_cast<uint32_t>(cp.tilesX) * cp.tilesY;
+    switch (batch) {
+        case 64:
+            k_blend<TW, TH, 64><<<tiles, TW * TH, 0, stream>>>(ranges.p, last.sortedGauss, rec0.p, rec1.p, rec2.p,
+                                                              cp.width, cp.height, cp.tilesX, out8, outf, ctl.p);
+            break;
+        case 128:
+            k_blend<TW, TH, 128><<<tiles, TW * TH, 0, stream>>>(ranges.p, last.sortedGauss, rec0.p, rec1.p, rec2.p,
+                                                               cp.width, cp.height, cp.tilesX, out8, outf, ctl.p);
+            break;
+        default:
+            k_blend<TW, TH, 256><<<tiles, TW * TH, 0, stream>>>(ranges.p, last.sortedGauss, rec0.p, rec1.p, rec2.p,
+                                                               cp.width, cp.height, cp.tilesX, out8, outf, ctl.p);
+            break;
+    }
+}
+
+void Engine::launchBlend(const CamParams& cp, const ans_options& o, uint8_t* out8, float* outf) {
+    if (o.tile_w == 8) launchBlendT<8, 8>(cp, o.batch_depth, out8, outf);
+    else if (o.tile_w == 32) launchBlendT<32, 16>(cp, o.batch_depth, out8, outf);
+    else launchBlendT<16, 16>(cp, o.batch_depth, out8, outf);
+}
+
+// Renders a frame and waits for it; grows the pair buffers and renders again if the pair count
+// exceeded them. Reads back the control block.
+int Engine::renderSync(const SceneDev& s, const CamParams& cp, const ans_options& o, uint8_t* out8, float* outf) {
+    if (pairCap == 0) reservePairs(std::max<size_t>(static_cast<size_t>(s.n) * 8, size_t(1) << 20));
+    int retries = 0;
+    while (true) {
+        enqueueFrame(s, cp, o, out8, outf);
+        hipCheck(hipMemcpyAsync(ctlHost, ctl.p, 32, hipMemcpyDeviceToHost, stream), "hipMemcpyAsync ctl");
+        hipCheck(hipMemsetAsync(&ctl.p->sticky, 0, 4, stream), "hipMemsetAsync");
+        hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        if (!ctlHost->overflow) break;
+        if (ctlHost->pairs >= (1u << 30) - 1) throw std::runtime_error("pair count exceeds 2^30");
+        reservePairs(static_cast<size_t>(ctlHost->pairs) + ctlHost->pairs / 4 + 1024);
+        ++retries;
+    }
+    last.visible = ctlHost->visible;
+    last.pairs = ctlHost->pairs;
+    last.retries = retries;
+    return retries;
+}
+
+void Engine::fillStats(ans_stats* st) {
+    if (!st) return;
+    float a = 0, b = 0, c = 0;
+    hipCheck(hipEventElapsedTime(&a, ev[0], ev[1]), "hipEventElapsedTime");
+    hipCheck(hipEventElapsedTime(&b, ev[1], ev[2]), "hipEventElapsedTime");
+    hipCheck(hipEventElapsedTime(&c, ev[2], ev[3]), "hipEventElapsedTime");
+    st->visible = last.visible;
+    st->tile_pairs = last.pairs;
+    st->project_ms = a;
+    st->sort_ms = b;
+    st->blend_ms = c;
+    st->pair_capacity = static_cast<int64_t>(pairCap);
+    st->retries = last.retries;
+    st->reserved = 0;
+}
+
+}  // namespace ans
+
+// ---------------------------------------------------------------------------------------------
+// C ABI
+
+using namespace ans;
+
+struct ans_renderer {
+    SceneDev scene;
+    DevBuf<float> pos, scale, opacity, sh;
+    DevBuf<float4> rot;
+    DevBuf<uint8_t> image8;
+    DevBuf<float> imagef;
+    std::unique_ptr<Engine> engine;
+};
+
+namespace {
+
+template <class Fn>
+int guarded(Fn&& fn) {
+    try {
+        g_lastError.clear();
+        return fn();
+    } catch (const std::invalid_argument& e) {
+        g_lastError = e.what();
+        return ANS_ERR_ARGUMENT;
+    } catch (const std::exception& e) {
+        g_lastError = e.what();
+        return ANS_ERR_HIP;
+    } catch (...) {
+        g_lastError = "unknown error";
+        return ANS_ERR_HIP;
+    }
+}
+
+void upload(void* dst, const void* src, size_t bytes) {
+    if (bytes) hipCheck(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+}
+
+}  // namespace
+
+extern "C" {
+
+int ans_available(void) {
+    int count = 0;
+    return hipGetDeviceCount(&count) == hipSuccess && count > 0 ? 1 : 0;
+}
+
+const char* ans_last_error(void) { return g_lastError.c_str(); }
+
+void ans_activate(int64_t n, const float* log_scale_xyz, const float* opacity_logit, const float* quat_wxyz,
+                  float* scale_xyz, float* opacity, float* rot_wxyz) {
+    // src/ply_loader.cpp:124-139, in float with std::exp / std::sqrt.
+    for (int64_t i = 0; i < n; ++i) {
+        for (int j = 0; j < 3; ++j) scale_xyz[3 * i + j] = std::exp(log_scale_xyz[3 * i + j]);
+        opacity[i] = 1.0f / (1.0f + std::exp(-opacity_logit[i]));
+        const float* q = quat_wxyz + 4 * i;
+        const float len = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+        for (int j = 0; j < 4; ++j) rot_wxyz[4 * i + j] = len > 0 ? q[j] / len : (j == 0 ? 1.0f : 0.0f);
+    }
+}
+
+int ans_renderer_create(int64_t n, int32_t sh_degree, const float* pos_xyz, const float* scale_xyz,
+                        const float* rot_wxyz, const float* opacity, const float* sh, ans_renderer** out) {
+    return guarded([&] {
+        if (!out) throw std::invalid_argument("out is null");
+        *out = nullptr;
+        if (n < 0 || n > (int64_t(1) << 30)) throw std::invalid_argument("Gaussian count must be in 0..2^30");
+        if (sh_degree < 0 || sh_degree > 3) throw std::invalid_argument("sh_degree must be in 0..3");
+        if (n > 0 && (!pos_xyz || !scale_xyz || !rot_wxyz || !opacity || !sh))
+            throw std::invalid_argument("scene arrays must not be null");
+        if (!ans_available()) throw std::runtime_error("no HIP device");
+        auto r = std::make_unique<ans_renderer>();
+        const size_t m = static_cast<size_t>(std::max<int64_t>(n, 1));
+        const size_t coeffs = static_cast<size_t>((sh_degree + 1) * (sh_degree + 1));
+        r->pos.reserve(3 * m); r->scale.reserve(3 * m); r->rot.reserve(m); r->opacity.reserve(m);
+        r->sh.reserve(coeffs * 3 * m);
+        upload(r->pos.p, pos_xyz, sizeof(float) * 3 * n);
+        upload(r->scale.p, scale_xyz, sizeof(float) * 3 * n);
+        upload(r->rot.p, rot_wxyz, sizeof(float) * 4 * n);
+        upload(r->opacity.p, opacity, sizeof(float) * n);
+        upload(r->sh.p, sh, sizeof(float) * coeffs * 3 * n);
+        r->scene = SceneDev{n, sh_degree, r->pos.p, r->scale.p, r->rot.p, r->opacity.p, r->sh.p};
+        r->engine = std::make_unique<Engine>();
+        *out = r.release();
+        return ANS_OK;
+    });
+}
+
+int ans_renderer_create_raw(int64_t n, int32_t sh_degree, const float* pos_xyz, const float* log_scale_xyz,
+                            const float* quat_wxyz, const float* opacity_logit, const float* sh,
+                            ans_renderer** out) {
+    return guarded([&] {
+        if (n < 0 || n > (int64_t(1) << 30)) throw std::invalid_argument("Gaussian count must be in 0..2^30");
+        if (n > 0 && (!log_scale_xyz || !quat_wxyz || !opacity_logit))
+            throw std::invalid_argument("scene arrays must not be null");
+        std::vector<float> scale(3 * n), opacity(n), rot(4 * n);
+        ans_activate(n, log_scale_xyz, opacity_logit, quat_wxyz, scale.data(), opacity.data(), rot.data());
+        const int rc = ans_renderer_create(n, sh_degree, pos_xyz, scale.data(), rot.data(), opacity.data(), sh, out);
+        if (rc != ANS_OK) throw std::runtime_error(g_lastError);
+        return ANS_OK;
+    });
+}
+
+void ans_renderer_destroy(ans_renderer* r) { delete r; }
+
+int ans_render_float(ans_renderer* r, const ans_camera* cam, const ans_options* options, uint8_t* rgb_out,
+                     float* rgbf_out, size_t pixels_capacity, ans_stats* stats) {
+    return guarded([&] {
+        if (!r) throw std::invalid_argument("renderer is null");
+        checkCamera(cam);
+        const ans_options o = resolveOptions(options);
+        const size_t pixels = static_cast<size_t>(cam->width) * cam->height;
+        if ((rgb_out || rgbf_out) && pixels_capacity < pixels)
+            throw std::invalid_argument("output buffer smaller than width * height pixels");
+        const CamParams cp = makeCamParams(*cam, o, r->scene.shDegree);
+        if (cp.tilesX > 65535 || cp.tilesY > 65535) throw std::invalid_argument("too many tiles");
+        r->image8.reserve(3 * pixels);
+        if (rgbf_out) r->imagef.reserve(3 * pixels);
+        Engine& e = *r->engine;
+        e.renderSync(r->scene, cp, o, r->image8.p, rgbf_out ? r->imagef.p : nullptr);
+        if (rgb_out)
+            hipCheck(hipMemcpy(rgb_out, r->image8.p, 3 * pixels, hipMemcpyDeviceToHost), "hipMemcpy image");
+        if (rgbf_out)
+            hipCheck(hipMemcpy(rgbf_out, r->imagef.p, 3 * pixels * sizeof(float), hipMemcpyDeviceToHost),
+                     "hipMemcpy image");
+        e.fillStats(stats);
+        return ANS_OK;
+    });
+}
+
+int ans_render(ans_renderer* r, const ans_camera* cam, const ans_options* options, uint8_t* rgb_out,
+               size_t rgb_capacity, ans_stats* stats) {
+    if (cam && rgb_out && rgb_capacity < static_cast<size_t>(std::max(cam->width, 0)) * std::max(cam->height, 0) * 3) {
+        g_lastError = "rgb_out smaller than width * height * 3 bytes";
+        return ANS_ERR_ARGUMENT;
+    }
+    return ans_render_float(r, cam, options, rgb_out, nullptr, rgb_capacity / 3, stats);
+}
+
+int ans_render_async(ans_renderer* r, const ans_camera* cam, const ans_options* options, uint8_t* dev_rgb,
+                     float* dev_rgbf) {
+    return guarded([&] {
+        if (!r) throw std::invalid_argument("renderer is null");
+        checkCamera(cam);
+        const ans_options o = resolveOptions(options);
+        const CamParams cp = makeCamParams(*cam, o, r->scene.shDegree);
+        if (cp.tilesX > 65535 || cp.tilesY > 65535) throw std::invalid_argument("too many tiles");
+        Engine& e = *r->engine;
+        if (e.pairCap == 0) e.reservePairs(std::max<size_t>(static_cast<size_t>(r->scene.n) * 8, size_t(1) << 20));
+        e.enqueueFrame(r->scene, cp, o, dev_rgb, dev_rgbf);
+        return ANS_OK;
+    });
+}
+
+int ans_render_wait(ans_renderer* r, ans_stats* stats) {
+    return guarded([&] {
+        if (!r) throw std::invalid_argument("renderer is null");
+        Engine& e = *r->engine;
+        hipCheck(hipMemcpyAsync(e.ctlHost, e.ctl.p, 32, hipMemcpyDeviceToHost, e.stream), "hipMemcpyAsync ctl");
+        hipCheck(hipMemsetAsync(&e.ctl.p->sticky, 0, 4, e.stream), "hipMemsetAsync");
+        hipCheck(hipStreamSynchronize(e.stream), "hipStreamSynchronize");
+        const bool overflow = e.ctlHost->sticky != 0;
+        e.last.visible = e.ctlHost->visible;
+        e.last.pairs = e.ctlHost->pairs;
+        e.last.retries = 0;
+        e.fillStats(stats);
+        if (overflow) {
+            e.reservePairs(static_cast<size_t>(e.ctlHost->pairs) + e.ctlHost->pairs / 4 + 1024);
+            g_lastError = "pair count exceeded the pair buffers; buffers grown, render again";
+            return ANS_ERR_CAPACITY;
+        }
+        return ANS_OK;
+    });
+}
+
+int ans_frame_counts(ans_renderer* r, int64_t* n, int64_t* visible, int64_t* pairs, int64_t* tiles) {
+    return guarded([&] {
+        if (!r) throw std::invalid_argument("renderer is null");
+        const Engine& e = *r->engine;
+        if (n) *n = e.last.n;
+        if (visible) *visible = e.last.visible;
+        if (pairs) *pairs = e.last.pairs;
+        if (tiles) *tiles = e.last.tiles;
+        return ANS_OK;
+    });
+}
+
+int ans_frame_dump(ans_renderer* r, uint32_t* splat_bits, int32_t* rect, uint64_t* keys, uint32_t* values,
+                   uint32_t* ranges) {
+    return guarded([&] {
+        if (!r) throw std::invalid_argument("renderer is null");
+        Engine& e = *r->engine;
+        const size_t n = e.last.n, P = e.last.pairs, T = e.last.tiles;
+        hipCheck(hipStreamSynchronize(e.stream), "hipStreamSynchronize");
+        std::vector<float4> r0(n), r1(n), r2(n);
+        std::vector<uint2> rc(n);
+        std::vector<uint32_t> tt(n);
+        auto down = [](void* dst, const void* src, size_t bytes) {
+            if (bytes) hipCheck(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        };
+        down(tt.data(), e.tilesTouched.p, 4 * n);
+        if (splat_bits || keys) {
+            down(r0.data(), e.rec0.p, 16 * n);
+            down(r1.data(), e.rec1.p, 16 * n);
+            down(r2.data(), e.rec2.p, 16 * n);
+        }
+        if (rect) down(rc.data(), e.rect.p, 8 * n);
+        for (size_t i = 0; i < n; ++i) {
+            const bool vis = tt[i] > 0;
+            if (splat_bits) {
+                const float v[10] = {r0[i].x, r0[i].y, r0[i].z, r0[i].w, r1[i].x, r1[i].y, r2[i].w, r2[i].x,
+                                     r2[i].y, r2[i].z};
+                if (vis) std::memcpy(splat_bits + 10 * i, v, sizeof v);
+                else std::memset(splat_bits + 10 * i, 0, sizeof v);
+            }
+            if (rect) {
+                if (vis) {
+                    rect[4 * i + 0] = static_cast<int32_t>(rc[i].x & 0xffffu);
+                    rect[4 * i + 1] = static_cast<int32_t>(rc[i].x >> 16);
+                    rect[4 * i + 2] = static_cast<int32_t>(rc[i].y & 0xffffu);
+                    rect[4 * i + 3] = static_cast<int32_t>(rc[i].y >> 16);
+                } else {
+                    rect[4 * i + 0] = 0; rect[4 * i + 1] = 0; rect[4 * i + 2] = -1; rect[4 * i + 3] = -1;
+                }
+            }
+        }
+        std::vector<uint32_t> st(P), sg(P);
+        if (P && (keys || values)) {
+            down(st.data(), e.last.sortedTile, 4 * P);
+            down(sg.data(), e.last.sortedGauss, 4 * P);
+        }
+        for (size_t p = 0; p < P; ++p) {
+            if (values) values[p] = sg[p];
+            if (keys) {
+                uint32_t bits;
+                std::memcpy(&bits, &r2[sg[p]].w, 4);
+                keys[p] = (static_cast<uint64_t>(st[p]) << 32) | bits;
+            }
+        }
+        if (ranges) down(ranges, e.ranges.p, 8 * T);
+        return ANS_OK;
+    });
+}
+
+}  // extern "C"

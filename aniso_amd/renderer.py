@@ -1,0 +1,157 @@
+"""Python mirror of the reference's GPU renderer API (include/aniso/gpu_render.h:22-38,
+include/aniso/render.h:9-36) over the C ABI:
+
+    renderer = GpuRenderer(scene)                 # uploads once
+    image = renderer.render(camera, stats, options)  # uint8 [H, W, 3], rows top to bottom
+    GpuRenderer.available()
+
+Errors raise (ValueError for bad arguments, RuntimeError for HIP failures), as the reference
+throws std::runtime_error.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+
+import numpy as np
+
+from . import _lib
+from .scenes import Camera, RawScene
+
+
+@dataclasses.dataclass
+class RenderOptions:
+    """aniso::RenderOptions (render.h:28-33) plus tile shape and LDS batch depth."""
+
+    sh_degree: int = 3
+    splat_scale: float = 1.0
+    tile: tuple = (16, 16)
+    batch_depth: int = 256
+
+    def c(self) -> _lib.Options_:
+        return _lib.Options_(int(self.sh_degree), float(self.splat_scale), int(self.tile[0]), int(self.tile[1]),
+                             int(self.batch_depth))
+
+
+@dataclasses.dataclass
+class RenderStats:
+    """aniso::RenderStats (render.h:9-13); GPU stage times from HIP events."""
+
+    visible: int = 0
+    tile_pairs: int = 0
+    project_ms: float = 0.0
+    sort_ms: float = 0.0
+    blend_ms: float = 0.0
+    pair_capacity: int = 0
+    retries: int = 0
+
+
+def camera_c(camera: Camera) -> _lib.Camera_:
+    c = _lib.Camera_()
+    for i, v in enumerate(np.asarray(camera.R, np.float64).reshape(9)):
+        c.R[i] = float(v)
+    for i, v in enumerate(np.asarray(camera.t, np.float64).reshape(3)):
+        c.t[i] = float(v)
+    c.fx, c.fy, c.cx, c.cy = float(camera.fx), float(camera.fy), float(camera.cx), float(camera.cy)
+    c.width, c.height = int(camera.width), int(camera.height)
+    return c
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class GpuRenderer:
+    """Owns a device copy of one scene; renders it from any camera."""
+
+    def __init__(self, scene, activated: bool = False):
+        """`scene` is a RawScene (raw 3DGS values, activated on the host like the reference
+        loader), or, with activated=True, any object with pos/scale/rot/opacity/sh arrays."""
+        L = _lib.load()
+        h = ctypes.c_void_p()
+        if isinstance(scene, RawScene) and not activated:
+            arrays = [_f32(scene.means), _f32(scene.log_scales), _f32(scene.quats), _f32(scene.opacity_logits),
+                      _f32(scene.sh)]
+            fn = L.ans_renderer_create_raw
+        else:
+            arrays = [_f32(scene.pos), _f32(scene.scale), _f32(scene.rot), _f32(scene.opacity), _f32(scene.sh)]
+            fn = L.ans_renderer_create
+        n = arrays[0].shape[0]
+        degree = int(round(np.sqrt(arrays[4].shape[1]))) - 1 if n else 0
+        _lib.check(fn(n, degree, *[_ptr(a) for a in arrays], ctypes.byref(h)))
+        self._h = h
+        self.n = n
+        self.sh_degree = degree
+
+    @staticmethod
+    def available() -> bool:
+        return bool(_lib.load().ans_available())
+
+    def render(self, camera: Camera, stats: RenderStats | None = None,
+               options: RenderOptions | None = None) -> np.ndarray:
+        img, _ = self.render_float(camera, stats, options, want_float=False)
+        return img
+
+    def render_float(self, camera: Camera, stats: RenderStats | None = None, options: RenderOptions | None = None,
+                     want_float: bool = True):
+        """(uint8 [H,W,3], float32 [H,W,3] unclamped colour or None)."""
+        opts = (options or RenderOptions()).c()
+        img = np.empty((camera.height, camera.width, 3), np.uint8)
+        imgf = np.empty((camera.height, camera.width, 3), np.float32) if want_float else None
+        st = _lib.Stats_()
+        _lib.check(_lib.load().ans_render_float(self._h, ctypes.byref(camera_c(camera)), ctypes.byref(opts),
+                                                _ptr(img), _ptr(imgf) if want_float else None,
+                                                camera.width * camera.height, ctypes.byref(st)))
+        if stats is not None:
+            for f in dataclasses.fields(RenderStats):
+                setattr(stats, f.name, getattr(st, f.name))
+        return img, imgf
+
+    def frame_counts(self):
+        vals = [ctypes.c_int64() for _ in range(4)]
+        _lib.check(_lib.load().ans_frame_counts(self._h, *[ctypes.byref(v) for v in vals]))
+        return tuple(v.value for v in vals)  # n, visible, pairs, tiles
+
+    def dump(self):
+        """Intermediates of the last frame: dict with splat_bits u32[n,10], rect i32[n,4],
+        keys u64[P], values u32[P], ranges u32[T,2]."""
+        n, _, pairs, tiles = self.frame_counts()
+        out = dict(splat_bits=np.empty((n, 10), np.uint32), rect=np.empty((n, 4), np.int32),
+                   keys=np.empty(pairs, np.uint64), values=np.empty(pairs, np.uint32),
+                   ranges=np.empty((tiles, 2), np.uint32))
+        _lib.check(_lib.load().ans_frame_dump(self._h, *[_ptr(out[k]) for k in
+                                                          ("splat_bits", "rect", "keys", "values", "ranges")]))
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().ans_renderer_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def activate(scene: RawScene):
+    """Host activation as the renderer library does it (src/ply_loader.cpp:124-139); returns
+    (scale [n,3], opacity [n], rot [n,4])."""
+    n = scene.n
+    scale = np.empty((n, 3), np.float32)
+    opacity = np.empty(n, np.float32)
+    rot = np.empty((n, 4), np.float32)
+    _lib.load().ans_activate(n, _ptr(_f32(scene.log_scales)), _ptr(_f32(scene.opacity_logits)),
+                             _ptr(_f32(scene.quats)), _ptr(scale), _ptr(opacity), _ptr(rot))
+    return scale, opacity, rot

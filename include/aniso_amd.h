@@ -1,0 +1,124 @@
+/* C ABI of the MI355X Gaussian-splat rasterizer (libaniso_amd.so, HIP kernels for gfx950).
+ *
+ * This is the drop-in boundary for the reference's GPU renderer. Every entry point takes plain
+ * pointers and sizes; none takes a C++ or torch type. Status: 0 = ok, nonzero = error, with the
+ * message in ans_last_error() (thread-local). The C++ shim in aniso_amd/gpu_render.hpp rethrows
+ * errors as std::runtime_error, the reference's convention (gpu_render.cu:18-21).
+ *
+ * Threading: one HIP stream per renderer handle; calls on one handle are not thread-safe;
+ * distinct handles may be used concurrently.
+ */
+#ifndef ANISO_AMD_H
+#define ANISO_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ANS_OK 0
+#define ANS_ERR_ARGUMENT 1
+#define ANS_ERR_HIP 2
+#define ANS_ERR_CAPACITY 3
+#define ANS_ERR_NO_DEVICE 4
+
+/* Pinhole camera, COLMAP convention: X_cam = R * X_world + t, R row-major, +x right, +y down,
+ * +z forward. Doubles, like aniso::Camera (include/aniso/camera.h:16-26), so the host derives
+ * the FP32 per-frame constants identically for the kernels and the CPU oracle. */
+typedef struct ans_camera {
+    double R[9];
+    double t[3];
+    double fx, fy, cx, cy;
+    int32_t width, height;
+} ans_camera;
+
+/* Render options: aniso::RenderOptions (include/aniso/render.h:28-33) plus the tile shape and
+ * the blend's LDS batch depth. Zero for tile_w / tile_h / batch_depth picks the defaults
+ * (16 x 16, 256). Supported tiles: 8x8, 16x16, 32x16 (w x h). */
+typedef struct ans_options {
+    int32_t sh_degree;   /* clamped to the scene's degree */
+    float splat_scale;   /* multiplies every standard deviation */
+    int32_t tile_w, tile_h;
+    int32_t batch_depth; /* splats staged in LDS per blend round: 64, 128 or 256 */
+} ans_options;
+
+/* aniso::RenderStats (include/aniso/render.h:9-13) plus the pair-buffer bookkeeping. Times are
+ * GPU time per stage from HIP events: project = preprocess + depth sort, sort = pair expansion
+ * + pair sort + tile ranges, blend = blend + output. */
+typedef struct ans_stats {
+    int64_t visible;
+    int64_t tile_pairs;
+    double project_ms, sort_ms, blend_ms;
+    int64_t pair_capacity; /* pair slots allocated after this call */
+    int32_t retries;       /* renders repeated because the pair count exceeded the capacity */
+    int32_t reserved;
+} ans_stats;
+
+typedef struct ans_renderer ans_renderer;
+
+/* 1 if a HIP device is present and usable (GpuRenderer::available, gpu_render.cu:262-265). */
+int ans_available(void);
+
+/* Message of the last error on this thread, "" if none. */
+const char* ans_last_error(void);
+
+/* Uploads an activated scene once (GpuRenderer::GpuRenderer(const Scene&), gpu_render.cu:267-292):
+ * pos_xyz [n][3], scale_xyz [n][3] standard deviations, rot_wxyz [n][4] unit quaternions (w first),
+ * opacity [n] in [0, 1], sh [n][(sh_degree+1)^2][3] laid out [coefficient][channel]
+ * (include/aniso/gaussian.h:33-39). Host arrays are copied; the caller keeps ownership. */
+int ans_renderer_create(int64_t n, int32_t sh_degree, const float* pos_xyz, const float* scale_xyz,
+                        const float* rot_wxyz, const float* opacity, const float* sh, ans_renderer** out);
+
+/* Same, from the raw values a 3DGS PLY stores: log-scales, opacity logits, unnormalized
+ * quaternions. Activates on the host exactly as the reference loader does
+ * (src/ply_loader.cpp:124-139: exp, 1/(1+exp(-x)), q/|q| in float). */
+int ans_renderer_create_raw(int64_t n, int32_t sh_degree, const float* pos_xyz, const float* log_scale_xyz,
+                            const float* quat_wxyz, const float* opacity_logit, const float* sh,
+                            ans_renderer** out);
+
+void ans_renderer_destroy(ans_renderer* r);
+
+/* Renders one frame into rgb_out (width * height * 3 bytes, RGB8, rows top to bottom), like
+ * GpuRenderer::render (gpu_render.cu:299-375). options and stats may be NULL. An empty view
+ * (nothing visible) gives a black image and status 0. */
+int ans_render(ans_renderer* r, const ans_camera* cam, const ans_options* options, uint8_t* rgb_out,
+               size_t rgb_capacity, ans_stats* stats);
+
+/* Same, writing the unclamped colour as float [height][width][3] (the image before the [0, 1]
+ * clamp and quantization). Either output may be NULL. */
+int ans_render_float(ans_renderer* r, const ans_camera* cam, const ans_options* options, uint8_t* rgb_out,
+                     float* rgbf_out, size_t pixels_capacity, ans_stats* stats);
+
+/* Asynchronous variant for pipelined callers (the reference's `aniso path` loop renders one
+ * view after another, main.cpp:234-244): enqueues the frame on the handle's stream and returns
+ * without waiting. dev_rgb (width*height*3 bytes) and dev_rgbf (width*height*3 floats) are
+ * DEVICE pointers; either may be NULL. Call ans_render_wait before reading them. */
+int ans_render_async(ans_renderer* r, const ans_camera* cam, const ans_options* options, uint8_t* dev_rgb,
+                     float* dev_rgbf);
+
+/* Waits for every frame enqueued on the handle and fills stats for the last one. Returns
+ * ANS_ERR_CAPACITY if any of them had more pairs than the pair buffers held; the buffers have
+ * then been grown and the caller renders those frames again. */
+int ans_render_wait(ans_renderer* r, ans_stats* stats);
+
+/* Intermediates of the last frame rendered on this handle, copied to host buffers (any may be
+ * NULL). Sizes: splat_bits [n][10] uint32 = float bits of {u, v, conic A, B, C, opacity, depth,
+ * r, g, b} (zeros for culled Gaussians); rect [n][4] int32 tile rect {x0, y0, x1, y1}
+ * (x1 < x0 when culled); keys [pairs] uint64 = tile << 32 | depth bits; values [pairs] uint32
+ * Gaussian index; ranges [tiles][2] uint32 [first, last) into the sorted pairs ([0, 0) if empty);
+ * the keys and ranges of gpu_render.cu:157 and :164-177. */
+int ans_frame_counts(ans_renderer* r, int64_t* n, int64_t* visible, int64_t* pairs, int64_t* tiles);
+int ans_frame_dump(ans_renderer* r, uint32_t* splat_bits, int32_t* rect, uint64_t* keys, uint32_t* values,
+                   uint32_t* ranges);
+
+/* Host activation used by ans_renderer_create_raw, exposed for parity tests. */
+void ans_activate(int64_t n, const float* log_scale_xyz, const float* opacity_logit, const float* quat_wxyz,
+                  float* scale_xyz, float* opacity, float* rot_wxyz);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ANISO_AMD_H */

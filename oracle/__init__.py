@@ -1,0 +1,235 @@
+"""CPU oracle for the rasterizer -- TEST INFRASTRUCTURE ONLY.
+
+Only the tests, the smoke check and the benchmark's CPU-baseline leg import this package, as the
+checker. It wraps two native libraries:
+
+* ``_build/liboracle.so``: this project's C++ restatement of the reference CPU renderer
+  (src/render.cpp, src/sh.cpp, src/camera.cpp:12-32, src/ply_loader.cpp:124-139), built from
+  ``oracle.cpp`` / ``grad.cpp`` by ``make -C oracle``;
+* ``_ref/libaniso_ref.so``: the reference CPU renderer compiled from its own sources by
+  ``make -C oracle ref`` (only where the reference tree is present; the built file travels).
+
+Modes: REFERENCE (the reference's arithmetic; reproduces the compiled reference bit for bit at
+8x8 tiles), KEYEXACT (FP32 camera transform in the kernel's op order; the bit-exact contract for
+the GPU), F64 (double throughout).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libaniso_ref.so")
+
+REFERENCE, KEYEXACT, F64 = 0, 1, 2
+
+
+class _Camera(ctypes.Structure):
+    _fields_ = [("R", ctypes.c_double * 9), ("t", ctypes.c_double * 3), ("fx", ctypes.c_double),
+                ("fy", ctypes.c_double), ("cx", ctypes.c_double), ("cy", ctypes.c_double),
+                ("width", ctypes.c_int), ("height", ctypes.c_int)]
+
+
+class _Stats(ctypes.Structure):
+    _fields_ = [("visible", ctypes.c_int64), ("pairs", ctypes.c_int64), ("project_ms", ctypes.c_double),
+                ("sort_ms", ctypes.c_double), ("blend_ms", ctypes.c_double)]
+
+
+def build(reference: bool = True) -> None:
+    """Compiles the oracle (and the compiled reference when its sources exist)."""
+    subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
+    if reference and os.path.isdir("/root/reference/src"):
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build(reference=False)
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        L.orc_frame_run.restype = P
+        L.orc_frame_run.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int, P, P, P, P, P,
+                                    ctypes.POINTER(_Camera), ctypes.c_int, ctypes.c_float, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_int]
+        L.orc_frame_free.argtypes = [P]
+        L.orc_frame_stats.argtypes = [P, ctypes.POINTER(_Stats)]
+        L.orc_frame_tile_count.restype = ctypes.c_int64
+        L.orc_frame_tile_count.argtypes = [P]
+        L.orc_frame_image.argtypes = [P, P, P]
+        L.orc_frame_ncontrib.argtypes = [P, P]
+        L.orc_frame_pairs.argtypes = [P, P, P, P]
+        L.orc_frame_splats.argtypes = [P, P, P, P, P]
+        L.orc_frame_splat_bits.argtypes = [P, P]
+        L.orc_activate.argtypes = [ctypes.c_int64, P, P, P, P, P, P]
+        L.orc_mt19937_scene.argtypes = [ctypes.c_uint32, ctypes.c_int64, P, P, P, P, P]
+        _lib = L
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_PATH)
+
+
+def ref_lib():
+    global _ref
+    if _ref is None:
+        R = ctypes.CDLL(REF_PATH)
+        P = ctypes.c_void_p
+        R.ref_render.restype = ctypes.c_int
+        R.ref_render.argtypes = [ctypes.c_int64, ctypes.c_int, P, P, P, P, P, ctypes.POINTER(_Camera),
+                                 ctypes.c_int, ctypes.c_float, P, ctypes.POINTER(_Stats)]
+        _ref = R
+    return _ref
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _cam(camera) -> _Camera:
+    c = _Camera()
+    for i, v in enumerate(np.asarray(camera.R, dtype=np.float64).reshape(9)):
+        c.R[i] = float(v)
+    for i, v in enumerate(np.asarray(camera.t, dtype=np.float64).reshape(3)):
+        c.t[i] = float(v)
+    c.fx, c.fy, c.cx, c.cy = float(camera.fx), float(camera.fy), float(camera.cx), float(camera.cy)
+    c.width, c.height = int(camera.width), int(camera.height)
+    return c
+
+
+class ActivatedScene:
+    """Activated float32 arrays as the renderers take them (gaussian.h:19-40)."""
+
+    def __init__(self, pos, scale, rot, opacity, sh):
+        f = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+        self.pos, self.scale, self.rot, self.opacity, self.sh = f(pos), f(scale), f(rot), f(opacity), f(sh)
+        self.n = self.pos.shape[0]
+        self.sh_degree = int(round(np.sqrt(self.sh.shape[1]))) - 1
+
+
+def mt19937_scene(n: int = 20000, seed: int = 7) -> ActivatedScene:
+    """The random SH3 scene of tests/test_gpu_render.cpp:41-58 (std::mt19937(7))."""
+    pos = np.empty((n, 3), np.float32)
+    scale = np.empty((n, 3), np.float32)
+    rot = np.empty((n, 4), np.float32)
+    opacity = np.empty(n, np.float32)
+    sh = np.empty((n, 16, 3), np.float32)
+    lib().orc_mt19937_scene(seed, n, _ptr(pos), _ptr(scale), _ptr(rot), _ptr(opacity), _ptr(sh))
+    return ActivatedScene(pos, scale, rot, opacity, sh)
+
+
+def activate(raw) -> ActivatedScene:
+    """The reference loader's activations (ply_loader.cpp:124-139), in float with std::exp."""
+    n = raw.n
+    scale = np.empty((n, 3), np.float32)
+    opacity = np.empty(n, np.float32)
+    rot = np.empty((n, 4), np.float32)
+    ls = np.ascontiguousarray(raw.log_scales, np.float32)
+    lg = np.ascontiguousarray(raw.opacity_logits, np.float32)
+    q = np.ascontiguousarray(raw.quats, np.float32)
+    lib().orc_activate(n, _ptr(ls), _ptr(lg), _ptr(q), _ptr(scale), _ptr(opacity), _ptr(rot))
+    return ActivatedScene(raw.means, scale, rot, opacity, raw.sh)
+
+
+class Frame:
+    """One oracle frame; every intermediate is available until close()."""
+
+    def __init__(self, scene: ActivatedScene, camera, mode=KEYEXACT, sh_degree=3, splat_scale=1.0, tile=(16, 16),
+                 threads=0):
+        self._scene = scene  # keep the arrays alive
+        self.mode = mode
+        self.camera = camera
+        self.tile = tile
+        self._h = lib().orc_frame_run(mode, scene.n, scene.sh_degree, _ptr(scene.pos), _ptr(scene.scale),
+                                      _ptr(scene.rot), _ptr(scene.opacity), _ptr(scene.sh),
+                                      ctypes.byref(_cam(camera)), sh_degree, splat_scale, tile[0], tile[1], threads)
+        if not self._h:
+            raise ValueError("oracle rejected the frame arguments")
+        st = _Stats()
+        lib().orc_frame_stats(self._h, ctypes.byref(st))
+        self.visible, self.pairs = st.visible, st.pairs
+        self.project_ms, self.sort_ms, self.blend_ms = st.project_ms, st.sort_ms, st.blend_ms
+        self.tiles = lib().orc_frame_tile_count(self._h)
+
+    def image(self) -> np.ndarray:
+        out = np.empty((self.camera.height, self.camera.width, 3), np.uint8)
+        lib().orc_frame_image(self._h, _ptr(out), None)
+        return out
+
+    def image_float(self) -> np.ndarray:
+        out = np.empty((self.camera.height, self.camera.width, 3), np.float64)
+        lib().orc_frame_image(self._h, None, _ptr(out))
+        return out
+
+    def n_contrib(self) -> np.ndarray:
+        out = np.empty((self.camera.height, self.camera.width), np.uint32)
+        lib().orc_frame_ncontrib(self._h, _ptr(out))
+        return out
+
+    def pairs_arrays(self):
+        """(keys u64[P], values u32[P], ranges u32[T,2]) as the GPU sort produces them."""
+        keys = np.empty(self.pairs, np.uint64)
+        values = np.empty(self.pairs, np.uint32)
+        ranges = np.empty((self.tiles, 2), np.uint32)
+        lib().orc_frame_pairs(self._h, _ptr(keys), _ptr(values), _ptr(ranges))
+        return keys, values, ranges
+
+    def splats(self):
+        """(records f64[n,10] = u v A B C opacity depth r g b, rect i32[n,4], radius i32[n], alive u8[n])."""
+        n = self._scene.n
+        rec = np.empty((n, 10), np.float64)
+        rect = np.empty((n, 4), np.int32)
+        radius = np.empty(n, np.int32)
+        alive = np.empty(n, np.uint8)
+        lib().orc_frame_splats(self._h, _ptr(rec), _ptr(rect), _ptr(radius), _ptr(alive))
+        return rec, rect, radius, alive
+
+    def splat_bits(self) -> np.ndarray:
+        out = np.empty((self._scene.n, 10), np.uint32)
+        lib().orc_frame_splat_bits(self._h, _ptr(out))
+        return out
+
+    def close(self):
+        if self._h:
+            lib().orc_frame_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def render(scene: ActivatedScene, camera, mode=REFERENCE, sh_degree=3, splat_scale=1.0, tile=(8, 8), threads=0):
+    """RGB8 image [H, W, 3] from the oracle."""
+    with Frame(scene, camera, mode, sh_degree, splat_scale, tile, threads) as f:
+        return f.image()
+
+
+def ref_render(scene: ActivatedScene, camera, sh_degree=3, splat_scale=1.0):
+    """RGB8 image and stats from the compiled reference CPU renderer (aniso::render)."""
+    out = np.empty((camera.height, camera.width, 3), np.uint8)
+    st = _Stats()
+    rc = ref_lib().ref_render(scene.n, scene.sh_degree, _ptr(scene.pos), _ptr(scene.scale), _ptr(scene.rot),
+                              _ptr(scene.opacity), _ptr(scene.sh), ctypes.byref(_cam(camera)), sh_degree,
+                              splat_scale, _ptr(out), ctypes.byref(st))
+    if rc != 0:
+        raise RuntimeError("reference render failed")
+    return out, st

@@ -1,0 +1,287 @@
+"""The CPU oracle is pinned before anything is compared with it:
+
+* golden hashes of the compiled reference CPU renderer (tests/golden/golden.json) on synthetic
+  scenes, posed cameras, SH0 and splat scale, and on the reference GPU test's mt19937 scene;
+* the reference's own known answers (tests/test_render.cpp, tests/test_sh.cpp), restated;
+* textbook maths in FP64 numpy (EWA covariance by matrix products, eigenvalues, projection);
+* the per-tile lists against an independent stable sort of every (tile, depth, index) pair.
+"""
+import hashlib
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from aniso_amd import scenes
+
+GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "golden.json")))["hashes"]
+
+
+def h(b: bytes) -> str:
+    return hashlib.sha256(b).hexdigest()[:16]
+
+
+@pytest.fixture(scope="module")
+def c1():
+    return oracle.activate(scenes.synthetic_scene(200_000, 640, 360))
+
+
+@pytest.fixture(scope="module")
+def mt():
+    return oracle.mt19937_scene(20000, 7)
+
+
+def mt_camera(w, hh):
+    return scenes.Camera(np.eye(3), np.zeros(3), 0.8 * w, 0.8 * w, w / 2.0, hh / 2.0, w, hh)
+
+
+def test_scene_files_match_golden():
+    for name in ("synthetic-200k-640x360", "synthetic-741883-1280x720"):
+        kw = scenes.WORKLOADS[name]
+        raw = scenes.synthetic_scene(kw["n"], kw["width"], kw["height"])
+        assert h(scenes.ply_bytes(raw)) == GOLDEN["scene:" + name]
+
+
+def test_reference_mode_matches_compiled_reference_c1(c1):
+    cam = scenes.pinhole_camera(640, 360)
+    assert h(oracle.render(c1, cam, oracle.REFERENCE).tobytes()) == GOLDEN["image:synthetic-200k-640x360:sh3"]
+    assert h(oracle.render(c1, cam, oracle.REFERENCE, sh_degree=0).tobytes()) == \
+        GOLDEN["image:synthetic-200k-640x360:sh0"]
+    assert h(oracle.render(c1, cam, oracle.REFERENCE, splat_scale=0.5).tobytes()) == \
+        GOLDEN["image:synthetic-200k-640x360:splat0.5"]
+
+
+def test_reference_mode_matches_posed_cameras(c1):
+    from tests.golden.make_golden import posed_cameras
+
+    for name, cam in posed_cameras().items():
+        assert h(oracle.render(c1, cam, oracle.REFERENCE).tobytes()) == GOLDEN[f"image:synthetic-200k:{name}:sh3"]
+
+
+def test_reference_mode_matches_c2():
+    act = oracle.activate(scenes.synthetic_scene(741_883, 1280, 720))
+    with oracle.Frame(act, scenes.pinhole_camera(1280, 720), oracle.REFERENCE, tile=(8, 8)) as f:
+        assert h(f.image().tobytes()) == GOLDEN["image:synthetic-741883-1280x720:sh3"]
+        assert (f.visible, f.pairs) == (741_883, 8_664_728)  # counts the compiled reference prints
+
+
+def test_reference_mode_matches_mt19937_scene(mt):
+    for w, hh in ((320, 240), (333, 217)):
+        for deg in (3, 0):
+            with oracle.Frame(mt, mt_camera(w, hh), oracle.REFERENCE, sh_degree=deg, tile=(8, 8)) as f:
+                assert h(f.image().tobytes()) == GOLDEN[f"image:mt19937-20k:{w}x{hh}:sh{deg}"]
+                assert [f.visible, f.pairs] == GOLDEN[f"stats:mt19937-20k:{w}x{hh}:sh{deg}"]
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="compiled reference not built")
+def test_reference_mode_equals_compiled_reference_on_random_views(mt):
+    rng = np.random.default_rng(3)
+    for k in range(4):
+        yaw = rng.uniform(-0.4, 0.4)
+        cam = scenes.pose_camera(rng.uniform(-0.3, 0.3, 3), [math.sin(yaw), rng.uniform(-0.1, 0.1), math.cos(yaw)],
+                                 [0, -1, 0], int(rng.integers(64, 300)), int(rng.integers(48, 200)), 65.0)
+        deg, scale = k % 4, float(rng.uniform(0.5, 1.5))
+        ref, st = oracle.ref_render(mt, cam, sh_degree=deg, splat_scale=scale)
+        with oracle.Frame(mt, cam, oracle.REFERENCE, sh_degree=deg, splat_scale=scale, tile=(8, 8)) as f:
+            assert np.array_equal(f.image(), ref)
+            assert (f.visible, f.pairs) == (st.visible, st.pairs)
+
+
+# --- the reference's known answers (tests/test_render.cpp, tests/test_sh.cpp) -----------------
+
+def tiny_scene(items):
+    """items: (pos, sigma, opacity, rgb) with DC colour (v - 0.5) / C0 (test_render.cpp:28-36)."""
+    n = len(items)
+    pos = np.array([it[0] for it in items], np.float32).reshape(n, 3)
+    scale = np.array([[it[1]] * 3 for it in items], np.float32).reshape(n, 3)
+    rot = np.tile(np.array([1, 0, 0, 0], np.float32), (n, 1))
+    op = np.array([it[2] for it in items], np.float32)
+    sh = np.array([[(c - 0.5) / scenes.SH_C0 for c in it[3]] for it in items], np.float32).reshape(n, 1, 3)
+    return oracle.ActivatedScene(pos, scale, rot, op, sh)
+
+
+def cam64():
+    return scenes.Camera(np.eye(3), np.zeros(3), 64.0, 64.0, 32.0, 32.0, 64, 64)
+
+
+@pytest.mark.parametrize("mode", [oracle.REFERENCE, oracle.KEYEXACT])
+def test_single_gaussian_closed_form(mode):
+    img = oracle.render(tiny_scene([((0, 0, 4), 0.2, 0.8, (1, 0, 0))]), cam64(), mode)
+    variance = 3.2 * 3.2 + 0.3  # sigma_px = 0.2 * 64 / 4, plus the 0.3 low-pass
+    expected = 0.8 * math.exp(-0.5 * (0.25 + 0.25) / variance) * 255.0
+    assert abs(int(img[32, 32, 0]) - int(expected + 0.5)) <= 1
+    assert img[32, 32, 1] == 0 and img[2, 2, 0] == 0
+
+
+@pytest.mark.parametrize("mode", [oracle.REFERENCE, oracle.KEYEXACT])
+def test_front_hides_back(mode):
+    s = tiny_scene([((0, 0, 8), 0.5, 0.99, (0, 0, 1)), ((0, 0, 4), 0.2, 0.99, (0, 1, 0))])
+    img = oracle.render(s, cam64(), mode)
+    assert img[32, 32, 1] > 240 and img[32, 32, 2] < 10
+
+
+def test_behind_camera_is_culled():
+    with oracle.Frame(tiny_scene([((0, 0, -4), 0.5, 0.99, (1, 1, 1))]), cam64(), oracle.REFERENCE) as f:
+        assert f.visible == 0 and f.image()[32, 32, 0] == 0
+
+
+def sh_probe(coeffs, direction, degree):
+    """Colour of one Gaussian at the origin seen from -direction: SH evaluated along `direction`."""
+    sh = np.zeros((1, 16, 3), np.float32)
+    for (k, c), v in coeffs.items():
+        sh[0, k, c] = v
+    d = np.asarray(direction, np.float64)
+    cam_pos = -d * 4.0
+    fwd = d
+    up = [0, -1, 0] if abs(d[1]) < 0.9 else [1, 0, 0]
+    cam = scenes.pose_camera(cam_pos, fwd, up, 32, 32, 60.0)
+    s = oracle.ActivatedScene(np.zeros((1, 3)), np.full((1, 3), 0.5), [[1, 0, 0, 0]], [0.99], sh)
+    with oracle.Frame(s, cam, oracle.F64, sh_degree=degree) as f:
+        rec = f.splats()[0]
+    return rec[0, 7:10]
+
+
+def test_sh_known_answers():
+    C0, C1 = 0.28209479177387814, 0.4886025119029199
+    for d in ((1, 0, 0), (0, 0, -1)):
+        c = sh_probe({(0, 0): 0.4}, d, 3)
+        assert abs(c[0] - (0.5 + C0 * 0.4)) < 1e-5 and abs(c[1] - 0.5) < 1e-6
+    up = sh_probe({(1, 1): 0.5}, (0, -1, 0), 3)
+    down = sh_probe({(1, 1): 0.5}, (0, 1, 0), 3)
+    side = sh_probe({(1, 1): 0.5}, (1, 0, 0), 3)
+    assert abs(up[1] - (0.5 + C1 * 0.5)) < 1e-5 and abs(down[1] - (0.5 - C1 * 0.5)) < 1e-5
+    assert abs(side[1] - 0.5) < 1e-6
+    assert abs(sh_probe({(1, 1): 0.5}, (0, -1, 0), 0)[1] - 0.5) < 1e-6
+    assert sh_probe({(0, 2): -10.0}, (0, 0, 1), 3)[2] == 0.0
+
+
+def test_sh_basis_is_orthonormal_real_sh():
+    """The 16 basis functions (with the reference's signs) are orthonormal over the sphere:
+    Monte-Carlo Gram matrix of the colours of unit-coefficient probes is 4 pi / N * B^T B = I."""
+    rng = np.random.default_rng(0)
+    dirs = rng.normal(size=(4000, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    n = len(dirs)
+    eye = np.zeros((n, 3))
+    # One Gaussian per direction; colour = 0.5 + b_k(dir) for a unit coefficient k on red.
+    B = np.zeros((n, 16))
+    pos = dirs * 5.0
+    for k in range(16):
+        sh = np.zeros((n, 16, 3), np.float32)
+        sh[:, k, 0] = 1.0
+        s = oracle.ActivatedScene(pos, np.full((n, 3), 0.01), np.tile([1, 0, 0, 0], (n, 1)), np.full(n, 0.5), sh)
+        # Camera at the origin; every Gaussian is in front of one of six axis views.
+        for axis in range(6):
+            f = np.zeros(3)
+            f[axis // 2] = 1 if axis % 2 == 0 else -1
+            upv = [0, -1, 0] if axis // 2 != 1 else [1, 0, 0]
+            cam = scenes.pose_camera(eye[0], f, upv, 64, 64, 120.0)
+            with oracle.Frame(s, cam, oracle.F64) as fr:
+                rec, _, _, alive = fr.splats()
+            m = alive.astype(bool) & (B[:, k] == 0)
+            B[m, k] = rec[m, 7] - 0.5
+    G = 4 * np.pi / n * (B.T @ B)
+    assert np.allclose(G, np.eye(16), atol=0.12)
+
+
+# --- preprocess against textbook EWA maths in FP64 ---------------------------------------------
+
+def quat_to_R(q):
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def test_preprocess_matches_ewa_textbook(mt):
+    cam = scenes.pose_camera([0.2, -0.1, 0.3], [0.1, 0.05, 1.0], [0, -1, 0], 320, 240, 60.0)
+    with oracle.Frame(mt, cam, oracle.F64, tile=(16, 16)) as f:
+        rec, rect, radius, alive = f.splats()
+    idx = np.flatnonzero(alive)[:300]
+    assert len(idx) > 200
+    W = cam.R
+    lim = np.array([1.3 * 0.5 * cam.width / cam.fx, 1.3 * 0.5 * cam.height / cam.fy])
+    for i in idx:
+        p = W @ mt.pos[i].astype(np.float64) + cam.t
+        Rq = quat_to_R(mt.rot[i].astype(np.float64))
+        # Rotation matrices are orthogonal; a transposed or mis-signed entry breaks this.
+        assert np.allclose(Rq @ Rq.T, np.eye(3), atol=1e-6)
+        Sigma = Rq @ np.diag(mt.scale[i].astype(np.float64) ** 2) @ Rq.T
+        t = np.clip(p[:2] / p[2], -lim, lim) * p[2]
+        J = np.array([[cam.fx / p[2], 0, -cam.fx * t[0] / p[2] ** 2], [0, cam.fy / p[2], -cam.fy * t[1] / p[2] ** 2]])
+        cov = J @ W @ Sigma @ W.T @ J.T + 0.3 * np.eye(2)
+        conic = np.array([[rec[i, 2], rec[i, 3]], [rec[i, 3], rec[i, 4]]])
+        assert np.allclose(conic @ cov, np.eye(2), atol=1e-5)
+        lam = np.linalg.eigvalsh(cov).max()
+        assert radius[i] == math.ceil(3 * math.sqrt(lam) - 1e-9) or abs(3 * math.sqrt(lam) - round(3 * math.sqrt(lam))) < 1e-6
+        # The oracle takes fx, fy, cx, cy as floats; the textbook uses the doubles.
+        assert abs(rec[i, 0] - (cam.fx * p[0] / p[2] + cam.cx)) < 1e-5 * max(1.0, abs(rec[i, 0]))
+        assert abs(rec[i, 1] - (cam.fy * p[1] / p[2] + cam.cy)) < 1e-5 * max(1.0, abs(rec[i, 1]))
+        assert abs(rec[i, 6] - p[2]) < 1e-9
+        # The tile rect contains the centre's tile.
+        tx, ty = int(rec[i, 0]) // 16, int(rec[i, 1]) // 16
+        if 0 <= rec[i, 0] < cam.width and 0 <= rec[i, 1] < cam.height:
+            assert rect[i, 0] <= tx <= rect[i, 2] and rect[i, 1] <= ty <= rect[i, 3]
+
+
+# --- binning and sort against an independent stable sort ---------------------------------------
+
+@pytest.mark.parametrize("tile", [(16, 16), (8, 8), (32, 16)])
+def test_tile_lists_equal_stable_sort_of_keys(mt, tile):
+    cam = scenes.pose_camera([0.1, 0.0, 0.2], [0.05, 0.0, 1.0], [0, -1, 0], 333, 217, 60.0)
+    with oracle.Frame(mt, cam, oracle.KEYEXACT, tile=tile) as f:
+        keys, values, ranges = f.pairs_arrays()
+        bits = f.splat_bits()
+        _, rect, _, alive = f.splats()
+        tiles_x = (cam.width + tile[0] - 1) // tile[0]
+    # Enumerate pairs per Gaussian in index order, row-major over its rect (gpu_render.cu:152-160).
+    gk, gv = [], []
+    for i in np.flatnonzero(alive):
+        x0, y0, x1, y1 = rect[i]
+        for ty in range(y0, y1 + 1):
+            for tx in range(x0, x1 + 1):
+                gk.append((ty * tiles_x + tx) << 32 | int(bits[i, 6]))
+                gv.append(i)
+    gk = np.array(gk, np.uint64)
+    gv = np.array(gv, np.uint32)
+    order = np.argsort(gk, kind="stable")
+    assert np.array_equal(keys, gk[order]) and np.array_equal(values, gv[order])
+    # Ranges partition the list and agree with the keys' tile field.
+    nonempty = ranges[:, 1] > ranges[:, 0]
+    assert ranges[nonempty, 1].sum() - ranges[nonempty, 0].sum() == len(keys)
+    for t in np.flatnonzero(nonempty)[:50]:
+        assert np.all((keys[ranges[t, 0]:ranges[t, 1]] >> np.uint64(32)) == t)
+
+
+def test_keyexact_and_reference_modes_agree_closely(c1):
+    cam = scenes.pinhole_camera(640, 360)
+    with oracle.Frame(c1, cam, oracle.REFERENCE, tile=(16, 16)) as a, \
+            oracle.Frame(c1, cam, oracle.KEYEXACT, tile=(16, 16)) as b:
+        ka, _, _ = a.pairs_arrays()
+        kb, _, _ = b.pairs_arrays()
+        ia, ib = a.image(), b.image()
+    # The FP32 camera transform moves a few depths by an ulp; images agree within 1/255.
+    n = min(len(ka), len(kb))
+    differing = int(np.count_nonzero(ka[:n] != kb[:n])) + abs(len(ka) - len(kb))
+    assert differing < 0.05 * len(ka)
+    assert np.abs(ia.astype(int) - ib.astype(int)).max() <= 1
+
+
+def test_ply_round_trip(tmp_path):
+    raw = scenes.synthetic_scene(1000, 64, 48, seed=5)
+    p = tmp_path / "s.ply"
+    scenes.write_ply(p, raw)
+    back = scenes.read_ply(p)
+    for a, b in zip((raw.means, raw.log_scales, raw.quats, raw.opacity_logits, raw.sh),
+                    (back.means, back.log_scales, back.quats, back.opacity_logits, back.sh)):
+        assert np.array_equal(a, b)
+
+
+def test_ply_rejects_ascii(tmp_path):
+    p = tmp_path / "a.ply"
+    p.write_bytes(b"ply\nformat ascii 1.0\nelement vertex 1\nproperty float x\nend_header\n0\n")
+    with pytest.raises(scenes.PlyError):
+        scenes.read_ply(p)
