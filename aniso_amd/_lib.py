@@ -22,7 +22,10 @@ class Camera_(ctypes.Structure):
 
 class Options_(ctypes.Structure):
     _fields_ = [("sh_degree", ctypes.c_int32), ("splat_scale", ctypes.c_float), ("tile_w", ctypes.c_int32),
-                ("tile_h", ctypes.c_int32), ("batch_depth", ctypes.c_int32)]
+                ("tile_h", ctypes.c_int32), ("batch_depth", ctypes.c_int32), ("flags", ctypes.c_int32)]
+
+
+FLAG_NO_CULL = 1
 
 
 class Stats_(ctypes.Structure):

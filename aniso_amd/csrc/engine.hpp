@@ -45,6 +45,7 @@ struct SceneDev {
     const float4* rot = nullptr;
     const float* opacity = nullptr;
     const float* sh = nullptr;
+    bool shPlanar = false;  // [coefficient * 3 + channel][n] instead of [n][coefficient][channel]
 };
 
 CamParams makeCamParams(const ans_camera& c, const ans_options& o, int sceneShDegree);
@@ -76,7 +77,10 @@ struct Engine {
     DevBuf<uint32_t> tileA, gaussA, tileB, gaussB;
     size_t pairCap = 0;
     DevBuf<uint2> ranges;
-    DevBuf<uint64_t> status;
+    DevBuf<uint32_t> hist;        // per-block digit histograms, scanned in place
+    DevBuf<uint32_t> rowTotal;    // per-digit totals of one pass
+    DevBuf<uint32_t> blockPairs;  // pairs per expansion block, scanned in place
+    DevBuf<uint32_t> blockVisible;  // visible Gaussians per expansion block
     LastFrame last;
 
     Engine();
@@ -86,11 +90,13 @@ struct Engine {
 
     void reserveGaussians(size_t n);
     void reservePairs(size_t cap);
-    void reserveStatus(size_t n);
+    void reserveScratch(size_t n);
     void enqueueFrame(const SceneDev& s, const CamParams& cp, const ans_options& o, uint8_t* out8, float* outf);
     void launchBlend(const CamParams& cp, const ans_options& o, uint8_t* out8, float* outf);
     template <int TW, int TH>
-    void launchBlendT(const CamParams& cp, int batch, uint8_t* out8, float* outf);
+    void launchBlendT(const CamParams& cp, int batch, bool cull, uint8_t* out8, float* outf);
+    template <int TW, int TH, int BATCH>
+    void launchBlendB(const CamParams& cp, bool cull, uint8_t* out8, float* outf);
     int renderSync(const SceneDev& s, const CamParams& cp, const ans_options& o, uint8_t* out8, float* outf);
     void fillStats(ans_stats* st);
 };

@@ -19,8 +19,6 @@ __global__ __launch_bounds__(256) void k_initFrame(FrameCtl* __restrict__ ctl, u
         ctl->pairs = 0;
         ctl->overflow = 0;
     }
-    constexpr uint32_t words = (sizeof(FrameCtl) - 32) / 4;
-    if (i < words) reinterpret_cast<uint32_t*>(ctl)[8 + i] = 0;
     if (i < tiles) ranges[i] = make_uint2(0, 0);
 }
 
@@ -36,7 +34,10 @@ __constant__ float kShC3[7] = {-0.5900435899266435f, 2.890611442640554f, -0.4570
                                -0.4570457994644658f, 1.445305721320277f, -0.5900435899266435f};
 
 // SH colour along a unit direction (src/sh.cpp:18-58, same basis order and operation order).
-__device__ inline float3 shColor(const float* __restrict__ sh, int degree, float x, float y, float z) {
+// Coefficient j = k * 3 + channel sits at sh[j * stride]: stride 1 for the [coefficient][channel]
+// layout of one Gaussian, stride n for the planar device layout [coefficient * 3 + channel][n].
+template <bool PLANAR>
+__device__ inline float3 shColor(const float* __restrict__ sh, size_t stride, int degree, float x, float y, float z) {
     float b[16];
     int count = 1;
     b[0] = 0.28209479177387814f;
@@ -67,12 +68,17 @@ __device__ inline float3 shColor(const float* __restrict__ sh, int degree, float
     }
     float3 c = make_float3(0.5f, 0.5f, 0.5f);
     if (count == 16) {
-        const float4* s4 = reinterpret_cast<const float4*>(sh);
         float v[48];
+        if constexpr (PLANAR) {
 #pragma unroll
-        for (int q = 0; q < 12; ++q) {
-            const float4 t = s4[q];
-            v[4 * q] = t.x; v[4 * q + 1] = t.y; v[4 * q + 2] = t.z; v[4 * q + 3] = t.w;
+            for (int j = 0; j < 48; ++j) v[j] = sh[j * stride];
+        } else {
+            const float4* s4 = reinterpret_cast<const float4*>(sh);
+#pragma unroll
+            for (int q = 0; q < 12; ++q) {
+                const float4 t = s4[q];
+                v[4 * q] = t.x; v[4 * q + 1] = t.y; v[4 * q + 2] = t.z; v[4 * q + 3] = t.w;
+            }
         }
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
@@ -84,9 +90,9 @@ __device__ inline float3 shColor(const float* __restrict__ sh, int degree, float
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
             if (k < count) {
-                c.x += b[k] * sh[k * 3 + 0];
-                c.y += b[k] * sh[k * 3 + 1];
-                c.z += b[k] * sh[k * 3 + 2];
+                c.x += b[k] * sh[(k * 3 + 0) * stride];
+                c.y += b[k] * sh[(k * 3 + 1) * stride];
+                c.z += b[k] * sh[(k * 3 + 2) * stride];
             }
         }
     }
@@ -108,6 +114,8 @@ __device__ inline float clampf(float v, float lo, float hi) { return (v < lo) ? 
 // tile rect, the tile count and the depth sort key (0xffffffff for culled Gaussians). ext_x/y are
 // conservative half-extents of the region where alpha can reach 1/255, used by the blend to skip
 // splats that cannot touch a wavefront's pixels (they do not affect any output).
+// SH is read planar ([coefficient * 3 + channel][n], coalesced) when PLANAR, else per Gaussian.
+template <bool PLANAR>
 __global__ __launch_bounds__(256) void k_preprocess(int n, const float* __restrict__ pos,
                                                     const float* __restrict__ scale, const float4* __restrict__ rot,
                                                     const float* __restrict__ opacity, const float* __restrict__ sh,
@@ -116,7 +124,6 @@ __global__ __launch_bounds__(256) void k_preprocess(int n, const float* __restri
                                                     uint2* __restrict__ rect, uint32_t* __restrict__ tilesTouched,
                                                     uint32_t* __restrict__ depthKey, FrameCtl* __restrict__ ctl) {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    bool vis = false;
     if (i < n) {
         uint32_t touched = 0;
         uint32_t key = 0xffffffffu;
@@ -182,7 +189,8 @@ __global__ __launch_bounds__(256) void k_preprocess(int n, const float* __restri
 
             const float d0 = g.x - cam.eye[0], d1 = g.y - cam.eye[1], d2 = g.z - cam.eye[2];
             const float len = sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
-            const float3 rgb = shColor(sh + static_cast<size_t>(i) * cam.shCoeffs * 3, cam.shDegree, d0 / len,
+            const float* shBase = PLANAR ? sh + i : sh + static_cast<size_t>(i) * cam.shCoeffs * 3;
+            const float3 rgb = shColor<PLANAR>(shBase, PLANAR ? static_cast<size_t>(n) : 1, cam.shDegree, d0 / len,
                                        d1 / len, d2 / len);
             const float op = opacity[i];
 
@@ -202,125 +210,150 @@ __global__ __launch_bounds__(256) void k_preprocess(int n, const float* __restri
                                  static_cast<uint32_t>(x1) | (static_cast<uint32_t>(y1) << 16));
             touched = static_cast<uint32_t>((x1 - x0 + 1) * (y1 - y0 + 1));
             key = __float_as_uint(pz);
-            vis = true;
         } while (false);
         tilesTouched[i] = touched;
         depthKey[i] = key;
     }
-    const uint64_t m = __ballot(vis);
-    if (__lane_id() == 0 && m) atomicAdd(&ctl->visible, static_cast<uint32_t>(__popcll(m)));
 }
 
-constexpr int kDupBlock = 256;
-constexpr int kDupItems = 4;
-constexpr int kDupTile = kDupBlock * kDupItems;
+constexpr int kDupBlock = 256;                       // 4 wavefronts x 64 depth ranks
+constexpr int kDupTile = kDupBlock;                  // depth ranks per block
 
-// Pair expansion in depth order. Ranks [part * 1024, +1024) of the depth-sorted Gaussians get
-// their pair offsets by a block scan plus decoupled look-back (the prefix sum of
-// gpu_render.cu:324-327 without the host read-back of :329); the block then writes its pairs
-// cooperatively, one pair per lane per step, each Gaussian's tiles in row-major order
-// (gpu_render.cu:155-156). Pair = (tile id, Gaussian index). pairStart[g] records each
-// Gaussian's first pair slot. Digit histograms of the tile ids feed the pair sort.
-// A pair count above `cap` sets ctl->overflow and writes nothing past the capacity.
+// Pairs and visible Gaussians per block of 256 depth ranks.
+__global__ __launch_bounds__(kDupBlock) void k_pairTotals(const uint32_t* __restrict__ sortedIdx,
+                                                          const uint32_t* __restrict__ tilesTouched, uint32_t n,
+                                                          uint32_t* __restrict__ blockPairs,
+                                                          uint32_t* __restrict__ blockVisible) {
+    __shared__ uint32_t sTmp[kDupBlock / kWave];
+    const uint32_t r = blockIdx.x * kDupTile + threadIdx.x;
+    const uint32_t c = r < n ? tilesTouched[sortedIdx[r]] : 0u;
+    uint32_t total, visible;
+    blockExclusiveScan<kDupBlock>(c, sTmp, &total);
+    blockExclusiveScan<kDupBlock>(c ? 1u : 0u, sTmp, &visible);
+    if (threadIdx.x == 0) {
+        blockPairs[blockIdx.x] = total;
+        blockVisible[blockIdx.x] = visible;
+    }
+}
+
+// One workgroup: exclusive scan of the per-block pair counts; publishes the pair total (the sum
+// gpu_render.cu:324-329 reads back to the host) and the visible count, and flags a pair total
+// above the pair buffers' capacity.
+__global__ __launch_bounds__(kDupBlock) void k_scanPairs(uint32_t* __restrict__ blockPairs,
+                                                         const uint32_t* __restrict__ blockVisible, uint32_t blocks,
+                                                         uint32_t cap, FrameCtl* __restrict__ ctl) {
+    __shared__ uint32_t sTmp[kDupBlock / kWave];
+    constexpr int kPer = 8;
+    uint64_t carry = 0;
+    uint32_t visible = 0;
+    for (uint32_t base = 0; base < blocks; base += kDupBlock * kPer) {
+        const uint32_t first = base + threadIdx.x * kPer;
+        uint32_t v[kPer];
+        uint32_t sum = 0, vis = 0;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const bool in = first + k < blocks;
+            v[k] = in ? blockPairs[first + k] : 0u;
+            vis += in ? blockVisible[first + k] : 0u;
+            sum += v[k];
+        }
+        uint32_t total, visTotal;
+        uint32_t run = static_cast<uint32_t>(carry) + blockExclusiveScan<kDupBlock>(sum, sTmp, &total);
+        blockExclusiveScan<kDupBlock>(vis, sTmp, &visTotal);
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            if (first + k < blocks) blockPairs[first + k] = run;
+            run += v[k];
+        }
+        carry += total;
+        visible += visTotal;
+    }
+    if (threadIdx.x == 0) {
+        ctl->visible = visible;
+        ctl->pairs = carry > 0xffffffffull ? 0xffffffffu : static_cast<uint32_t>(carry);
+        if (carry > cap) {
+            ctl->overflow = 1;
+            ctl->sticky = 1;
+        }
+    }
+}
+
+// Inclusive max-scan across a 64-lane wavefront.
+__device__ inline int waveInclusiveMax(int v) {
+    const int lane = __lane_id();
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const int t = __shfl_up(v, d, kWave);
+        if (lane >= d) v = max(v, t);
+    }
+    return v;
+}
+
+// Pair expansion in depth order: each wavefront takes 64 consecutive depth ranks and writes their
+// pairs (tile id, Gaussian index) contiguously, each Gaussian's tiles row-major (the order
+// gpu_render.cu:155-156 writes them), 64 pairs per step with one pair per lane. A step finds
+// the Gaussian owning each slot by marking the slots where Gaussians start (LDS head flags) and
+// a wavefront max-scan. pairStart[g] records each Gaussian's first slot, the contiguous range
+// the backward pass reduces over.
 __global__ __launch_bounds__(kDupBlock) void k_expandPairs(
     const uint32_t* __restrict__ sortedIdx, const uint32_t* __restrict__ tilesTouched, const uint2* __restrict__ rect,
-    uint32_t n, uint32_t cap, int tilesX, int digitBits, int passes, uint32_t* __restrict__ pairStart,
-    uint32_t* __restrict__ tileOut, uint32_t* __restrict__ gaussOut, uint32_t* __restrict__ partCounter,
-    uint64_t* __restrict__ status, FrameCtl* __restrict__ ctl, uint32_t pass) {
-    __shared__ uint32_t sIncl[kDupTile];
-    __shared__ uint32_t sGauss[kDupTile];
-    __shared__ uint2 sRect[kDupTile];
-    __shared__ uint32_t sHist[3 * kRadix];
-    __shared__ uint32_t sScan[kDupBlock / kWave];
-    __shared__ uint32_t sPart, sBase;
-
-    const int tid = threadIdx.x;
-    if (tid == 0) sPart = atomicAdd(partCounter, 1u);
-    for (int i = tid; i < 3 * kRadix; i += kDupBlock) sHist[i] = 0;
-    __syncthreads();
-    const uint32_t part = sPart;
-    const uint32_t base = part * kDupTile;
-
-    uint32_t cnt[kDupItems];
-    uint32_t sum = 0;
-#pragma unroll
-    for (int k = 0; k < kDupItems; ++k) {
-        const uint32_t r = base + tid * kDupItems + k;
-        uint32_t c = 0, g = 0;
-        if (r < n) {
-            g = sortedIdx[r];
-            c = tilesTouched[g];
-        }
-        sGauss[tid * kDupItems + k] = g;
-        cnt[k] = c;
-        sum += c;
+    uint32_t n, int tilesX, const uint32_t* __restrict__ blockBase, uint32_t* __restrict__ pairStart,
+    uint32_t* __restrict__ tileOut, uint32_t* __restrict__ gaussOut, const FrameCtl* __restrict__ ctl) {
+    __shared__ uint32_t sWave[kDupBlock / kWave];
+    __shared__ int sHead[kDupBlock / kWave][kWave];
+    if (ctl->overflow) return;
+    const int wave = threadIdx.x / kWave, lane = __lane_id();
+    const uint32_t r = blockIdx.x * kDupTile + threadIdx.x;
+    uint32_t g = 0, cnt = 0;
+    uint2 rp = make_uint2(0, 0);
+    if (r < n) {
+        g = sortedIdx[r];
+        cnt = tilesTouched[g];
+        if (cnt) rp = rect[g];
     }
-    uint32_t total;
-    const uint32_t excl = blockExclusiveScan<kDupBlock>(sum, sScan, &total);
-    if (tid == 0) {
-        const uint32_t tag = ctl->epoch * 16u + pass;
-        uint64_t* st = status + part;
-        uint32_t earlier = 0;
-        if (part == 0) {
-            storeStatus(st, statusWord(tag, kFlagPrefix, total));
-        } else {
-            storeStatus(st, statusWord(tag, kFlagAggregate, total));
-            earlier = lookBack(status, part, 1, 0, tag);
-            storeStatus(st, statusWord(tag, kFlagPrefix, earlier + total));
-        }
-        sBase = earlier;
-        if (part == gridDim.x - 1) {
-            const uint64_t all = static_cast<uint64_t>(earlier) + total;
-            ctl->pairs = static_cast<uint32_t>(all > 0xffffffffull ? 0xffffffffull : all);
-            if (all > cap) {
-                ctl->overflow = 1;
-                ctl->sticky = 1;
+    const uint32_t incl = waveInclusiveScan(cnt);
+    const uint32_t excl = incl - cnt;
+    const uint32_t waveTotal = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), kWave - 1));
+    if (lane == 0) sWave[wave] = waveTotal;
+    __syncthreads();
+    uint32_t out = blockBase[blockIdx.x];
+    for (int w = 0; w < wave; ++w) out += sWave[w];
+    if (cnt) pairStart[g] = out + excl;
+
+    const uint32_t x0 = rp.x & 0xffffu, y0 = rp.x >> 16;
+    const uint32_t w = (rp.y & 0xffffu) - x0 + 1;
+    int carry = -1;
+    for (uint32_t chunk = 0; chunk < waveTotal; chunk += kWave) {
+        waveSync();
+        sHead[wave][lane] = -1;
+        waveSync();
+        if (cnt && excl >= chunk && excl < chunk + kWave) sHead[wave][excl - chunk] = lane;
+        waveSync();
+        const int owner = max(carry, waveInclusiveMax(sHead[wave][lane]));
+        carry = __builtin_amdgcn_readlane(owner, kWave - 1);
+        const uint32_t oStart = static_cast<uint32_t>(__shfl(static_cast<int>(excl), owner, kWave));
+        const uint32_t oX0 = static_cast<uint32_t>(__shfl(static_cast<int>(x0), owner, kWave));
+        const uint32_t oY0 = static_cast<uint32_t>(__shfl(static_cast<int>(y0), owner, kWave));
+        const uint32_t oW = static_cast<uint32_t>(__shfl(static_cast<int>(w), owner, kWave));
+        const uint32_t oG = static_cast<uint32_t>(__shfl(static_cast<int>(g), owner, kWave));
+        const uint32_t slot = chunk + lane;
+        if (slot < waveTotal) {
+            // row = k / oW with a float reciprocal and one correction each way (rows are far
+            // below 2^20, so the estimate is within one of the quotient).
+            const uint32_t k = slot - oStart;
+            uint32_t row = static_cast<uint32_t>(static_cast<float>(k) * __builtin_amdgcn_rcpf(static_cast<float>(oW)));
+            int32_t col = static_cast<int32_t>(k) - static_cast<int32_t>(row * oW);
+            if (col < 0) {
+                row -= 1;
+                col += static_cast<int32_t>(oW);
+            } else if (col >= static_cast<int32_t>(oW)) {
+                row += 1;
+                col -= static_cast<int32_t>(oW);
             }
+            tileOut[out + slot] = (oY0 + row) * static_cast<uint32_t>(tilesX) + oX0 + static_cast<uint32_t>(col);
+            gaussOut[out + slot] = oG;
         }
     }
-    uint32_t run = excl;
-#pragma unroll
-    for (int k = 0; k < kDupItems; ++k) {
-        run += cnt[k];
-        sIncl[tid * kDupItems + k] = run;
-    }
-    __syncthreads();
-    const uint32_t gbase = sBase;
-#pragma unroll
-    for (int k = 0; k < kDupItems; ++k) {
-        const uint32_t r = base + tid * kDupItems + k;
-        const uint32_t j = tid * kDupItems + k;
-        if (r < n && cnt[k] > 0) {
-            const uint32_t g = sGauss[j];
-            pairStart[g] = gbase + sIncl[j] - cnt[k];
-            sRect[j] = rect[g];
-        }
-    }
-    __syncthreads();
-    if (static_cast<uint64_t>(gbase) + total > cap) return;
-
-    const uint32_t dmask = (1u << digitBits) - 1u;
-    for (uint32_t s = tid; s < total; s += kDupBlock) {
-        // First local rank whose inclusive end exceeds s.
-        uint32_t lo = 0, hi = kDupTile - 1;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (sIncl[mid] > s) hi = mid; else lo = mid + 1;
-        }
-        const uint32_t start = lo ? sIncl[lo - 1] : 0u;
-        const uint32_t k = s - start;
-        const uint2 rp = sRect[lo];
-        const uint32_t x0 = rp.x & 0xffffu, y0 = rp.x >> 16, x1 = rp.y & 0xffffu;
-        const uint32_t w = x1 - x0 + 1;
-        const uint32_t ty = y0 + k / w, tx = x0 + k % w;
-        const uint32_t tile = ty * static_cast<uint32_t>(tilesX) + tx;
-        tileOut[gbase + s] = tile;
-        gaussOut[gbase + s] = sGauss[lo];
-        for (int p = 0; p < passes; ++p) atomicAdd(&sHist[p * kRadix + ((tile >> (p * digitBits)) & dmask)], 1u);
-    }
-    __syncthreads();
-    for (int i = tid; i < passes * kRadix; i += kDupBlock)
-        if (sHist[i]) atomicAdd(&ctl->pairHist[0][0] + i, sHist[i]);
 }
 
 // Start and end of each tile's run in the sorted pairs (gpu_render.cu:164-177); ranges were
@@ -336,13 +369,35 @@ __global__ __launch_bounds__(256) void k_tileRanges(const uint32_t* __restrict__
     if (p == total - 1 || sortedTile[p + 1] != t) ranges[t].y = p + 1;
 }
 
+// Smallest value of q(d) = A dx^2 + 2 B dx dy + C dy^2 (A, C > 0, AC > B^2) over the box
+// dx in [dx0, dx1], dy in [dy0, dy1]: 0 if the box holds the origin, else the least of the
+// four edge minima (q is convex, so its minimum over the box lies on the boundary).
+// rB_C = -B / C and rB_A = -B / A are the unconstrained minimizers per unit of the other axis.
+__device__ inline float quadMinOverBox(float A, float B, float C, float rB_C, float rB_A, float dx0, float dx1,
+                                       float dy0, float dy1) {
+    if (dx0 <= 0.0f && dx1 >= 0.0f && dy0 <= 0.0f && dy1 >= 0.0f) return 0.0f;
+    auto q = [&](float dx, float dy) { return A * dx * dx + 2.0f * B * dx * dy + C * dy * dy; };
+    const float a = q(dx0, fminf(fmaxf(rB_C * dx0, dy0), dy1));
+    const float b = q(dx1, fminf(fmaxf(rB_C * dx1, dy0), dy1));
+    const float c = q(fminf(fmaxf(rB_A * dy0, dx0), dx1), dy0);
+    const float d = q(fminf(fmaxf(rB_A * dy1, dx0), dx1), dy1);
+    return fminf(fminf(a, b), fminf(c, d));
+}
+
 // Front-to-back blend, one workgroup per TW x TH tile, one lane per pixel; each wavefront owns
-// an 8 x 8 block of the tile (src/gpu_render.cu:180-233 and src/render.cpp:193-221 define the
-// per-pixel rule: skip power > 0, alpha = min(0.99, o exp(power)), skip alpha < 1/255, stop once
-// T (1 - alpha) < 1e-4, C += rgb alpha T). BATCH splat records are staged in LDS per round;
-// a wavefront skips records whose 1/255 extent misses its 8 x 8 block, stops when all of its
-// pixels are done, and the workgroup stops when all of its wavefronts are.
-template <int TW, int TH, int BATCH>
+// an 8 x 8 block of the tile. The per-pixel rule is the reference's (src/gpu_render.cu:180-233,
+// src/render.cpp:193-221): skip power > 0; alpha = min(0.99, o exp(power)); skip
+// alpha < 1/255; stop once T (1 - alpha) < 1e-4; C += rgb alpha T. power is evaluated in the
+// same operation order as the CPU renderer, so every skip/stop decision matches it except where
+// the exponential's last bits move alpha across 1/255 or T across 1e-4.
+//
+// Per round, up to BATCH splat records of the tile's depth-sorted list are staged in LDS. With
+// CULL, the staging lane also tests the splat against each wavefront's 8 x 8 block: a splat
+// whose region alpha >= 1/255 (q <= 2 ln(255 o), widened by 5% + 0.1 for rounding) misses the
+// block cannot change any of its pixels, so the block's wavefront never visits it. Each
+// wavefront compacts the indices it must visit with ballots, walks them in order, and stops as
+// soon as all of its pixels are done; the workgroup stops when all wavefronts are.
+template <int TW, int TH, int BATCH, bool CULL>
 __global__ __launch_bounds__(TW* TH) void k_blend(const uint2* __restrict__ ranges,
                                                   const uint32_t* __restrict__ sortedGauss,
                                                   const float4* __restrict__ rec0, const float4* __restrict__ rec1,
@@ -350,8 +405,12 @@ __global__ __launch_bounds__(TW* TH) void k_blend(const uint2* __restrict__ rang
                                                   uint8_t* __restrict__ out8, float* __restrict__ outf,
                                                   const FrameCtl* __restrict__ ctl) {
     constexpr int kThreads = TW * TH;
+    constexpr int kWaves = kThreads / kWave;
     constexpr int kWavesX = TW / 8;
+    static_assert(kWaves <= 8, "at most eight 8x8 blocks per tile");
     __shared__ float4 s0[BATCH], s1[BATCH], s2[BATCH];
+    __shared__ uint8_t sMask[BATCH];
+    __shared__ uint8_t sList[kWaves][BATCH];
 
     if (ctl->overflow) return;
     const int tile = blockIdx.x;
@@ -362,8 +421,7 @@ __global__ __launch_bounds__(TW* TH) void k_blend(const uint2* __restrict__ rang
     const int x = bx + (lane & 7), y = by + (lane >> 3);
     const bool inside = x < width && y < height;
     const float px = x + 0.5f, py = y + 0.5f;
-    // Pixel-centre bounds of this wavefront's 8 x 8 block.
-    const float bx0 = bx + 0.5f, bx1 = bx + 7.5f, by0 = by + 0.5f, by1 = by + 7.5f;
+    const uint64_t below = lanesBelow();
 
     const uint2 range = ranges[tile];
     float T = 1.0f;
@@ -374,38 +432,68 @@ __global__ __launch_bounds__(TW* TH) void k_blend(const uint2* __restrict__ rang
         const int cnt = min(static_cast<uint32_t>(BATCH), range.y - start);
         for (int k = tid; k < cnt; k += kThreads) {
             const uint32_t g = sortedGauss[start + k];
-            s0[k] = rec0[g];
-            s1[k] = rec1[g];
+            const float4 a = rec0[g];
+            const float4 b = rec1[g];
+            s0[k] = a;
+            s1[k] = b;
             s2[k] = rec2[g];
+            if constexpr (CULL) {
+                uint32_t m = 0;
+                const float op = b.y;
+                if (op >= 1.0f / 255.0f) {
+                    const float kq = fmaxf(2.0f * __logf(255.0f * op), 0.0f) * 1.05f + 0.1f;
+                    const float A = a.z, B = a.w, C = b.x;
+                    const float rB_C = -B / C, rB_A = -B / A;
+#pragma unroll
+                    for (int w = 0; w < kWaves; ++w) {
+                        const float x0 = tileX * TW + (w % kWavesX) * 8 + 0.5f;
+                        const float y0 = tileY * TH + (w / kWavesX) * 8 + 0.5f;
+                        const float dx0 = x0 - a.x, dy0 = y0 - a.y;
+                        if (quadMinOverBox(A, B, C, rB_C, rB_A, dx0, dx0 + 7.0f, dy0, dy0 + 7.0f) <= kq) m |= 1u << w;
+                    }
+                }
+                sMask[k] = static_cast<uint8_t>(m);
+            }
         }
         __syncthreads();
-        if (__ballot(!done) != 0) {
-            for (int j = 0; j < cnt; ++j) {
-                const float4 a = s0[j];
-                const float4 b = s1[j];
-                if (a.x + b.z < bx0 || a.x - b.z > bx1 || a.y + b.w < by0 || a.y - b.w > by1) continue;
-                if (!done) {
-                    const float dx = px - a.x, dy = py - a.y;
-                    const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
-                    if (!(power > 0.0f)) {
-                        const float alpha = fminf(0.99f, b.y * __expf(power));
-                        if (alpha >= 1.0f / 255.0f) {
-                            const float nextT = T * (1.0f - alpha);
-                            if (nextT < 1e-4f) {
-                                done = true;
-                            } else {
-                                const float4 c = s2[j];
-                                const float w = alpha * T;
-                                cr = __builtin_fmaf(c.x, w, cr);
-                                cg = __builtin_fmaf(c.y, w, cg);
-                                cb = __builtin_fmaf(c.z, w, cb);
-                                T = nextT;
-                            }
+        if (__ballot(!done) == 0) continue;
+        int count = cnt;
+        if constexpr (CULL) {
+            count = 0;
+            for (int g0 = 0; g0 < cnt; g0 += kWave) {
+                const int j = g0 + lane;
+                const bool hit = j < cnt && ((sMask[j] >> wave) & 1u);
+                const uint64_t bal = __ballot(hit);
+                if (hit) sList[wave][count + __popcll(bal & below)] = static_cast<uint8_t>(j);
+                count += __popcll(bal);
+            }
+            waveSync();
+        }
+        for (int i = 0; i < count; ++i) {
+            const int j = CULL ? sList[wave][i] : i;
+            const float4 a = s0[j];
+            const float4 b = s1[j];
+            if (!done) {
+                const float dx = px - a.x, dy = py - a.y;
+                const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
+                if (!(power > 0.0f)) {
+                    const float alpha = fminf(0.99f, b.y * __expf(power));
+                    if (alpha >= 1.0f / 255.0f) {
+                        const float nextT = T * (1.0f - alpha);
+                        if (nextT < 1e-4f) {
+                            done = true;
+                        } else {
+                            const float4 c = s2[j];
+                            const float w = alpha * T;
+                            cr = __builtin_fmaf(c.x, w, cr);
+                            cg = __builtin_fmaf(c.y, w, cg);
+                            cb = __builtin_fmaf(c.z, w, cb);
+                            T = nextT;
                         }
                     }
                 }
-                if ((j & 7) == 7 && __ballot(!done) == 0) break;
             }
+            if ((i & 7) == 7 && __ballot(!done) == 0) break;
         }
     }
     if (inside) {

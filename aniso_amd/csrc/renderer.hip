@@ -52,8 +52,9 @@ CamParams makeCamParams(const ans_camera& c, const ans_options& o, int sceneShDe
 }
 
 ans_options resolveOptions(const ans_options* o) {
-    ans_options r{3, 1.0f, 16, 16, 256};
+    ans_options r{3, 1.0f, 16, 16, 256, 0};
     if (o) {
+        r.flags = o->flags;
         r.sh_degree = o->sh_degree;
         r.splat_scale = o->splat_scale;
         if (o->tile_w) r.tile_w = o->tile_w;
@@ -111,16 +112,17 @@ void Engine::reservePairs(size_t cap) {
     pairCap = cap;
 }
 
-void Engine::reserveStatus(size_t n) {
-    const size_t partsDepth = (n + kSortTile - 1) / kSortTile;
-    const size_t partsDup = (n + kDupTile - 1) / kDupTile;
-    const size_t partsPair = (pairCap + kSortTile - 1) / kSortTile;
-    const size_t need = std::max({partsDepth * kRadix, partsDup, partsPair * kRadix, size_t(1)});
-    if (need > status.cap) {
-        // Fresh status words must not carry a tag a look-back could mistake for this frame's.
-        status.reserve(need);
-        hipCheck(hipMemsetAsync(status.p, 0, need * sizeof(uint64_t), stream), "hipMemsetAsync status");
-    }
+constexpr int kDepthItems = 4;   // 1024 depth keys per sort block
+constexpr int kPairItems = 16;   // 4096 pairs per sort block
+
+void Engine::reserveScratch(size_t n) {
+    const size_t blocksDepth = (n + kSortBlock * kDepthItems - 1) / (kSortBlock * kDepthItems);
+    const size_t blocksDup = (n + kDupTile - 1) / kDupTile;
+    const size_t blocksPair = (pairCap + kSortBlock * kPairItems - 1) / (kSortBlock * kPairItems);
+    hist.reserve(kRadix * std::max({blocksDepth, blocksPair, size_t(1)}));
+    rowTotal.reserve(kRadix);
+    blockPairs.reserve(std::max<size_t>(blocksDup, 1));
+    blockVisible.reserve(std::max<size_t>(blocksDup, 1));
 }
 
 // Enqueues one frame on the engine's stream. Outputs (either may be null) are device pointers.
@@ -130,56 +132,68 @@ void Engine::enqueueFrame(const SceneDev& s, const CamParams& cp, const ans_opti
     const uint32_t tiles = static_cast<uint32_t>(cp.tilesX) * static_cast<uint32_t>(cp.tilesY);
     reserveGaussians(n);
     ranges.reserve(tiles);
-    reserveStatus(n);
+    reserveScratch(n);
     last.tiles = tiles;
     last.n = n;
     last.cp = cp;
 
     hipCheck(hipEventRecord(ev[0], stream), "hipEventRecord");
-    const uint32_t initThreads = std::max<uint32_t>(tiles, (sizeof(FrameCtl) - 32) / 4);
-    k_initFrame<<<(initThreads + 255) / 256, 256, 0, stream>>>(ctl.p, ranges.p, tiles);
+    k_initFrame<<<(std::max<uint32_t>(tiles, 1) + 255) / 256, 256, 0, stream>>>(ctl.p, ranges.p, tiles);
     if (n > 0) {
-        k_preprocess<<<(n + 255) / 256, 256, 0, stream>>>(static_cast<int>(n), s.pos, s.scale, s.rot, s.opacity,
-                                                          s.sh, cp, rec0.p, rec1.p, rec2.p, rect.p,
-                                                          tilesTouched.p, depthKey.p, ctl.p);
-        const uint32_t histBlocks = std::min<uint32_t>(512, (n + 255) / 256);
-        k_histogram4<<<histBlocks, 256, 0, stream>>>(depthKey.p, n, &ctl.p->depthHist[0][0]);
-        // Depth sort: four stable 8-bit passes, ties keep ascending Gaussian index.
-        const uint32_t partsDepth = (n + kSortTile - 1) / kSortTile;
+        if (s.shPlanar)
+            k_preprocess<true><<<(n + 255) / 256, 256, 0, stream>>>(static_cast<int>(n), s.pos, s.scale, s.rot,
+                                                                    s.opacity, s.sh, cp, rec0.p, rec1.p, rec2.p,
+                                                                    rect.p, tilesTouched.p, depthKey.p, ctl.p);
+        else
+            k_preprocess<false><<<(n + 255) / 256, 256, 0, stream>>>(static_cast<int>(n), s.pos, s.scale, s.rot,
+                                                                     s.opacity, s.sh, cp, rec0.p, rec1.p, rec2.p,
+                                                                     rect.p, tilesTouched.p, depthKey.p, ctl.p);
+        // Depth sort: four stable 8-bit passes over all Gaussians (culled ones carry the largest
+        // key); ties keep ascending Gaussian index.
+        const uint32_t blocks = (n + kSortBlock * kDepthItems - 1) / (kSortBlock * kDepthItems);
         const uint32_t* kin = depthKey.p;
         const uint32_t* vin = nullptr;
         uint32_t* kout[4] = {sortKA.p, sortKB.p, sortKA.p, nullptr};
         uint32_t* vout[4] = {sortVA.p, sortVB.p, sortVA.p, sortVB.p};
         for (int pass = 0; pass < 4; ++pass) {
-            k_sortPass<<<partsDepth, kSortBlock, 0, stream>>>(kin, vin, kout[pass], vout[pass], nullptr, n, n,
-                                                              8 * pass, 8, &ctl.p->depthHist[pass][0],
-                                                              &ctl.p->partCounter[pass], status.p, ctl.p, pass);
+            k_radixUpsweep<kDepthItems><<<blocks, kSortBlock, 0, stream>>>(kin, nullptr, n, n, 8 * pass, 8, hist.p,
+                                                                           blocks, ctl.p);
+            k_scanRows<<<kRadix, kSortBlock, 0, stream>>>(hist.p, blocks, blocks, rowTotal.p, ctl.p);
+            k_radixDownsweep<kDepthItems><<<blocks, kSortBlock, 0, stream>>>(
+                kin, vin, kout[pass], vout[pass], nullptr, n, n, 8 * pass, 8, hist.p, rowTotal.p, blocks, ctl.p);
             kin = kout[pass];
             vin = vout[pass];
         }
     }
     hipCheck(hipEventRecord(ev[1], stream), "hipEventRecord");
+    // Pairs in depth order, then a stable sort by tile id.
     const int tileBits = std::max(1, bitsFor(tiles - 1));
     const int passes = (tileBits + 7) / 8;
     const int digitBits = (tileBits + passes - 1) / passes;
     last.sortedTile = tileA.p;
     last.sortedGauss = gaussA.p;
     if (n > 0) {
-        const uint32_t partsDup = (n + kDupTile - 1) / kDupTile;
-        k_expandPairs<<<partsDup, kDupBlock, 0, stream>>>(sortVB.p, tilesTouched.p, rect.p, n,
-                                                          static_cast<uint32_t>(pairCap), cp.tilesX, digitBits,
-                                                          passes, pairStart.p, tileA.p, gaussA.p,
-                                                          &ctl.p->partCounter[4], status.p, ctl.p, 4);
-        const uint32_t partsPair = static_cast<uint32_t>((pairCap + kSortTile - 1) / kSortTile);
+        const uint32_t blocksDup = (n + kDupTile - 1) / kDupTile;
+        const uint32_t cap = static_cast<uint32_t>(pairCap);
+        k_pairTotals<<<blocksDup, kDupBlock, 0, stream>>>(sortVB.p, tilesTouched.p, n, blockPairs.p,
+                                                          blockVisible.p);
+        k_scanPairs<<<1, kDupBlock, 0, stream>>>(blockPairs.p, blockVisible.p, blocksDup, cap, ctl.p);
+        k_expandPairs<<<blocksDup, kDupBlock, 0, stream>>>(sortVB.p, tilesTouched.p, rect.p, n, cp.tilesX,
+                                                           blockPairs.p, pairStart.p, tileA.p, gaussA.p, ctl.p);
+        const uint32_t blocksPair =
+            static_cast<uint32_t>((pairCap + kSortBlock * kPairItems - 1) / (kSortBlock * kPairItems));
         uint32_t* tk[2] = {tileA.p, tileB.p};
         uint32_t* tv[2] = {gaussA.p, gaussB.p};
         for (int pass = 0; pass < passes; ++pass) {
             const int src = pass & 1, dst = src ^ 1;
-            k_sortPass<<<partsPair, kSortBlock, 0, stream>>>(tk[src], tv[src], tk[dst], tv[dst], &ctl.p->pairs, 0,
-                                                             static_cast<uint32_t>(pairCap), pass * digitBits,
-                                                             digitBits, &ctl.p->pairHist[pass][0],
-                                                             &ctl.p->partCounter[5 + pass], status.p, ctl.p,
-                                                             5 + pass);
+            const int shift = pass * digitBits;
+            const int bits = std::min(digitBits, tileBits - shift);
+            k_radixUpsweep<kPairItems><<<blocksPair, kSortBlock, 0, stream>>>(tk[src], &ctl.p->pairs, 0, cap, shift,
+                                                                              bits, hist.p, blocksPair, ctl.p);
+            k_scanRows<<<1u << bits, kSortBlock, 0, stream>>>(hist.p, blocksPair, blocksPair, rowTotal.p, ctl.p);
+            k_radixDownsweep<kPairItems><<<blocksPair, kSortBlock, 0, stream>>>(
+                tk[src], tv[src], tk[dst], tv[dst], &ctl.p->pairs, 0, cap, shift, bits, hist.p, rowTotal.p,
+                blocksPair, ctl.p);
         }
         last.sortedTile = tk[passes & 1];
         last.sortedGauss = tv[passes & 1];
@@ -192,29 +206,31 @@ void Engine::enqueueFrame(const SceneDev& s, const CamParams& cp, const ans_opti
     hipCheck(hipEventRecord(ev[3], stream), "hipEventRecord");
 }
 
-template <int TW, int TH>
-void Engine::launchBlendT(const CamParams& cp, int batch, uint8_t* out8, float* outf) {
+template <int TW, int TH, int BATCH>
+void Engine::launchBlendB(const CamParams& cp, bool cull, uint8_t* out8, float* outf) {
     const uint32_t tiles = static_cast<uint32_t>(cp.tilesX) * cp.tilesY;
-    switch (batch) {
-        case 64:
-            k_blend<TW, TH, 64><<<tiles, TW * TH, 0, stream>>>(ranges.p, last.sortedGauss, rec0.p, rec1.p, rec2.p,
-                                                              cp.width, cp.height, cp.tilesX, out8, outf, ctl.p);
-            break;
-        case 128:
-            k_blend<TW, TH, 128><<<tiles, TW * TH, 0, stream>>>(ranges.p, last.sortedGauss, rec0.p, rec1.p, rec2.p,
-                                                               cp.width, cp.height, cp.tilesX, out8, outf, ctl.p);
-            break;
-        default:
-            k_blend<TW, TH, 256><<<tiles, TW * TH, 0, stream>>>(ranges.p, last.sortedGauss, rec0.p, rec1.p, rec2.p,
-                                                               cp.width, cp.height, cp.tilesX, out8, outf, ctl.p);
-            break;
-    }
+    if (cull)
+        k_blend<TW, TH, BATCH, true><<<tiles, TW * TH, 0, stream>>>(ranges.p, last.sortedGauss, rec0.p, rec1.p,
+                                                                     rec2.p, cp.width, cp.height, cp.tilesX, out8,
+                                                                     outf, ctl.p);
+    else
+        k_blend<TW, TH, BATCH, false><<<tiles, TW * TH, 0, stream>>>(ranges.p, last.sortedGauss, rec0.p, rec1.p,
+                                                                      rec2.p, cp.width, cp.height, cp.tilesX, out8,
+                                                                      outf, ctl.p);
+}
+
+template <int TW, int TH>
+void Engine::launchBlendT(const CamParams& cp, int batch, bool cull, uint8_t* out8, float* outf) {
+    if (batch == 64) launchBlendB<TW, TH, 64>(cp, cull, out8, outf);
+    else if (batch == 128) launchBlendB<TW, TH, 128>(cp, cull, out8, outf);
+    else launchBlendB<TW, TH, 256>(cp, cull, out8, outf);
 }
 
 void Engine::launchBlend(const CamParams& cp, const ans_options& o, uint8_t* out8, float* outf) {
-    if (o.tile_w == 8) launchBlendT<8, 8>(cp, o.batch_depth, out8, outf);
-    else if (o.tile_w == 32) launchBlendT<32, 16>(cp, o.batch_depth, out8, outf);
-    else launchBlendT<16, 16>(cp, o.batch_depth, out8, outf);
+    const bool cull = !(o.flags & ANS_FLAG_NO_CULL);
+    if (o.tile_w == 8) launchBlendT<8, 8>(cp, o.batch_depth, cull, out8, outf);
+    else if (o.tile_w == 32) launchBlendT<32, 16>(cp, o.batch_depth, cull, out8, outf);
+    else launchBlendT<16, 16>(cp, o.batch_depth, cull, out8, outf);
 }
 
 // Renders a frame and waits for it; grows the pair buffers and renders again if the pair count
@@ -335,8 +351,15 @@ int ans_renderer_create(int64_t n, int32_t sh_degree, const float* pos_xyz, cons
         upload(r->scale.p, scale_xyz, sizeof(float) * 3 * n);
         upload(r->rot.p, rot_wxyz, sizeof(float) * 4 * n);
         upload(r->opacity.p, opacity, sizeof(float) * n);
-        upload(r->sh.p, sh, sizeof(float) * coeffs * 3 * n);
-        r->scene = SceneDev{n, sh_degree, r->pos.p, r->scale.p, r->rot.p, r->opacity.p, r->sh.p};
+        {
+            // Planar SH on the device, [coefficient * 3 + channel][n], so the preprocess reads
+            // each coefficient with one coalesced load per wavefront.
+            std::vector<float> planar(coeffs * 3 * static_cast<size_t>(n));
+            for (int64_t i = 0; i < n; ++i)
+                for (size_t j = 0; j < coeffs * 3; ++j) planar[j * n + i] = sh[static_cast<size_t>(i) * coeffs * 3 + j];
+            upload(r->sh.p, planar.data(), sizeof(float) * planar.size());
+        }
+        r->scene = SceneDev{n, sh_degree, r->pos.p, r->scale.p, r->rot.p, r->opacity.p, r->sh.p, true};
         r->engine = std::make_unique<Engine>();
         *out = r.release();
         return ANS_OK;

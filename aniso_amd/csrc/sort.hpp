@@ -1,10 +1,11 @@
-// Hand-written stable LSD radix sort pass for gfx950: one kernel per digit, onesweep style.
-//
-// Each block claims a partition of kSortTile consecutive items through an atomic counter, ranks
-// its items by digit with wavefront ballots (stable: items keep their input order within a
-// digit), publishes its per-digit counts, finds the counts of all earlier partitions by
-// decoupled look-back, and scatters through LDS so that each digit's run is written to HBM
-// with consecutive addresses. The global digit histogram comes from an earlier kernel.
+// Hand-written stable LSD radix sort for gfx950, reduce-then-scan. Per digit pass:
+//   k_radixUpsweep    per-block digit histograms -> hist[bin][block] (one row per digit);
+//   k_scanRows        one workgroup per digit scans its row in place and writes the row total;
+//   k_radixDownsweep  re-reads its block, ranks items by digit with wavefront ballots (stable:
+//                     equal digits keep input order), adds the digit's global start (a block
+//                     scan of the row totals) and its row prefix, stages the items in LDS in
+//                     digit order and writes each digit's run to consecutive HBM addresses.
+// No block ever waits on another block, so there is no serial chain across the chip's XCDs.
 //
 // Keys and values are 32-bit. Replaces cub::DeviceRadixSort::SortPairs (gpu_render.cu:338-344)
 // in a design that sorts 32-bit depth keys per Gaussian and 32-bit tile ids per pair instead of
@@ -16,8 +17,6 @@
 namespace ans {
 
 constexpr int kSortBlock = 256;
-constexpr int kSortItems = 16;
-constexpr int kSortTile = kSortBlock * kSortItems;
 constexpr int kRadix = 256;
 constexpr int kSortWaves = kSortBlock / kWave;
 
@@ -69,58 +68,109 @@ __device__ inline uint64_t matchDigit(uint32_t digit, int bits, bool valid) {
     return peers;
 }
 
-// One stable sort pass over bits [shift, shift + bits) of the keys (bits <= 8).
-//   keysIn / valsIn: input (valsIn null: value = item index).
-//   keysOut: may be null when the keys are not needed after this pass.
-//   count: device item count (null: use nHost); items beyond cap are ignored.
-//   hist: global histogram of this digit over all items (256 bins).
-//   status: [parts][256] look-back words; tag = epoch * 16 + pass.
-__global__ __launch_bounds__(kSortBlock) void k_sortPass(const uint32_t* __restrict__ keysIn,
-                                                         const uint32_t* __restrict__ valsIn,
-                                                         uint32_t* __restrict__ keysOut,
-                                                         uint32_t* __restrict__ valsOut,
-                                                         const uint32_t* __restrict__ count, uint32_t nHost,
-                                                         uint32_t cap, int shift, int bits,
-                                                         const uint32_t* __restrict__ hist,
-                                                         uint32_t* __restrict__ partCounter,
-                                                         uint64_t* __restrict__ status,
-                                                         const FrameCtl* __restrict__ ctl, uint32_t pass) {
+__device__ inline uint32_t itemCount(const uint32_t* count, uint32_t nHost, uint32_t cap) {
+    return count ? min(*count, cap) : nHost;
+}
+
+// Per-block histogram of bits [shift, shift + bits) into hist[bin * stride + block]. Every block
+// of the grid writes its whole column (zeros past the item count).
+template <int ITEMS>
+__global__ __launch_bounds__(kSortBlock) void k_radixUpsweep(const uint32_t* __restrict__ keys,
+                                                             const uint32_t* __restrict__ count, uint32_t nHost,
+                                                             uint32_t cap, int shift, int bits,
+                                                             uint32_t* __restrict__ hist, uint32_t stride,
+                                                             const FrameCtl* __restrict__ ctl) {
+    __shared__ uint32_t h[kRadix];
+    if (ctl->overflow) return;
+    const uint32_t n = itemCount(count, nHost, cap);
+    const int tid = threadIdx.x;
+    const int radix = 1 << bits;
+    for (int i = tid; i < radix; i += kSortBlock) h[i] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * (kSortBlock * ITEMS);
+    const uint32_t mask = radix - 1;
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        const uint32_t idx = base + j * kSortBlock + tid;
+        if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & mask], 1u);
+    }
+    __syncthreads();
+    for (int i = tid; i < radix; i += kSortBlock) hist[static_cast<size_t>(i) * stride + blockIdx.x] = h[i];
+}
+
+// Exclusive scan of one row hist[bin * stride, + len) in place by one workgroup (blockIdx.x =
+// bin); the row total goes to rowTotal[bin].
+__global__ __launch_bounds__(kSortBlock) void k_scanRows(uint32_t* __restrict__ hist, uint32_t stride, uint32_t len,
+                                                         uint32_t* __restrict__ rowTotal,
+                                                         const FrameCtl* __restrict__ ctl) {
+    __shared__ uint32_t sTmp[kSortWaves];
+    if (ctl->overflow) return;
+    constexpr int kPer = 8;
+    uint32_t* row = hist + static_cast<size_t>(blockIdx.x) * stride;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < len; base += kSortBlock * kPer) {
+        const uint32_t first = base + threadIdx.x * kPer;
+        uint32_t v[kPer];
+        uint32_t sum = 0;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            v[k] = (first + k < len) ? row[first + k] : 0u;
+            sum += v[k];
+        }
+        uint32_t total;
+        uint32_t run = carry + blockExclusiveScan<kSortBlock>(sum, sTmp, &total);
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            if (first + k < len) row[first + k] = run;
+            run += v[k];
+        }
+        carry += total;
+    }
+    if (threadIdx.x == 0) rowTotal[blockIdx.x] = carry;
+}
+
+// One stable pass over bits [shift, shift + bits) (bits <= 8). keysOut may be null when the keys
+// are not needed after this pass; valsIn null means value = item index. hist holds the
+// row-scanned histogram of this pass and rowTotal the per-digit totals.
+template <int ITEMS>
+__global__ __launch_bounds__(kSortBlock) void k_radixDownsweep(
+    const uint32_t* __restrict__ keysIn, const uint32_t* __restrict__ valsIn, uint32_t* __restrict__ keysOut,
+    uint32_t* __restrict__ valsOut, const uint32_t* __restrict__ count, uint32_t nHost, uint32_t cap, int shift,
+    int bits, const uint32_t* __restrict__ hist, const uint32_t* __restrict__ rowTotal, uint32_t stride,
+    const FrameCtl* __restrict__ ctl) {
+    constexpr int kTile = kSortBlock * ITEMS;
     __shared__ uint32_t sWaveHist[kSortWaves][kRadix];
     __shared__ uint32_t sBinStart[kRadix];
     __shared__ uint32_t sDst[kRadix];
-    __shared__ uint32_t sKeys[kSortTile];
-    __shared__ uint32_t sVals[kSortTile];
+    __shared__ uint32_t sKeys[kTile];
+    __shared__ uint32_t sVals[kTile];
     __shared__ uint32_t sScan[kSortWaves];
-    __shared__ uint32_t sPart;
 
     if (ctl->overflow) return;
-    const uint32_t n = count ? min(*count, cap) : nHost;
+    const uint32_t n = itemCount(count, nHost, cap);
+    const uint32_t base = blockIdx.x * kTile;
+    if (base >= n) return;
     const int tid = threadIdx.x;
     const int lane = __lane_id();
     const int wave = tid / kWave;
-    if (tid == 0) sPart = atomicAdd(partCounter, 1u);
     for (int i = tid; i < kSortWaves * kRadix; i += kSortBlock) (&sWaveHist[0][0])[i] = 0;
     __syncthreads();
-    const uint32_t part = sPart;
-    const uint32_t numParts = (n + kSortTile - 1) / kSortTile;
-    if (part >= numParts) return;
-    const uint32_t base = part * kSortTile;
     const uint32_t mask = (1u << bits) - 1u;
 
-    // Load in wave-striped order: item j of lane l in wave w is base + (w * items + j) * 64 + l,
-    // so (wave, j, lane) order is input order.
-    uint32_t key[kSortItems], val[kSortItems], rank[kSortItems];
+    // Wave-striped order: item j of lane l in wave w is base + (w * ITEMS + j) * 64 + l, so
+    // (wave, j, lane) order is input order.
+    uint32_t key[ITEMS], val[ITEMS], rank[ITEMS];
 #pragma unroll
-    for (int j = 0; j < kSortItems; ++j) {
-        const uint32_t idx = base + (wave * kSortItems + j) * kWave + lane;
+    for (int j = 0; j < ITEMS; ++j) {
+        const uint32_t idx = base + (wave * ITEMS + j) * kWave + lane;
         const bool valid = idx < n;
         key[j] = valid ? keysIn[idx] : 0u;
         val[j] = valid ? (valsIn ? valsIn[idx] : idx) : 0u;
     }
     const uint64_t below = lanesBelow();
 #pragma unroll
-    for (int j = 0; j < kSortItems; ++j) {
-        const uint32_t idx = base + (wave * kSortItems + j) * kWave + lane;
+    for (int j = 0; j < ITEMS; ++j) {
+        const uint32_t idx = base + (wave * ITEMS + j) * kWave + lane;
         const bool valid = idx < n;
         const uint32_t digit = (key[j] >> shift) & mask;
         const uint64_t peers = matchDigit<8>(digit, bits, valid);
@@ -132,11 +182,11 @@ __global__ __launch_bounds__(kSortBlock) void k_sortPass(const uint32_t* __restr
             if (before == 0) sWaveHist[wave][digit] = old + static_cast<uint32_t>(__popcll(peers));
         }
         rank[j] = r;
+        waveSync();
     }
     __syncthreads();
 
-    // Per digit (one thread per bin): exclusive offsets across waves, block totals, block-local
-    // digit starts, then the look-back for the global start of this partition's run.
+    // One thread per digit: offsets across waves, block-local digit start, global destination.
     const int b = tid;
     uint32_t run = 0;
 #pragma unroll
@@ -145,28 +195,17 @@ __global__ __launch_bounds__(kSortBlock) void k_sortPass(const uint32_t* __restr
         sWaveHist[w][b] = run;
         run += c;
     }
-    uint32_t blockTotal;
+    uint32_t blockTotal, allTotal;
     const uint32_t binStart = blockExclusiveScan<kSortBlock>(run, sScan, &blockTotal);
-    uint32_t histTotal;
-    const uint32_t globalStart = blockExclusiveScan<kSortBlock>(hist[b], sScan, &histTotal);
+    const bool live = b <= static_cast<int>(mask);
+    const uint32_t digitStart = blockExclusiveScan<kSortBlock>(live ? rowTotal[b] : 0u, sScan, &allTotal);
     sBinStart[b] = binStart;
-    const uint32_t tag = ctl->epoch * 16u + pass;
-    uint64_t* st = status + static_cast<uint64_t>(part) * kRadix + b;
-    uint32_t earlier = 0;
-    if (part == 0) {
-        storeStatus(st, statusWord(tag, kFlagPrefix, run));
-    } else {
-        storeStatus(st, statusWord(tag, kFlagAggregate, run));
-        earlier = lookBack(status, part, kRadix, b, tag);
-        storeStatus(st, statusWord(tag, kFlagPrefix, earlier + run));
-    }
-    sDst[b] = globalStart + earlier - binStart;
+    if (live) sDst[b] = digitStart + hist[static_cast<size_t>(b) * stride + blockIdx.x] - binStart;
     __syncthreads();
 
-    // Stage in block-sorted order, then write each digit's run contiguously.
 #pragma unroll
-    for (int j = 0; j < kSortItems; ++j) {
-        const uint32_t idx = base + (wave * kSortItems + j) * kWave + lane;
+    for (int j = 0; j < ITEMS; ++j) {
+        const uint32_t idx = base + (wave * ITEMS + j) * kWave + lane;
         if (idx < n) {
             const uint32_t digit = (key[j] >> shift) & mask;
             const uint32_t pos = sBinStart[digit] + sWaveHist[wave][digit] + rank[j];
@@ -175,29 +214,13 @@ __global__ __launch_bounds__(kSortBlock) void k_sortPass(const uint32_t* __restr
         }
     }
     __syncthreads();
-    const uint32_t inPart = min(static_cast<uint32_t>(kSortTile), n - base);
-    for (uint32_t i = tid; i < inPart; i += kSortBlock) {
+    const uint32_t inBlock = min(static_cast<uint32_t>(kTile), n - base);
+    for (uint32_t i = tid; i < inBlock; i += kSortBlock) {
         const uint32_t k = sKeys[i];
         const uint32_t dst = sDst[(k >> shift) & mask] + i;
         if (keysOut) keysOut[dst] = k;
         valsOut[dst] = sVals[i];
     }
-}
-
-// Digit histograms of 32-bit keys, all four 8-bit digits at once, into hist[4][256].
-__global__ __launch_bounds__(256) void k_histogram4(const uint32_t* __restrict__ keys, uint32_t n,
-                                                    uint32_t* __restrict__ hist) {
-    __shared__ uint32_t h[4 * kRadix];
-    for (int i = threadIdx.x; i < 4 * kRadix; i += 256) h[i] = 0;
-    __syncthreads();
-    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-        const uint32_t k = keys[i];
-#pragma unroll
-        for (int d = 0; d < 4; ++d) atomicAdd(&h[d * kRadix + ((k >> (8 * d)) & 255u)], 1u);
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < 4 * kRadix; i += 256)
-        if (h[i]) atomicAdd(&hist[i], h[i]);
 }
 
 }  // namespace ans

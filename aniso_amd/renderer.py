@@ -27,10 +27,11 @@ class RenderOptions:
     splat_scale: float = 1.0
     tile: tuple = (16, 16)
     batch_depth: int = 256
+    cull: bool = True  # conservative per-8x8-block culling in the blend (output identical either way)
 
     def c(self) -> _lib.Options_:
         return _lib.Options_(int(self.sh_degree), float(self.splat_scale), int(self.tile[0]), int(self.tile[1]),
-                             int(self.batch_depth))
+                             int(self.batch_depth), 0 if self.cull else _lib.FLAG_NO_CULL)
 
 
 @dataclasses.dataclass

@@ -42,7 +42,12 @@ typedef struct ans_options {
     float splat_scale;   /* multiplies every standard deviation */
     int32_t tile_w, tile_h;
     int32_t batch_depth; /* splats staged in LDS per blend round: 64, 128 or 256 */
+    int32_t flags;       /* ANS_FLAG_* */
 } ans_options;
+
+/* Blend every staged splat for every 8x8 pixel block, without the conservative per-block
+ * culling. The output is identical; used to validate the culling. */
+#define ANS_FLAG_NO_CULL 1
 
 /* aniso::RenderStats (include/aniso/render.h:9-13) plus the pair-buffer bookkeeping. Times are
  * GPU time per stage from HIP events: project = preprocess + depth sort, sort = pair expansion

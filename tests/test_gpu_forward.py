@@ -137,3 +137,15 @@ def test_741883_gaussians_1280x720_full_size(gpu):
         ref = oracle.render(act, cam, oracle.REFERENCE, tile=(8, 8))
         print(f"  pairs {st.tile_pairs}; vs compiled-reference image: PSNR {psnr(img, ref):.1f} dB, max diff "
               f"{np.abs(img.astype(int) - ref.astype(int)).max()}/255", flush=True)
+
+
+@pytest.mark.parametrize("tile", [(8, 8), (16, 16), (32, 16)])
+def test_blend_culling_changes_nothing(gpu, tile):
+    """The blend's per-8x8-block culling is conservative: float images with and without it are
+    bit-identical."""
+    raw = scenes.synthetic_scene(200_000, 640, 360, seed=1)
+    cam = scenes.pose_camera([0.1, -0.1, 0.0], [0.1, 0.0, 1.0], [0, -1, 0], 640, 360, 60.0)
+    with GpuRenderer(raw) as r:
+        _, a = r.render_float(cam, None, RenderOptions(tile=tile))
+        _, b = r.render_float(cam, None, RenderOptions(tile=tile, cull=False))
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
