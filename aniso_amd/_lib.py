@@ -34,6 +34,17 @@ class Stats_(ctypes.Structure):
                 ("retries", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
+class Params_(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("sh_degree", ctypes.c_int32), ("means", ctypes.c_void_p),
+                ("log_scales", ctypes.c_void_p), ("quats", ctypes.c_void_p), ("opacity_logits", ctypes.c_void_p),
+                ("sh", ctypes.c_void_p)]
+
+
+class Grads_(ctypes.Structure):
+    _fields_ = [("means", ctypes.c_void_p), ("log_scales", ctypes.c_void_p), ("quats", ctypes.c_void_p),
+                ("opacity_logits", ctypes.c_void_p), ("sh", ctypes.c_void_p)]
+
+
 # Every symbol include/aniso_amd.h declares, with (restype, argtypes).
 SIGNATURES = {
     "ans_available": (ctypes.c_int, []),
@@ -50,6 +61,15 @@ SIGNATURES = {
     "ans_frame_counts": (ctypes.c_int, [P, P, P, P, P]),
     "ans_frame_dump": (ctypes.c_int, [P, P, P, P, P, P]),
     "ans_activate": (None, [ctypes.c_int64, P, P, P, P, P, P]),
+    "ans_diff_create": (ctypes.c_int, [ctypes.POINTER(P)]),
+    "ans_diff_destroy": (None, [P]),
+    "ans_diff_forward": (ctypes.c_int, [P, ctypes.POINTER(Params_), ctypes.POINTER(Camera_), ctypes.POINTER(Options_),
+                                        P, P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(Stats_)]),
+    "ans_diff_backward": (ctypes.c_int, [P, ctypes.c_int64, ctypes.POINTER(Params_), P, ctypes.POINTER(Grads_), P]),
+    "ans_diff_release": (ctypes.c_int, [P, ctypes.c_int64]),
+    "ans_diff_splat_grads": (ctypes.c_int, [P, ctypes.c_int64, P]),
+    "ans_diff_frame_counts": (ctypes.c_int, [P, ctypes.c_int64, P, P, P]),
+    "ans_diff_frame_dump": (ctypes.c_int, [P, ctypes.c_int64, P, P]),
 }
 
 _lib = None

@@ -1,0 +1,184 @@
+"""Differentiable rasterization for PyTorch: a torch.autograd.Function over the C ABI's
+ans_diff_* entry points (hand-written HIP forward and backward kernels).
+
+    image = rasterize(means, log_scales, quats, opacity_logits, sh, camera, options)
+    loss = (image - target).abs().mean(); loss.backward()
+
+Parameters are the raw values an optimizer holds (the 3DGS PLY layout, pipeline/train_gsplat.py:85-92):
+means [N,3], log_scales [N,3], quats [N,4] (w, x, y, z; normalized inside), opacity_logits [N],
+sh [N,(d+1)^2,3]. The image is the colour before the [0, 1] clamp, float32 [H, W, 3].
+This stands where the reference's training script calls gsplat's rasterization
+(pipeline/train_gsplat.py:96-102) with exp/sigmoid/normalize applied by the caller; here the
+activations are part of the differentiated function.
+"""
+from __future__ import annotations
+
+import ctypes
+import threading
+
+import numpy as np
+import torch
+
+from . import _lib
+from .renderer import RenderOptions, RenderStats, camera_c
+from .scenes import Camera
+
+_handles = {}
+_lock = threading.Lock()
+
+
+def _handle(device_index: int):
+    """One ans_diff handle per device (created on first use, with that device current)."""
+    with _lock:
+        h = _handles.get(device_index)
+        if h is None:
+            with torch.cuda.device(device_index):
+                hp = ctypes.c_void_p()
+                _lib.check(_lib.load().ans_diff_create(ctypes.byref(hp)))
+            h = _handles[device_index] = hp
+        return h
+
+
+def _params(means, log_scales, quats, opacity_logits, sh) -> _lib.Params_:
+    n = means.shape[0]
+    degree = int(round(sh.shape[1] ** 0.5)) - 1
+    return _lib.Params_(n, degree, means.data_ptr(), log_scales.data_ptr(), quats.data_ptr(),
+                        opacity_logits.data_ptr(), sh.data_ptr())
+
+
+def _check(t, name, shape):
+    if not t.is_cuda or t.dtype != torch.float32:
+        raise ValueError(f"{name} must be a float32 CUDA tensor")
+    if tuple(t.shape[1:]) != tuple(shape[1:]) or t.shape[0] != shape[0]:
+        raise ValueError(f"{name} has shape {tuple(t.shape)}, expected {shape}")
+
+
+class _Frame:
+    """A forward frame kept on the device for its backward pass; released when dropped."""
+
+    def __init__(self, handle, fid):
+        self.handle, self.fid = handle, fid
+
+    def release(self):
+        if self.fid is not None:
+            _lib.load().ans_diff_release(self.handle, self.fid)
+            self.fid = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class RasterizeFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, means, log_scales, quats, opacity_logits, sh, camera: Camera, options: RenderOptions,
+                stats: RenderStats | None):
+        n = means.shape[0]
+        k = sh.shape[1]
+        for t, name, shape in ((means, "means", (n, 3)), (log_scales, "log_scales", (n, 3)), (quats, "quats", (n, 4)),
+                               (opacity_logits, "opacity_logits", (n,)), (sh, "sh", (n, k, 3))):
+            _check(t, name, shape)
+        means, log_scales, quats, opacity_logits, sh = (t.contiguous() for t in
+                                                        (means, log_scales, quats, opacity_logits, sh))
+        dev = means.device.index if means.device.index is not None else torch.cuda.current_device()
+        handle = _handle(dev)
+        image = torch.empty((camera.height, camera.width, 3), dtype=torch.float32, device=means.device)
+        fid = ctypes.c_int64()
+        st = _lib.Stats_()
+        _lib.check(_lib.load().ans_diff_forward(handle, ctypes.byref(_params(means, log_scales, quats, opacity_logits,
+                                                                               sh)),
+                                                ctypes.byref(camera_c(camera)), ctypes.byref(options.c()),
+                                                image.data_ptr(), _stream(), ctypes.byref(fid), ctypes.byref(st)))
+        frame = _Frame(handle, fid.value)
+        if stats is not None:
+            for f in ("visible", "tile_pairs", "project_ms", "sort_ms", "blend_ms", "pair_capacity", "retries"):
+                setattr(stats, f, getattr(st, f))
+        # Inside forward, grad mode is off; needs_input_grad says whether a backward will come.
+        if any(ctx.needs_input_grad[:5]):
+            ctx.frame = frame
+            ctx.save_for_backward(means, log_scales, quats, opacity_logits, sh)
+        else:
+            frame.release()
+        return image
+
+    @staticmethod
+    def backward(ctx, grad_image):
+        means, log_scales, quats, opacity_logits, sh = ctx.saved_tensors
+        grads = [torch.empty_like(t) for t in (means, log_scales, quats, opacity_logits, sh)]
+        g = grad_image.contiguous().float()
+        frame = ctx.frame
+        _lib.check(_lib.load().ans_diff_backward(frame.handle, frame.fid,
+                                                 ctypes.byref(_params(means, log_scales, quats, opacity_logits, sh)),
+                                                 g.data_ptr(), ctypes.byref(_lib.Grads_(*[t.data_ptr() for t in grads])),
+                                                 _stream()))
+        # Releasing on the host is safe: the device buffers go back to a pool that is only reused
+        # by later work on the same stream.
+        frame.release()
+        return (*grads, None, None, None)
+
+
+def rasterize(means, log_scales, quats, opacity_logits, sh, camera: Camera, options: RenderOptions | None = None,
+              stats: RenderStats | None = None):
+    """Differentiable render: unclamped colour image [H, W, 3] (float32)."""
+    return RasterizeFunction.apply(means, log_scales, quats, opacity_logits, sh, camera, options or RenderOptions(),
+                                   stats)
+
+
+def frame_lists(means, log_scales, quats, opacity_logits, sh, camera: Camera, options: RenderOptions | None = None):
+    """Per-tile lists (start [T+1], concatenated lists [P]) of a forward frame, for evaluating a
+    reference gradient on exactly the GPU's lists."""
+    dev = means.device.index if means.device.index is not None else torch.cuda.current_device()
+    handle = _handle(dev)
+    L = _lib.load()
+    image = torch.empty((camera.height, camera.width, 3), dtype=torch.float32, device=means.device)
+    fid = ctypes.c_int64()
+    _lib.check(L.ans_diff_forward(handle, ctypes.byref(_params(means, log_scales, quats, opacity_logits, sh)),
+                                  ctypes.byref(camera_c(camera)), ctypes.byref((options or RenderOptions()).c()),
+                                  image.data_ptr(), _stream(), ctypes.byref(fid), None))
+    try:
+        vis, pairs, tiles = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _lib.check(L.ans_diff_frame_counts(handle, fid.value, ctypes.byref(vis), ctypes.byref(pairs),
+                                           ctypes.byref(tiles)))
+        values = np.empty(pairs.value, np.uint32)
+        ranges = np.empty((tiles.value, 2), np.uint32)
+        _lib.check(L.ans_diff_frame_dump(handle, fid.value, values.ctypes.data_as(ctypes.c_void_p),
+                                         ranges.ctypes.data_as(ctypes.c_void_p)))
+    finally:
+        L.ans_diff_release(handle, fid.value)
+    # Ranges of empty tiles are [0, 0); rebuild starts from the lengths in tile order.
+    lengths = (ranges[:, 1] - ranges[:, 0]).astype(np.int64)
+    start = np.zeros(len(ranges) + 1, np.uint32)
+    start[1:] = np.cumsum(lengths)
+    return start, values, image
+
+
+def forward_backward(means, log_scales, quats, opacity_logits, sh, camera: Camera, grad_image,
+                     options: RenderOptions | None = None):
+    """One forward and backward without autograd: returns (image, [grads of the five parameter
+    tensors], splat gradients [n, 9] = dL/d{u, v, conic A, B, C, opacity, r, g, b} as a host
+    array; u, v are the screen-space mean gradients densification heuristics use)."""
+    dev = means.device.index if means.device.index is not None else torch.cuda.current_device()
+    handle = _handle(dev)
+    L = _lib.load()
+    params = _params(means, log_scales, quats, opacity_logits, sh)
+    image = torch.empty((camera.height, camera.width, 3), dtype=torch.float32, device=means.device)
+    fid = ctypes.c_int64()
+    _lib.check(L.ans_diff_forward(handle, ctypes.byref(params), ctypes.byref(camera_c(camera)),
+                                  ctypes.byref((options or RenderOptions()).c()), image.data_ptr(), _stream(),
+                                  ctypes.byref(fid), None))
+    try:
+        grads = [torch.empty_like(t) for t in (means, log_scales, quats, opacity_logits, sh)]
+        g = grad_image.contiguous().float()
+        _lib.check(L.ans_diff_backward(handle, fid.value, ctypes.byref(params), g.data_ptr(),
+                                       ctypes.byref(_lib.Grads_(*[t.data_ptr() for t in grads])), _stream()))
+        splat = np.empty((means.shape[0], 9), np.float32)
+        _lib.check(L.ans_diff_splat_grads(handle, fid.value, splat.ctypes.data_as(ctypes.c_void_p)))
+    finally:
+        L.ans_diff_release(handle, fid.value)
+    return image, grads, splat

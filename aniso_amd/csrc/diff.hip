@@ -1,0 +1,218 @@
+// Differentiable path: forward frames kept for their backward pass, and the backward launch
+// sequence (blend adjoint, then per-Gaussian reduction and projection adjoint). C ABI in
+// include/aniso_amd.h.
+#include "aniso_amd.h"
+#include "backward.hpp"
+#include "engine.hpp"
+
+#include <map>
+#include <memory>
+#include <vector>
+
+namespace ans {
+
+void dumpFrame(hipStream_t stream, const FrameState& f, uint32_t* splat_bits, int32_t* rect, uint64_t* keys,
+               uint32_t* values, uint32_t* ranges);
+
+template <int TW, int TH>
+static void launchBlendBackward(Engine& e, FrameState& f, const float* gradImage) {
+    constexpr int kBatch = 128;
+    const CamParams& cp = f.cp;
+    const uint32_t tiles = static_cast<uint32_t>(cp.tilesX) * cp.tilesY;
+    k_blendBackward<TW, TH, kBatch><<<tiles, TW * TH, 0, e.stream>>>(
+        f.ranges.p, f.sortedGauss, f.rec0.p, f.rec1.p, f.rec2.p, f.rect.p, f.pairStart.p, cp.width, cp.height,
+        cp.tilesX, gradImage, e.partial.p, f.ctl.p);
+}
+
+void Engine::enqueueBackward(FrameState& f, const SceneDev& s, const float* gradImage, float* gMeans,
+                             float* gScales, float* gQuats, float* gOpacity, float* gSh) {
+    partial.reserve(std::max<size_t>(f.pairs, 1) * 9);
+    if (f.n > 0) {
+        if (f.opts.tile_w == 8) launchBlendBackward<8, 8>(*this, f, gradImage);
+        else if (f.opts.tile_w == 32) launchBlendBackward<32, 16>(*this, f, gradImage);
+        else launchBlendBackward<16, 16>(*this, f, gradImage);
+        k_gaussBackward<<<static_cast<uint32_t>((f.n + 255) / 256), 256, 0, stream>>>(
+            static_cast<int>(f.n), s.pos, s.scale, s.rot, s.opacity, s.sh, f.cp, f.tilesTouched.p, f.pairStart.p,
+            partial.p, gMeans, gScales, gQuats, gOpacity, gSh, f.ctl.p);
+    }
+    hipCheck(hipGetLastError(), "kernel launch");
+}
+
+}  // namespace ans
+
+using namespace ans;
+
+struct ans_diff {
+    std::unique_ptr<Engine> engine;
+    std::map<int64_t, std::unique_ptr<FrameState>> frames;
+    std::vector<std::unique_ptr<FrameState>> pool;
+    int64_t next = 1;
+};
+
+namespace {
+
+template <class Fn>
+int guardedDiff(Fn&& fn) {
+    try {
+        g_lastError.clear();
+        return fn();
+    } catch (const std::invalid_argument& e) {
+        g_lastError = e.what();
+        return ANS_ERR_ARGUMENT;
+    } catch (const std::exception& e) {
+        g_lastError = e.what();
+        return ANS_ERR_HIP;
+    } catch (...) {
+        g_lastError = "unknown error";
+        return ANS_ERR_HIP;
+    }
+}
+
+SceneDev sceneOf(const ans_params* p) {
+    if (!p) throw std::invalid_argument("params is null");
+    if (p->n < 0 || p->n > (int64_t(1) << 30)) throw std::invalid_argument("Gaussian count must be in 0..2^30");
+    if (p->sh_degree < 0 || p->sh_degree > 3) throw std::invalid_argument("sh_degree must be in 0..3");
+    if (p->n > 0 && (!p->means || !p->log_scales || !p->quats || !p->opacity_logits || !p->sh))
+        throw std::invalid_argument("parameter pointers must not be null");
+    if (reinterpret_cast<uintptr_t>(p->quats) % 16 != 0 || reinterpret_cast<uintptr_t>(p->sh) % 16 != 0)
+        throw std::invalid_argument("quats and sh must be 16-byte aligned");
+    SceneDev s;
+    s.n = p->n;
+    s.shDegree = p->sh_degree;
+    s.pos = p->means;
+    s.scale = p->log_scales;
+    s.rot = reinterpret_cast<const float4*>(p->quats);
+    s.opacity = p->opacity_logits;
+    s.sh = p->sh;
+    s.shPlanar = false;
+    s.raw = true;
+    return s;
+}
+
+FrameState& frameOf(ans_diff* d, int64_t id) {
+    if (!d) throw std::invalid_argument("handle is null");
+    auto it = d->frames.find(id);
+    if (it == d->frames.end()) throw std::invalid_argument("unknown or released frame");
+    return *it->second;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ans_diff_create(ans_diff** out) {
+    return guardedDiff([&] {
+        if (!out) throw std::invalid_argument("out is null");
+        *out = nullptr;
+        if (!ans_available()) throw std::runtime_error("no HIP device");
+        auto d = std::make_unique<ans_diff>();
+        d->engine = std::make_unique<Engine>();
+        *out = d.release();
+        return ANS_OK;
+    });
+}
+
+void ans_diff_destroy(ans_diff* d) {
+    if (!d) return;
+    (void)hipStreamSynchronize(d->engine->stream);
+    delete d;
+}
+
+int ans_diff_forward(ans_diff* d, const ans_params* params, const ans_camera* cam, const ans_options* options,
+                     float* image, void* stream, int64_t* frame, ans_stats* stats) {
+    return guardedDiff([&] {
+        if (!d) throw std::invalid_argument("handle is null");
+        checkCamera(cam);
+        const SceneDev s = sceneOf(params);
+        const ans_options o = resolveOptions(options);
+        const CamParams cp = makeCamParams(*cam, o, s.shDegree);
+        if (cp.tilesX > 65535 || cp.tilesY > 65535) throw std::invalid_argument("too many tiles");
+        Engine& e = *d->engine;
+        e.stream = static_cast<hipStream_t>(stream);  // NULL: the default stream, like any HIP call
+        std::unique_ptr<FrameState> f;
+        if (!d->pool.empty()) {
+            f = std::move(d->pool.back());
+            d->pool.pop_back();
+        } else {
+            f = std::make_unique<FrameState>();
+        }
+        e.renderSync(*f, s, cp, o, nullptr, image);
+        e.fillStats(*f, stats);
+        const int64_t id = d->next++;
+        d->frames[id] = std::move(f);
+        if (frame) *frame = id;
+        return ANS_OK;
+    });
+}
+
+int ans_diff_backward(ans_diff* d, int64_t frame, const ans_params* params, const float* grad_image,
+                      const ans_grads* grads, void* stream) {
+    return guardedDiff([&] {
+        FrameState& f = frameOf(d, frame);
+        const SceneDev s = sceneOf(params);
+        if (static_cast<size_t>(s.n) != f.n) throw std::invalid_argument("parameter count differs from the forward");
+        if (!grad_image || !grads || (s.n > 0 && (!grads->means || !grads->log_scales || !grads->quats ||
+                                                  !grads->opacity_logits || !grads->sh)))
+            throw std::invalid_argument("gradient pointers must not be null");
+        Engine& e = *d->engine;
+        e.stream = static_cast<hipStream_t>(stream);  // NULL: the default stream, like any HIP call
+        e.enqueueBackward(f, s, grad_image, grads->means, grads->log_scales, grads->quats, grads->opacity_logits,
+                          grads->sh);
+        return ANS_OK;
+    });
+}
+
+int ans_diff_release(ans_diff* d, int64_t frame) {
+    return guardedDiff([&] {
+        if (!d) throw std::invalid_argument("handle is null");
+        auto it = d->frames.find(frame);
+        if (it == d->frames.end()) throw std::invalid_argument("unknown or released frame");
+        d->pool.push_back(std::move(it->second));
+        d->frames.erase(it);
+        return ANS_OK;
+    });
+}
+
+int ans_diff_splat_grads(ans_diff* d, int64_t frame, float* out) {
+    return guardedDiff([&] {
+        const FrameState& f = frameOf(d, frame);
+        if (!out) throw std::invalid_argument("out is null");
+        Engine& e = *d->engine;
+        hipCheck(hipStreamSynchronize(e.stream), "hipStreamSynchronize");
+        std::vector<uint32_t> tt(f.n), ps(f.n);
+        std::vector<float> part(static_cast<size_t>(f.pairs) * 9);
+        if (f.n) {
+            hipCheck(hipMemcpy(tt.data(), f.tilesTouched.p, 4 * f.n, hipMemcpyDeviceToHost), "hipMemcpy");
+            hipCheck(hipMemcpy(ps.data(), f.pairStart.p, 4 * f.n, hipMemcpyDeviceToHost), "hipMemcpy");
+        }
+        if (!part.empty())
+            hipCheck(hipMemcpy(part.data(), e.partial.p, part.size() * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+        for (size_t i = 0; i < f.n; ++i)
+            for (int c = 0; c < 9; ++c) {
+                float s = 0.0f;
+                for (uint32_t k = 0; k < tt[i]; ++k) s += part[(static_cast<size_t>(ps[i]) + k) * 9 + c];
+                out[9 * i + c] = s;
+            }
+        return ANS_OK;
+    });
+}
+
+int ans_diff_frame_counts(ans_diff* d, int64_t frame, int64_t* visible, int64_t* pairs, int64_t* tiles) {
+    return guardedDiff([&] {
+        const FrameState& f = frameOf(d, frame);
+        if (visible) *visible = f.visible;
+        if (pairs) *pairs = f.pairs;
+        if (tiles) *tiles = static_cast<int64_t>(f.tiles);
+        return ANS_OK;
+    });
+}
+
+int ans_diff_frame_dump(ans_diff* d, int64_t frame, uint32_t* values, uint32_t* ranges) {
+    return guardedDiff([&] {
+        const FrameState& f = frameOf(d, frame);
+        dumpFrame(d->engine->stream, f, nullptr, nullptr, nullptr, values, ranges);
+        return ANS_OK;
+    });
+}
+
+}  // extern "C"

@@ -35,70 +35,84 @@ struct DevBuf {
     }
 };
 
-// Activated scene in device memory (layout of gpu_render.cu:243-248: pos f3, scale f3,
-// rot f4 (w first), opacity f, sh [n][coeffs][3]).
+// A scene in device memory. Activated (render API; layout of gpu_render.cu:243-248 with planar SH)
+// or raw (the differentiable path: log-scales, opacity logits, unnormalized quaternions, SH
+// [n][coefficient][channel], as a training loop holds them).
 struct SceneDev {
     int64_t n = 0;
     int shDegree = 0;
     const float* pos = nullptr;
-    const float* scale = nullptr;
-    const float4* rot = nullptr;
-    const float* opacity = nullptr;
+    const float* scale = nullptr;    // activated: standard deviations; raw: log-scales
+    const float4* rot = nullptr;     // activated: unit (w, x, y, z); raw: unnormalized
+    const float* opacity = nullptr;  // activated: opacity; raw: logit
     const float* sh = nullptr;
     bool shPlanar = false;  // [coefficient * 3 + channel][n] instead of [n][coefficient][channel]
+    bool raw = false;
 };
 
 CamParams makeCamParams(const ans_camera& c, const ans_options& o, int sceneShDegree);
 ans_options resolveOptions(const ans_options* o);
 void checkCamera(const ans_camera* cam);
 
-struct LastFrame {
+// Everything one frame produces that its blend and its backward pass read.
+struct FrameState {
+    DevBuf<FrameCtl> ctl;
+    DevBuf<float4> rec0, rec1, rec2;
+    DevBuf<uint2> rect;
+    DevBuf<uint32_t> tilesTouched, pairStart;
+    DevBuf<uint32_t> tileA, gaussA, tileB, gaussB;
+    size_t pairCap = 0;
+    DevBuf<uint2> ranges;
+
     size_t n = 0, tiles = 0;
     uint32_t visible = 0, pairs = 0;
     int retries = 0;
     CamParams cp{};
+    ans_options opts{};
     const uint32_t* sortedTile = nullptr;
     const uint32_t* sortedGauss = nullptr;
+
+    FrameState();
+    void reserve(size_t gaussians, size_t tileCount);
+    void reservePairs(size_t cap);
 };
 
-// Per-stream working set: every per-Gaussian, per-pair and per-tile buffer of a frame.
+// Per-stream working set: scratch shared by all frames plus the render API's own frame.
 struct Engine {
     int device = 0;
     hipStream_t ownStream = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {};
+    FrameCtl* ctlHost = nullptr;  // pinned
 
-    DevBuf<FrameCtl> ctl;
-    FrameCtl* ctlHost = nullptr;
-    DevBuf<float4> rec0, rec1, rec2;
-    DevBuf<uint2> rect;
-    DevBuf<uint32_t> tilesTouched, depthKey, pairStart;
-    DevBuf<uint32_t> sortKA, sortVA, sortKB, sortVB;
-    DevBuf<uint32_t> tileA, gaussA, tileB, gaussB;
-    size_t pairCap = 0;
-    DevBuf<uint2> ranges;
-    DevBuf<uint32_t> hist;        // per-block digit histograms, scanned in place
-    DevBuf<uint32_t> rowTotal;    // per-digit totals of one pass
-    DevBuf<uint32_t> blockPairs;  // pairs per expansion block, scanned in place
+    DevBuf<uint32_t> depthKey, sortKA, sortVA, sortKB, sortVB;
+    DevBuf<uint32_t> hist;          // per-block digit histograms, scanned in place
+    DevBuf<uint32_t> rowTotal;      // per-digit totals of one pass
+    DevBuf<uint32_t> blockPairs;    // pairs per expansion block, scanned in place
     DevBuf<uint32_t> blockVisible;  // visible Gaussians per expansion block
-    LastFrame last;
+    DevBuf<float> partial;          // backward: per-pair partial gradients, 9 floats per pair
+    FrameState frame;
 
     Engine();
     ~Engine();
     Engine(const Engine&) = delete;
     Engine& operator=(const Engine&) = delete;
 
-    void reserveGaussians(size_t n);
-    void reservePairs(size_t cap);
-    void reserveScratch(size_t n);
-    void enqueueFrame(const SceneDev& s, const CamParams& cp, const ans_options& o, uint8_t* out8, float* outf);
-    void launchBlend(const CamParams& cp, const ans_options& o, uint8_t* out8, float* outf);
+    void reserveScratch(size_t n, size_t pairCap);
+    void enqueueFrame(FrameState& f, const SceneDev& s, const CamParams& cp, const ans_options& o, uint8_t* out8,
+                      float* outf);
+    void launchBlend(FrameState& f, uint8_t* out8, float* outf);
     template <int TW, int TH>
-    void launchBlendT(const CamParams& cp, int batch, bool cull, uint8_t* out8, float* outf);
+    void launchBlendT(FrameState& f, bool cull, uint8_t* out8, float* outf);
     template <int TW, int TH, int BATCH>
-    void launchBlendB(const CamParams& cp, bool cull, uint8_t* out8, float* outf);
-    int renderSync(const SceneDev& s, const CamParams& cp, const ans_options& o, uint8_t* out8, float* outf);
-    void fillStats(ans_stats* st);
+    void launchBlendB(FrameState& f, bool cull, uint8_t* out8, float* outf);
+    // Copies the frame's control block to the host and waits; true if the pair buffers overflowed.
+    bool readBack(FrameState& f);
+    int renderSync(FrameState& f, const SceneDev& s, const CamParams& cp, const ans_options& o, uint8_t* out8,
+                   float* outf);
+    void fillStats(const FrameState& f, ans_stats* st);
+    void enqueueBackward(FrameState& f, const SceneDev& s, const float* gradImage, float* gMeans, float* gScales,
+                         float* gQuats, float* gOpacity, float* gSh);
 };
 
 }  // namespace ans

@@ -10,7 +10,7 @@
 namespace ans {
 
 // Clears the per-frame control block and the tile ranges, and bumps the epoch.
-__global__ __launch_bounds__(256) void k_initFrame(FrameCtl* __restrict__ ctl, uint2* __restrict__ ranges,
+static __global__ __launch_bounds__(256) void k_initFrame(FrameCtl* __restrict__ ctl, uint2* __restrict__ ranges,
                                                    uint32_t tiles) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i == 0) {
@@ -27,18 +27,15 @@ struct F3 {
 };
 
 // Real SH constants (src/sh.cpp:10-14; closed forms rounded to float).
-__constant__ float kShC1 = 0.4886025119029199f;
-__constant__ float kShC2[5] = {1.0925484305920792f, -1.0925484305920792f, 0.31539156525252005f, -1.0925484305920792f,
+constexpr float kShC1 = 0.4886025119029199f;
+constexpr float kShC2[5] = {1.0925484305920792f, -1.0925484305920792f, 0.31539156525252005f, -1.0925484305920792f,
                                0.5462742152960396f};
-__constant__ float kShC3[7] = {-0.5900435899266435f, 2.890611442640554f, -0.4570457994644658f, 0.3731763325901154f,
+constexpr float kShC3[7] = {-0.5900435899266435f, 2.890611442640554f, -0.4570457994644658f, 0.3731763325901154f,
                                -0.4570457994644658f, 1.445305721320277f, -0.5900435899266435f};
 
-// SH colour along a unit direction (src/sh.cpp:18-58, same basis order and operation order).
-// Coefficient j = k * 3 + channel sits at sh[j * stride]: stride 1 for the [coefficient][channel]
-// layout of one Gaussian, stride n for the planar device layout [coefficient * 3 + channel][n].
-template <bool PLANAR>
-__device__ inline float3 shColor(const float* __restrict__ sh, size_t stride, int degree, float x, float y, float z) {
-    float b[16];
+// Basis values of the unit direction (x, y, z) up to `degree`, in the order the coefficients are
+// stored (src/sh.cpp:23-51, same operation order); returns the coefficient count.
+__host__ __device__ inline int shBasis(int degree, float x, float y, float z, float b[16]) {
     int count = 1;
     b[0] = 0.28209479177387814f;
     if (degree >= 1) {
@@ -66,6 +63,14 @@ __device__ inline float3 shColor(const float* __restrict__ sh, size_t stride, in
             count = 16;
         }
     }
+    return count;
+}
+
+// 0.5 + sum_k b_k sh_k per channel (before the clamp at zero; src/sh.cpp:53-55). Coefficient
+// j = k * 3 + channel sits at sh[j * stride]: stride 1 for one Gaussian's [coefficient][channel]
+// block, stride n for the planar device layout [coefficient * 3 + channel][n].
+template <bool PLANAR>
+__host__ __device__ inline float3 shAccumulate(const float* __restrict__ sh, size_t stride, int count, const float b[16]) {
     float3 c = make_float3(0.5f, 0.5f, 0.5f);
     if (count == 16) {
         float v[48];
@@ -96,131 +101,180 @@ __device__ inline float3 shColor(const float* __restrict__ sh, size_t stride, in
             }
         }
     }
-    // std::max(0, v) semantics (NaN -> 0).
-    c.x = (0.0f < c.x) ? c.x : 0.0f;
-    c.y = (0.0f < c.y) ? c.y : 0.0f;
-    c.z = (0.0f < c.z) ? c.z : 0.0f;
     return c;
 }
 
-__device__ inline float clampf(float v, float lo, float hi) { return (v < lo) ? lo : (hi < v) ? hi : v; }
+__host__ __device__ inline float clampf(float v, float lo, float hi) { return (v < lo) ? lo : (hi < v) ? hi : v; }
 
-// Per-Gaussian preprocess, one lane per Gaussian (the maths of src/render.cpp:39-104 with the
-// FP32 camera transform of src/gpu_render.cu:94-96):
-//   world -> camera; near cull; Sigma = M M^T, M = R(q) diag(s); EWA 2D covariance with the
-//   clamped Jacobian and the 0.3 low-pass; conic; radius = ceil(3 sqrt(lambda_max)); centre in
-//   pixels; screen cull; tile rect; SH colour.
-// Writes the splat record {u, v, A, B | C, opacity, ext_x, ext_y | r, g, b, depth}, the packed
-// tile rect, the tile count and the depth sort key (0xffffffff for culled Gaussians). ext_x/y are
-// conservative half-extents of the region where alpha can reach 1/255, used by the blend to skip
-// splats that cannot touch a wavefront's pixels (they do not affect any output).
-// SH is read planar ([coefficient * 3 + channel][n], coalesced) when PLANAR, else per Gaussian.
-template <bool PLANAR>
+// One projected Gaussian and the intermediates its adjoint reuses.
+struct GaussProj {
+    float u, v, A, B, C, opacity, depth;
+    float3 rgb, rgbRaw;
+    int x0, y0, x1, y1;
+    float px, py, pz, tx, ty;
+    bool clampX, clampY;
+    float s[3], q[4], qlen;
+    float Rq[3][3], M[3][3], Sigma[3][3], T[2][3];
+    float a, b, c, det;
+    float dir[3], dlen;
+    float basis[16];
+    int shCount;
+};
+
+// Projects Gaussian i (the maths of src/render.cpp:39-104 with the FP32 camera transform of
+// src/gpu_render.cu:94-96): world -> camera; near cull; Sigma = M M^T, M = R(q) diag(s); EWA 2D
+// covariance with the clamped Jacobian and the 0.3 low-pass; conic; radius = ceil(3 sqrt(lambda));
+// centre in pixels; screen cull; tile rect; SH colour. RAW: the parameters are the optimizer's
+// (scale = exp(log-scale), opacity = sigmoid(logit), rotation = q / |q|, src/ply_loader.cpp:124-130),
+// activated here in FP32. Deliberate deviations shared with the CPU oracle: the near test is
+// !(z >= 0.2) so a NaN depth is culled, and a non-finite centre or a radius of 2^24 pixels or more
+// is culled. Returns false if culled.
+template <bool PLANAR, bool RAW>
+__host__ __device__ inline bool projectGaussian(int i, int n, const float* __restrict__ pos, const float* __restrict__ scale,
+                                       const float4* __restrict__ rot, const float* __restrict__ opacity,
+                                       const float* __restrict__ sh, const CamParams& cam, GaussProj& g) {
+    const F3 m = reinterpret_cast<const F3*>(pos)[i];
+    g.px = cam.R[0] * m.x + cam.R[1] * m.y + cam.R[2] * m.z + cam.t[0];
+    g.py = cam.R[3] * m.x + cam.R[4] * m.y + cam.R[5] * m.z + cam.t[1];
+    g.pz = cam.R[6] * m.x + cam.R[7] * m.y + cam.R[8] * m.z + cam.t[2];
+    const float px = g.px, py = g.py, pz = g.pz;
+    if (!(pz >= 0.2f)) return false;
+
+    const float4 q4 = rot[i];  // w, x, y, z
+    if constexpr (RAW) {
+        g.qlen = sqrtf(q4.x * q4.x + q4.y * q4.y + q4.z * q4.z + q4.w * q4.w);
+        if (g.qlen > 0.0f) {
+            g.q[0] = q4.x / g.qlen; g.q[1] = q4.y / g.qlen; g.q[2] = q4.z / g.qlen; g.q[3] = q4.w / g.qlen;
+        } else {
+            g.q[0] = 1.0f; g.q[1] = g.q[2] = g.q[3] = 0.0f;
+        }
+    } else {
+        g.qlen = 1.0f;
+        g.q[0] = q4.x; g.q[1] = q4.y; g.q[2] = q4.z; g.q[3] = q4.w;
+    }
+    const float w = g.q[0], x = g.q[1], y = g.q[2], z = g.q[3];
+    g.Rq[0][0] = 1 - 2 * (y * y + z * z); g.Rq[0][1] = 2 * (x * y - w * z);     g.Rq[0][2] = 2 * (x * z + w * y);
+    g.Rq[1][0] = 2 * (x * y + w * z);     g.Rq[1][1] = 1 - 2 * (x * x + z * z); g.Rq[1][2] = 2 * (y * z - w * x);
+    g.Rq[2][0] = 2 * (x * z - w * y);     g.Rq[2][1] = 2 * (y * z + w * x);     g.Rq[2][2] = 1 - 2 * (x * x + y * y);
+    const F3 sc = reinterpret_cast<const F3*>(scale)[i];
+    if constexpr (RAW) {
+        g.s[0] = expf(sc.x) * cam.splatScale; g.s[1] = expf(sc.y) * cam.splatScale; g.s[2] = expf(sc.z) * cam.splatScale;
+    } else {
+        g.s[0] = sc.x * cam.splatScale; g.s[1] = sc.y * cam.splatScale; g.s[2] = sc.z * cam.splatScale;
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) g.M[r][c] = g.Rq[r][c] * g.s[c];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) g.Sigma[r][c] = g.M[r][0] * g.M[c][0] + g.M[r][1] * g.M[c][1] + g.M[r][2] * g.M[c][2];
+
+    const float rx = px / pz, ry = py / pz;
+    g.clampX = rx < -cam.limX || cam.limX < rx;
+    g.clampY = ry < -cam.limY || cam.limY < ry;
+    g.tx = clampf(rx, -cam.limX, cam.limX) * pz;
+    g.ty = clampf(ry, -cam.limY, cam.limY) * pz;
+    const float J[2][3] = {{cam.fx / pz, 0, -cam.fx * g.tx / (pz * pz)}, {0, cam.fy / pz, -cam.fy * g.ty / (pz * pz)}};
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) g.T[r][c] = J[r][0] * cam.R[c] + J[r][1] * cam.R[3 + c] + J[r][2] * cam.R[6 + c];
+    float TS[2][3];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) TS[r][c] = g.T[r][0] * g.Sigma[0][c] + g.T[r][1] * g.Sigma[1][c] + g.T[r][2] * g.Sigma[2][c];
+    g.a = TS[0][0] * g.T[0][0] + TS[0][1] * g.T[0][1] + TS[0][2] * g.T[0][2] + 0.3f;
+    g.b = TS[0][0] * g.T[1][0] + TS[0][1] * g.T[1][1] + TS[0][2] * g.T[1][2];
+    g.c = TS[1][0] * g.T[1][0] + TS[1][1] * g.T[1][1] + TS[1][2] * g.T[1][2] + 0.3f;
+    g.det = g.a * g.c - g.b * g.b;
+    if (!(g.det > 0.0f)) return false;
+
+    const float mid = 0.5f * (g.a + g.c);
+    const float disc = mid * mid - g.det;
+    const float lambda = mid + sqrtf((0.1f < disc) ? disc : 0.1f);
+    const float rf = ceilf(3.0f * sqrtf(lambda));
+    if (!(rf < 16777216.0f)) return false;
+    const int radius = static_cast<int>(rf);
+    g.u = cam.fx * px / pz + cam.cx;
+    g.v = cam.fy * py / pz + cam.cy;
+    if (!__builtin_isfinite(g.u) || !__builtin_isfinite(g.v)) return false;
+    if (g.u + radius < 0 || g.v + radius < 0 || g.u - radius >= cam.width || g.v - radius >= cam.height) return false;
+
+    g.x0 = max(0, static_cast<int>(g.u - radius) / cam.tileW);
+    g.y0 = max(0, static_cast<int>(g.v - radius) / cam.tileH);
+    g.x1 = min(cam.tilesX - 1, static_cast<int>(g.u + radius) / cam.tileW);
+    g.y1 = min(cam.tilesY - 1, static_cast<int>(g.v + radius) / cam.tileH);
+
+    g.dir[0] = m.x - cam.eye[0];
+    g.dir[1] = m.y - cam.eye[1];
+    g.dir[2] = m.z - cam.eye[2];
+    g.dlen = sqrtf(g.dir[0] * g.dir[0] + g.dir[1] * g.dir[1] + g.dir[2] * g.dir[2]);
+    g.dir[0] = g.dir[0] / g.dlen;
+    g.dir[1] = g.dir[1] / g.dlen;
+    g.dir[2] = g.dir[2] / g.dlen;
+    g.shCount = shBasis(cam.shDegree, g.dir[0], g.dir[1], g.dir[2], g.basis);
+    const float* shBase = PLANAR ? sh + i : sh + static_cast<size_t>(i) * cam.shCoeffs * 3;
+    g.rgbRaw = shAccumulate<PLANAR>(shBase, PLANAR ? static_cast<size_t>(n) : 1, g.shCount, g.basis);
+    // std::max(0, v) semantics (NaN -> 0).
+    g.rgb.x = (0.0f < g.rgbRaw.x) ? g.rgbRaw.x : 0.0f;
+    g.rgb.y = (0.0f < g.rgbRaw.y) ? g.rgbRaw.y : 0.0f;
+    g.rgb.z = (0.0f < g.rgbRaw.z) ? g.rgbRaw.z : 0.0f;
+    if constexpr (RAW) g.opacity = 1.0f / (1.0f + expf(-opacity[i]));
+    else g.opacity = opacity[i];
+    g.A = g.c / g.det;
+    g.B = -g.b / g.det;
+    g.C = g.a / g.det;
+    g.depth = pz;
+    return true;
+}
+
+// Per-Gaussian preprocess, one lane per Gaussian. Writes the splat record
+// {u, v, A, B | C, opacity, ext_x, ext_y | r, g, b, depth}, the packed tile rect, the tile count and
+// the depth sort key (0xffffffff for culled Gaussians). ext_x/y are conservative half-extents of
+// the region where alpha can reach 1/255, used by the blend to skip work that cannot change a
+// pixel.
+template <bool PLANAR, bool RAW>
 __global__ __launch_bounds__(256) void k_preprocess(int n, const float* __restrict__ pos,
                                                     const float* __restrict__ scale, const float4* __restrict__ rot,
                                                     const float* __restrict__ opacity, const float* __restrict__ sh,
                                                     CamParams cam, float4* __restrict__ rec0,
                                                     float4* __restrict__ rec1, float4* __restrict__ rec2,
                                                     uint2* __restrict__ rect, uint32_t* __restrict__ tilesTouched,
-                                                    uint32_t* __restrict__ depthKey, FrameCtl* __restrict__ ctl) {
+                                                    uint32_t* __restrict__ depthKey) {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < n) {
-        uint32_t touched = 0;
-        uint32_t key = 0xffffffffu;
-        do {
-            const F3 g = reinterpret_cast<const F3*>(pos)[i];
-            const float px = cam.R[0] * g.x + cam.R[1] * g.y + cam.R[2] * g.z + cam.t[0];
-            const float py = cam.R[3] * g.x + cam.R[4] * g.y + cam.R[5] * g.z + cam.t[1];
-            const float pz = cam.R[6] * g.x + cam.R[7] * g.y + cam.R[8] * g.z + cam.t[2];
-            if (!(pz >= 0.2f)) break;
-
-            const float4 q = rot[i];  // w, x, y, z
-            const float w = q.x, x = q.y, y = q.z, z = q.w;
-            const float Rq[3][3] = {{1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)},
-                                    {2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)},
-                                    {2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)}};
-            const F3 sc = reinterpret_cast<const F3*>(scale)[i];
-            const float s[3] = {sc.x * cam.splatScale, sc.y * cam.splatScale, sc.z * cam.splatScale};
-            float M[3][3], S[3][3];
-#pragma unroll
-            for (int r = 0; r < 3; ++r)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) M[r][c] = Rq[r][c] * s[c];
-#pragma unroll
-            for (int r = 0; r < 3; ++r)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) S[r][c] = M[r][0] * M[c][0] + M[r][1] * M[c][1] + M[r][2] * M[c][2];
-
-            const float tx = clampf(px / pz, -cam.limX, cam.limX) * pz;
-            const float ty = clampf(py / pz, -cam.limY, cam.limY) * pz;
-            const float J[2][3] = {{cam.fx / pz, 0, -cam.fx * tx / (pz * pz)},
-                                   {0, cam.fy / pz, -cam.fy * ty / (pz * pz)}};
-            float T[2][3];
-#pragma unroll
-            for (int r = 0; r < 2; ++r)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) T[r][c] = J[r][0] * cam.R[c] + J[r][1] * cam.R[3 + c] + J[r][2] * cam.R[6 + c];
-            float TS[2][3];
-#pragma unroll
-            for (int r = 0; r < 2; ++r)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) TS[r][c] = T[r][0] * S[0][c] + T[r][1] * S[1][c] + T[r][2] * S[2][c];
-            const float a = TS[0][0] * T[0][0] + TS[0][1] * T[0][1] + TS[0][2] * T[0][2] + 0.3f;
-            const float b = TS[0][0] * T[1][0] + TS[0][1] * T[1][1] + TS[0][2] * T[1][2];
-            const float c = TS[1][0] * T[1][0] + TS[1][1] * T[1][1] + TS[1][2] * T[1][2] + 0.3f;
-            const float det = a * c - b * b;
-            if (!(det > 0.0f)) break;
-
-            const float mid = 0.5f * (a + c);
-            const float disc = mid * mid - det;
-            const float lambda = mid + sqrtf((0.1f < disc) ? disc : 0.1f);
-            const float rf = ceilf(3.0f * sqrtf(lambda));
-            if (!(rf < 16777216.0f)) break;
-            const int radius = static_cast<int>(rf);
-            const float u = cam.fx * px / pz + cam.cx;
-            const float v = cam.fy * py / pz + cam.cy;
-            if (!isfinite(u) || !isfinite(v)) break;
-            if (u + radius < 0 || v + radius < 0 || u - radius >= cam.width || v - radius >= cam.height) break;
-
-            const int x0 = max(0, static_cast<int>(u - radius) / cam.tileW);
-            const int y0 = max(0, static_cast<int>(v - radius) / cam.tileH);
-            const int x1 = min(cam.tilesX - 1, static_cast<int>(u + radius) / cam.tileW);
-            const int y1 = min(cam.tilesY - 1, static_cast<int>(v + radius) / cam.tileH);
-
-            const float d0 = g.x - cam.eye[0], d1 = g.y - cam.eye[1], d2 = g.z - cam.eye[2];
-            const float len = sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
-            const float* shBase = PLANAR ? sh + i : sh + static_cast<size_t>(i) * cam.shCoeffs * 3;
-            const float3 rgb = shColor<PLANAR>(shBase, PLANAR ? static_cast<size_t>(n) : 1, cam.shDegree, d0 / len,
-                                       d1 / len, d2 / len);
-            const float op = opacity[i];
-
-            // Region where op * exp(-q/2) >= 1/255: q <= 2 ln(255 op); half-extents
-            // sqrt(k * a), sqrt(k * c) of that ellipse, widened by a 5% + 0.1 margin that covers
-            // the FP32 rounding of the blend's exponent by orders of magnitude.
-            float ex = -1e30f, ey = -1e30f;
-            if (op >= 1.0f / 255.0f) {
-                const float k = fmaxf(2.0f * logf(255.0f * op), 0.0f) * 1.05f + 0.1f;
-                ex = sqrtf(k * a) + 0.01f;
-                ey = sqrtf(k * c) + 0.01f;
-            }
-            rec0[i] = make_float4(u, v, c / det, -b / det);
-            rec1[i] = make_float4(a / det, op, ex, ey);
-            rec2[i] = make_float4(rgb.x, rgb.y, rgb.z, pz);
-            rect[i] = make_uint2(static_cast<uint32_t>(x0) | (static_cast<uint32_t>(y0) << 16),
-                                 static_cast<uint32_t>(x1) | (static_cast<uint32_t>(y1) << 16));
-            touched = static_cast<uint32_t>((x1 - x0 + 1) * (y1 - y0 + 1));
-            key = __float_as_uint(pz);
-        } while (false);
-        tilesTouched[i] = touched;
-        depthKey[i] = key;
+    if (i >= n) return;
+    GaussProj g;
+    if (!projectGaussian<PLANAR, RAW>(i, n, pos, scale, rot, opacity, sh, cam, g)) {
+        tilesTouched[i] = 0;
+        depthKey[i] = 0xffffffffu;
+        return;
     }
+    // Region where op * exp(-q/2) >= 1/255: q <= 2 ln(255 op); half-extents sqrt(k a), sqrt(k c)
+    // of that ellipse, widened by a 5% + 0.1 margin far above the FP32 rounding of the blend's
+    // exponent.
+    float ex = -1e30f, ey = -1e30f;
+    if (g.opacity >= 1.0f / 255.0f) {
+        const float k = fmaxf(2.0f * logf(255.0f * g.opacity), 0.0f) * 1.05f + 0.1f;
+        ex = sqrtf(k * g.a) + 0.01f;
+        ey = sqrtf(k * g.c) + 0.01f;
+    }
+    rec0[i] = make_float4(g.u, g.v, g.A, g.B);
+    rec1[i] = make_float4(g.C, g.opacity, ex, ey);
+    rec2[i] = make_float4(g.rgb.x, g.rgb.y, g.rgb.z, g.depth);
+    rect[i] = make_uint2(static_cast<uint32_t>(g.x0) | (static_cast<uint32_t>(g.y0) << 16),
+                         static_cast<uint32_t>(g.x1) | (static_cast<uint32_t>(g.y1) << 16));
+    tilesTouched[i] = static_cast<uint32_t>((g.x1 - g.x0 + 1) * (g.y1 - g.y0 + 1));
+    depthKey[i] = __float_as_uint(g.depth);
 }
 
 constexpr int kDupBlock = 256;                       // 4 wavefronts x 64 depth ranks
 constexpr int kDupTile = kDupBlock;                  // depth ranks per block
 
 // Pairs and visible Gaussians per block of 256 depth ranks.
-__global__ __launch_bounds__(kDupBlock) void k_pairTotals(const uint32_t* __restrict__ sortedIdx,
+static __global__ __launch_bounds__(kDupBlock) void k_pairTotals(const uint32_t* __restrict__ sortedIdx,
                                                           const uint32_t* __restrict__ tilesTouched, uint32_t n,
                                                           uint32_t* __restrict__ blockPairs,
                                                           uint32_t* __restrict__ blockVisible) {
@@ -239,7 +293,7 @@ __global__ __launch_bounds__(kDupBlock) void k_pairTotals(const uint32_t* __rest
 // One workgroup: exclusive scan of the per-block pair counts; publishes the pair total (the sum
 // gpu_render.cu:324-329 reads back to the host) and the visible count, and flags a pair total
 // above the pair buffers' capacity.
-__global__ __launch_bounds__(kDupBlock) void k_scanPairs(uint32_t* __restrict__ blockPairs,
+static __global__ __launch_bounds__(kDupBlock) void k_scanPairs(uint32_t* __restrict__ blockPairs,
                                                          const uint32_t* __restrict__ blockVisible, uint32_t blocks,
                                                          uint32_t cap, FrameCtl* __restrict__ ctl) {
     __shared__ uint32_t sTmp[kDupBlock / kWave];
@@ -295,7 +349,7 @@ __device__ inline int waveInclusiveMax(int v) {
 // the Gaussian owning each slot by marking the slots where Gaussians start (LDS head flags) and
 // a wavefront max-scan. pairStart[g] records each Gaussian's first slot, the contiguous range
 // the backward pass reduces over.
-__global__ __launch_bounds__(kDupBlock) void k_expandPairs(
+static __global__ __launch_bounds__(kDupBlock) void k_expandPairs(
     const uint32_t* __restrict__ sortedIdx, const uint32_t* __restrict__ tilesTouched, const uint2* __restrict__ rect,
     uint32_t n, int tilesX, const uint32_t* __restrict__ blockBase, uint32_t* __restrict__ pairStart,
     uint32_t* __restrict__ tileOut, uint32_t* __restrict__ gaussOut, const FrameCtl* __restrict__ ctl) {
@@ -358,7 +412,7 @@ __global__ __launch_bounds__(kDupBlock) void k_expandPairs(
 
 // Start and end of each tile's run in the sorted pairs (gpu_render.cu:164-177); ranges were
 // cleared to [0, 0) by the frame init.
-__global__ __launch_bounds__(256) void k_tileRanges(const uint32_t* __restrict__ sortedTile,
+static __global__ __launch_bounds__(256) void k_tileRanges(const uint32_t* __restrict__ sortedTile,
                                                     const FrameCtl* __restrict__ ctl, uint2* __restrict__ ranges) {
     if (ctl->overflow) return;
     const uint32_t total = ctl->pairs;

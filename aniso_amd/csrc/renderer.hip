@@ -80,16 +80,31 @@ void checkCamera(const ans_camera* cam) {
 
 // ---------------------------------------------------------------------------------------------
 
+FrameState::FrameState() {
+    ctl.reserve(1);
+    hipCheck(hipMemset(ctl.p, 0, sizeof(FrameCtl)), "hipMemset ctl");
+}
+
+void FrameState::reserve(size_t gaussians, size_t tileCount) {
+    const size_t m = std::max<size_t>(gaussians, 1);
+    rec0.reserve(m); rec1.reserve(m); rec2.reserve(m);
+    rect.reserve(m); tilesTouched.reserve(m); pairStart.reserve(m);
+    ranges.reserve(std::max<size_t>(tileCount, 1));
+}
+
+void FrameState::reservePairs(size_t cap) {
+    if (cap <= pairCap) return;
+    tileA.reserve(cap); gaussA.reserve(cap); tileB.reserve(cap); gaussB.reserve(cap);
+    pairCap = cap;
+}
+
 Engine::Engine() {
     hipCheck(hipGetDevice(&device), "hipGetDevice");
     hipCheck(hipStreamCreateWithFlags(&ownStream, hipStreamNonBlocking), "hipStreamCreate");
     stream = ownStream;
     for (auto& e : ev) hipCheck(hipEventCreate(&e), "hipEventCreate");
-    ctl.reserve(1);
-    hipCheck(hipMemsetAsync(ctl.p, 0, sizeof(FrameCtl), stream), "hipMemsetAsync ctl");
     hipCheck(hipHostMalloc(reinterpret_cast<void**>(&ctlHost), sizeof(FrameCtl), hipHostMallocDefault),
              "hipHostMalloc");
-    hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
 }
 
 Engine::~Engine() {
@@ -99,23 +114,12 @@ Engine::~Engine() {
     if (ownStream) (void)hipStreamDestroy(ownStream);
 }
 
-void Engine::reserveGaussians(size_t n) {
-    const size_t m = std::max<size_t>(n, 1);
-    rec0.reserve(m); rec1.reserve(m); rec2.reserve(m);
-    rect.reserve(m); tilesTouched.reserve(m); depthKey.reserve(m); pairStart.reserve(m);
-    sortKA.reserve(m); sortVA.reserve(m); sortKB.reserve(m); sortVB.reserve(m);
-}
-
-void Engine::reservePairs(size_t cap) {
-    if (cap <= pairCap) return;
-    tileA.reserve(cap); gaussA.reserve(cap); tileB.reserve(cap); gaussB.reserve(cap);
-    pairCap = cap;
-}
-
 constexpr int kDepthItems = 4;   // 1024 depth keys per sort block
 constexpr int kPairItems = 16;   // 4096 pairs per sort block
 
-void Engine::reserveScratch(size_t n) {
+void Engine::reserveScratch(size_t n, size_t pairCap) {
+    const size_t m = std::max<size_t>(n, 1);
+    depthKey.reserve(m); sortKA.reserve(m); sortVA.reserve(m); sortKB.reserve(m); sortVB.reserve(m);
     const size_t blocksDepth = (n + kSortBlock * kDepthItems - 1) / (kSortBlock * kDepthItems);
     const size_t blocksDup = (n + kDupTile - 1) / kDupTile;
     const size_t blocksPair = (pairCap + kSortBlock * kPairItems - 1) / (kSortBlock * kPairItems);
@@ -125,29 +129,37 @@ void Engine::reserveScratch(size_t n) {
     blockVisible.reserve(std::max<size_t>(blocksDup, 1));
 }
 
-// Enqueues one frame on the engine's stream. Outputs (either may be null) are device pointers.
-void Engine::enqueueFrame(const SceneDev& s, const CamParams& cp, const ans_options& o, uint8_t* out8,
-                          float* outf) {
+// Enqueues one frame on the engine's stream into f. Outputs (either may be null) are device
+// pointers.
+void Engine::enqueueFrame(FrameState& f, const SceneDev& s, const CamParams& cp, const ans_options& o,
+                          uint8_t* out8, float* outf) {
     const uint32_t n = static_cast<uint32_t>(s.n);
     const uint32_t tiles = static_cast<uint32_t>(cp.tilesX) * static_cast<uint32_t>(cp.tilesY);
-    reserveGaussians(n);
-    ranges.reserve(tiles);
-    reserveScratch(n);
-    last.tiles = tiles;
-    last.n = n;
-    last.cp = cp;
+    if (f.pairCap == 0) f.reservePairs(std::max<size_t>(static_cast<size_t>(n) * 8, size_t(1) << 20));
+    f.reserve(n, tiles);
+    reserveScratch(n, f.pairCap);
+    f.tiles = tiles;
+    f.n = n;
+    f.cp = cp;
+    f.opts = o;
+    FrameCtl* ctl = f.ctl.p;
 
     hipCheck(hipEventRecord(ev[0], stream), "hipEventRecord");
-    k_initFrame<<<(std::max<uint32_t>(tiles, 1) + 255) / 256, 256, 0, stream>>>(ctl.p, ranges.p, tiles);
+    k_initFrame<<<(std::max<uint32_t>(tiles, 1) + 255) / 256, 256, 0, stream>>>(ctl, f.ranges.p, tiles);
     if (n > 0) {
-        if (s.shPlanar)
-            k_preprocess<true><<<(n + 255) / 256, 256, 0, stream>>>(static_cast<int>(n), s.pos, s.scale, s.rot,
-                                                                    s.opacity, s.sh, cp, rec0.p, rec1.p, rec2.p,
-                                                                    rect.p, tilesTouched.p, depthKey.p, ctl.p);
+        const uint32_t grid = (n + 255) / 256;
+        if (s.raw)
+            k_preprocess<false, true><<<grid, 256, 0, stream>>>(static_cast<int>(n), s.pos, s.scale, s.rot, s.opacity,
+                                                                s.sh, cp, f.rec0.p, f.rec1.p, f.rec2.p, f.rect.p,
+                                                                f.tilesTouched.p, depthKey.p);
+        else if (s.shPlanar)
+            k_preprocess<true, false><<<grid, 256, 0, stream>>>(static_cast<int>(n), s.pos, s.scale, s.rot, s.opacity,
+                                                                s.sh, cp, f.rec0.p, f.rec1.p, f.rec2.p, f.rect.p,
+                                                                f.tilesTouched.p, depthKey.p);
         else
-            k_preprocess<false><<<(n + 255) / 256, 256, 0, stream>>>(static_cast<int>(n), s.pos, s.scale, s.rot,
-                                                                     s.opacity, s.sh, cp, rec0.p, rec1.p, rec2.p,
-                                                                     rect.p, tilesTouched.p, depthKey.p, ctl.p);
+            k_preprocess<false, false><<<grid, 256, 0, stream>>>(static_cast<int>(n), s.pos, s.scale, s.rot,
+                                                                 s.opacity, s.sh, cp, f.rec0.p, f.rec1.p, f.rec2.p,
+                                                                 f.rect.p, f.tilesTouched.p, depthKey.p);
         // Depth sort: four stable 8-bit passes over all Gaussians (culled ones carry the largest
         // key); ties keep ascending Gaussian index.
         const uint32_t blocks = (n + kSortBlock * kDepthItems - 1) / (kSortBlock * kDepthItems);
@@ -157,10 +169,10 @@ void Engine::enqueueFrame(const SceneDev& s, const CamParams& cp, const ans_opti
         uint32_t* vout[4] = {sortVA.p, sortVB.p, sortVA.p, sortVB.p};
         for (int pass = 0; pass < 4; ++pass) {
             k_radixUpsweep<kDepthItems><<<blocks, kSortBlock, 0, stream>>>(kin, nullptr, n, n, 8 * pass, 8, hist.p,
-                                                                           blocks, ctl.p);
-            k_scanRows<<<kRadix, kSortBlock, 0, stream>>>(hist.p, blocks, blocks, rowTotal.p, ctl.p);
+                                                                           blocks, ctl);
+            k_scanRows<<<kRadix, kSortBlock, 0, stream>>>(hist.p, blocks, blocks, rowTotal.p, ctl);
             k_radixDownsweep<kDepthItems><<<blocks, kSortBlock, 0, stream>>>(
-                kin, vin, kout[pass], vout[pass], nullptr, n, n, 8 * pass, 8, hist.p, rowTotal.p, blocks, ctl.p);
+                kin, vin, kout[pass], vout[pass], nullptr, n, n, 8 * pass, 8, hist.p, rowTotal.p, blocks, ctl);
             kin = kout[pass];
             vin = vout[pass];
         }
@@ -170,104 +182,165 @@ void Engine::enqueueFrame(const SceneDev& s, const CamParams& cp, const ans_opti
     const int tileBits = std::max(1, bitsFor(tiles - 1));
     const int passes = (tileBits + 7) / 8;
     const int digitBits = (tileBits + passes - 1) / passes;
-    last.sortedTile = tileA.p;
-    last.sortedGauss = gaussA.p;
+    f.sortedTile = f.tileA.p;
+    f.sortedGauss = f.gaussA.p;
     if (n > 0) {
         const uint32_t blocksDup = (n + kDupTile - 1) / kDupTile;
-        const uint32_t cap = static_cast<uint32_t>(pairCap);
-        k_pairTotals<<<blocksDup, kDupBlock, 0, stream>>>(sortVB.p, tilesTouched.p, n, blockPairs.p,
+        const uint32_t cap = static_cast<uint32_t>(f.pairCap);
+        k_pairTotals<<<blocksDup, kDupBlock, 0, stream>>>(sortVB.p, f.tilesTouched.p, n, blockPairs.p,
                                                           blockVisible.p);
-        k_scanPairs<<<1, kDupBlock, 0, stream>>>(blockPairs.p, blockVisible.p, blocksDup, cap, ctl.p);
-        k_expandPairs<<<blocksDup, kDupBlock, 0, stream>>>(sortVB.p, tilesTouched.p, rect.p, n, cp.tilesX,
-                                                           blockPairs.p, pairStart.p, tileA.p, gaussA.p, ctl.p);
+        k_scanPairs<<<1, kDupBlock, 0, stream>>>(blockPairs.p, blockVisible.p, blocksDup, cap, ctl);
+        k_expandPairs<<<blocksDup, kDupBlock, 0, stream>>>(sortVB.p, f.tilesTouched.p, f.rect.p, n, cp.tilesX,
+                                                           blockPairs.p, f.pairStart.p, f.tileA.p, f.gaussA.p, ctl);
         const uint32_t blocksPair =
-            static_cast<uint32_t>((pairCap + kSortBlock * kPairItems - 1) / (kSortBlock * kPairItems));
-        uint32_t* tk[2] = {tileA.p, tileB.p};
-        uint32_t* tv[2] = {gaussA.p, gaussB.p};
+            static_cast<uint32_t>((f.pairCap + kSortBlock * kPairItems - 1) / (kSortBlock * kPairItems));
+        uint32_t* tk[2] = {f.tileA.p, f.tileB.p};
+        uint32_t* tv[2] = {f.gaussA.p, f.gaussB.p};
         for (int pass = 0; pass < passes; ++pass) {
             const int src = pass & 1, dst = src ^ 1;
             const int shift = pass * digitBits;
             const int bits = std::min(digitBits, tileBits - shift);
-            k_radixUpsweep<kPairItems><<<blocksPair, kSortBlock, 0, stream>>>(tk[src], &ctl.p->pairs, 0, cap, shift,
-                                                                              bits, hist.p, blocksPair, ctl.p);
-            k_scanRows<<<1u << bits, kSortBlock, 0, stream>>>(hist.p, blocksPair, blocksPair, rowTotal.p, ctl.p);
+            k_radixUpsweep<kPairItems><<<blocksPair, kSortBlock, 0, stream>>>(tk[src], &ctl->pairs, 0, cap, shift,
+                                                                              bits, hist.p, blocksPair, ctl);
+            k_scanRows<<<1u << bits, kSortBlock, 0, stream>>>(hist.p, blocksPair, blocksPair, rowTotal.p, ctl);
             k_radixDownsweep<kPairItems><<<blocksPair, kSortBlock, 0, stream>>>(
-                tk[src], tv[src], tk[dst], tv[dst], &ctl.p->pairs, 0, cap, shift, bits, hist.p, rowTotal.p,
-                blocksPair, ctl.p);
+                tk[src], tv[src], tk[dst], tv[dst], &ctl->pairs, 0, cap, shift, bits, hist.p, rowTotal.p,
+                blocksPair, ctl);
         }
-        last.sortedTile = tk[passes & 1];
-        last.sortedGauss = tv[passes & 1];
-        k_tileRanges<<<static_cast<uint32_t>((pairCap + 255) / 256), 256, 0, stream>>>(last.sortedTile, ctl.p,
-                                                                                        ranges.p);
+        f.sortedTile = tk[passes & 1];
+        f.sortedGauss = tv[passes & 1];
+        k_tileRanges<<<static_cast<uint32_t>((f.pairCap + 255) / 256), 256, 0, stream>>>(f.sortedTile, ctl,
+                                                                                          f.ranges.p);
     }
     hipCheck(hipEventRecord(ev[2], stream), "hipEventRecord");
-    launchBlend(cp, o, out8, outf);
+    launchBlend(f, out8, outf);
     hipCheck(hipGetLastError(), "kernel launch");
     hipCheck(hipEventRecord(ev[3], stream), "hipEventRecord");
 }
 
 template <int TW, int TH, int BATCH>
-void Engine::launchBlendB(const CamParams& cp, bool cull, uint8_t* out8, float* outf) {
+void Engine::launchBlendB(FrameState& f, bool cull, uint8_t* out8, float* outf) {
+    const CamParams& cp = f.cp;
     const uint32_t tiles = static_cast<uint32_t>(cp.tilesX) * cp.tilesY;
     if (cull)
-        k_blend<TW, TH, BATCH, true><<<tiles, TW * TH, 0, stream>>>(ranges.p, last.sortedGauss, rec0.p, rec1.p,
-                                                                     rec2.p, cp.width, cp.height, cp.tilesX, out8,
-                                                                     outf, ctl.p);
+        k_blend<TW, TH, BATCH, true><<<tiles, TW * TH, 0, stream>>>(f.ranges.p, f.sortedGauss, f.rec0.p, f.rec1.p,
+                                                                     f.rec2.p, cp.width, cp.height, cp.tilesX, out8,
+                                                                     outf, f.ctl.p);
     else
-        k_blend<TW, TH, BATCH, false><<<tiles, TW * TH, 0, stream>>>(ranges.p, last.sortedGauss, rec0.p, rec1.p,
-                                                                      rec2.p, cp.width, cp.height, cp.tilesX, out8,
-                                                                      outf, ctl.p);
+        k_blend<TW, TH, BATCH, false><<<tiles, TW * TH, 0, stream>>>(f.ranges.p, f.sortedGauss, f.rec0.p, f.rec1.p,
+                                                                      f.rec2.p, cp.width, cp.height, cp.tilesX, out8,
+                                                                      outf, f.ctl.p);
 }
 
 template <int TW, int TH>
-void Engine::launchBlendT(const CamParams& cp, int batch, bool cull, uint8_t* out8, float* outf) {
-    if (batch == 64) launchBlendB<TW, TH, 64>(cp, cull, out8, outf);
-    else if (batch == 128) launchBlendB<TW, TH, 128>(cp, cull, out8, outf);
-    else launchBlendB<TW, TH, 256>(cp, cull, out8, outf);
+void Engine::launchBlendT(FrameState& f, bool cull, uint8_t* out8, float* outf) {
+    if (f.opts.batch_depth == 64) launchBlendB<TW, TH, 64>(f, cull, out8, outf);
+    else if (f.opts.batch_depth == 128) launchBlendB<TW, TH, 128>(f, cull, out8, outf);
+    else launchBlendB<TW, TH, 256>(f, cull, out8, outf);
 }
 
-void Engine::launchBlend(const CamParams& cp, const ans_options& o, uint8_t* out8, float* outf) {
-    const bool cull = !(o.flags & ANS_FLAG_NO_CULL);
-    if (o.tile_w == 8) launchBlendT<8, 8>(cp, o.batch_depth, cull, out8, outf);
-    else if (o.tile_w == 32) launchBlendT<32, 16>(cp, o.batch_depth, cull, out8, outf);
-    else launchBlendT<16, 16>(cp, o.batch_depth, cull, out8, outf);
+void Engine::launchBlend(FrameState& f, uint8_t* out8, float* outf) {
+    const bool cull = !(f.opts.flags & ANS_FLAG_NO_CULL);
+    if (f.opts.tile_w == 8) launchBlendT<8, 8>(f, cull, out8, outf);
+    else if (f.opts.tile_w == 32) launchBlendT<32, 16>(f, cull, out8, outf);
+    else launchBlendT<16, 16>(f, cull, out8, outf);
+}
+
+bool Engine::readBack(FrameState& f) {
+    hipCheck(hipMemcpyAsync(ctlHost, f.ctl.p, 32, hipMemcpyDeviceToHost, stream), "hipMemcpyAsync ctl");
+    hipCheck(hipMemsetAsync(&f.ctl.p->sticky, 0, 4, stream), "hipMemsetAsync");
+    hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    f.visible = ctlHost->visible;
+    f.pairs = ctlHost->pairs;
+    if (ctlHost->sticky) {
+        if (ctlHost->pairs >= (1u << 30) - 1) throw std::runtime_error("pair count exceeds 2^30");
+        f.reservePairs(static_cast<size_t>(ctlHost->pairs) + ctlHost->pairs / 4 + 1024);
+        return true;
+    }
+    return false;
 }
 
 // Renders a frame and waits for it; grows the pair buffers and renders again if the pair count
-// exceeded them. Reads back the control block.
-int Engine::renderSync(const SceneDev& s, const CamParams& cp, const ans_options& o, uint8_t* out8, float* outf) {
-    if (pairCap == 0) reservePairs(std::max<size_t>(static_cast<size_t>(s.n) * 8, size_t(1) << 20));
+// exceeded them.
+int Engine::renderSync(FrameState& f, const SceneDev& s, const CamParams& cp, const ans_options& o, uint8_t* out8,
+                       float* outf) {
     int retries = 0;
     while (true) {
-        enqueueFrame(s, cp, o, out8, outf);
-        hipCheck(hipMemcpyAsync(ctlHost, ctl.p, 32, hipMemcpyDeviceToHost, stream), "hipMemcpyAsync ctl");
-        hipCheck(hipMemsetAsync(&ctl.p->sticky, 0, 4, stream), "hipMemsetAsync");
-        hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
-        if (!ctlHost->overflow) break;
-        if (ctlHost->pairs >= (1u << 30) - 1) throw std::runtime_error("pair count exceeds 2^30");
-        reservePairs(static_cast<size_t>(ctlHost->pairs) + ctlHost->pairs / 4 + 1024);
+        enqueueFrame(f, s, cp, o, out8, outf);
+        if (!readBack(f)) break;
         ++retries;
     }
-    last.visible = ctlHost->visible;
-    last.pairs = ctlHost->pairs;
-    last.retries = retries;
+    f.retries = retries;
     return retries;
 }
 
-void Engine::fillStats(ans_stats* st) {
+void Engine::fillStats(const FrameState& f, ans_stats* st) {
     if (!st) return;
     float a = 0, b = 0, c = 0;
     hipCheck(hipEventElapsedTime(&a, ev[0], ev[1]), "hipEventElapsedTime");
     hipCheck(hipEventElapsedTime(&b, ev[1], ev[2]), "hipEventElapsedTime");
     hipCheck(hipEventElapsedTime(&c, ev[2], ev[3]), "hipEventElapsedTime");
-    st->visible = last.visible;
-    st->tile_pairs = last.pairs;
+    st->visible = f.visible;
+    st->tile_pairs = f.pairs;
     st->project_ms = a;
     st->sort_ms = b;
     st->blend_ms = c;
-    st->pair_capacity = static_cast<int64_t>(pairCap);
-    st->retries = last.retries;
+    st->pair_capacity = static_cast<int64_t>(f.pairCap);
+    st->retries = f.retries;
     st->reserved = 0;
+}
+
+// Intermediates of a rendered frame, copied to host buffers (see ans_frame_dump).
+void dumpFrame(hipStream_t stream, const FrameState& f, uint32_t* splat_bits, int32_t* rect, uint64_t* keys,
+               uint32_t* values, uint32_t* ranges) {
+    const size_t n = f.n, P = f.pairs, T = f.tiles;
+    hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    std::vector<float4> r0(n), r1(n), r2(n);
+    std::vector<uint2> rc(n);
+    std::vector<uint32_t> tt(n);
+    auto down = [](void* dst, const void* src, size_t bytes) {
+        if (bytes) hipCheck(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    };
+    down(tt.data(), f.tilesTouched.p, 4 * n);
+    if (splat_bits || keys) {
+        down(r0.data(), f.rec0.p, 16 * n);
+        down(r1.data(), f.rec1.p, 16 * n);
+        down(r2.data(), f.rec2.p, 16 * n);
+    }
+    if (rect) down(rc.data(), f.rect.p, 8 * n);
+    for (size_t i = 0; i < n; ++i) {
+        const bool vis = tt[i] > 0;
+        if (splat_bits) {
+            const float v[10] = {r0[i].x, r0[i].y, r0[i].z, r0[i].w, r1[i].x, r1[i].y, r2[i].w, r2[i].x, r2[i].y,
+                                 r2[i].z};
+            if (vis) std::memcpy(splat_bits + 10 * i, v, sizeof v);
+            else std::memset(splat_bits + 10 * i, 0, sizeof v);
+        }
+        if (rect) {
+            if (vis) {
+                rect[4 * i + 0] = static_cast<int32_t>(rc[i].x & 0xffffu);
+                rect[4 * i + 1] = static_cast<int32_t>(rc[i].x >> 16);
+                rect[4 * i + 2] = static_cast<int32_t>(rc[i].y & 0xffffu);
+                rect[4 * i + 3] = static_cast<int32_t>(rc[i].y >> 16);
+            } else {
+                rect[4 * i + 0] = 0; rect[4 * i + 1] = 0; rect[4 * i + 2] = -1; rect[4 * i + 3] = -1;
+            }
+        }
+    }
+    std::vector<uint32_t> st(P), sg(P);
+    if (P && (keys || values)) {
+        down(st.data(), f.sortedTile, 4 * P);
+        down(sg.data(), f.sortedGauss, 4 * P);
+    }
+    for (size_t p = 0; p < P; ++p) {
+        if (values) values[p] = sg[p];
+        if (keys) {
+            uint32_t bits;
+            std::memcpy(&bits, &r2[sg[p]].w, 4);
+            keys[p] = (static_cast<uint64_t>(st[p]) << 32) | bits;
+        }
+    }
+    if (ranges) down(ranges, f.ranges.p, 8 * T);
 }
 
 }  // namespace ans
@@ -359,7 +432,7 @@ int ans_renderer_create(int64_t n, int32_t sh_degree, const float* pos_xyz, cons
                 for (size_t j = 0; j < coeffs * 3; ++j) planar[j * n + i] = sh[static_cast<size_t>(i) * coeffs * 3 + j];
             upload(r->sh.p, planar.data(), sizeof(float) * planar.size());
         }
-        r->scene = SceneDev{n, sh_degree, r->pos.p, r->scale.p, r->rot.p, r->opacity.p, r->sh.p, true};
+        r->scene = SceneDev{n, sh_degree, r->pos.p, r->scale.p, r->rot.p, r->opacity.p, r->sh.p, true, false};
         r->engine = std::make_unique<Engine>();
         *out = r.release();
         return ANS_OK;
@@ -397,13 +470,13 @@ int ans_render_float(ans_renderer* r, const ans_camera* cam, const ans_options* 
         r->image8.reserve(3 * pixels);
         if (rgbf_out) r->imagef.reserve(3 * pixels);
         Engine& e = *r->engine;
-        e.renderSync(r->scene, cp, o, r->image8.p, rgbf_out ? r->imagef.p : nullptr);
+        e.renderSync(e.frame, r->scene, cp, o, r->image8.p, rgbf_out ? r->imagef.p : nullptr);
         if (rgb_out)
             hipCheck(hipMemcpy(rgb_out, r->image8.p, 3 * pixels, hipMemcpyDeviceToHost), "hipMemcpy image");
         if (rgbf_out)
             hipCheck(hipMemcpy(rgbf_out, r->imagef.p, 3 * pixels * sizeof(float), hipMemcpyDeviceToHost),
                      "hipMemcpy image");
-        e.fillStats(stats);
+        e.fillStats(e.frame, stats);
         return ANS_OK;
     });
 }
@@ -426,8 +499,7 @@ int ans_render_async(ans_renderer* r, const ans_camera* cam, const ans_options* 
         const CamParams cp = makeCamParams(*cam, o, r->scene.shDegree);
         if (cp.tilesX > 65535 || cp.tilesY > 65535) throw std::invalid_argument("too many tiles");
         Engine& e = *r->engine;
-        if (e.pairCap == 0) e.reservePairs(std::max<size_t>(static_cast<size_t>(r->scene.n) * 8, size_t(1) << 20));
-        e.enqueueFrame(r->scene, cp, o, dev_rgb, dev_rgbf);
+        e.enqueueFrame(e.frame, r->scene, cp, o, dev_rgb, dev_rgbf);
         return ANS_OK;
     });
 }
@@ -436,16 +508,10 @@ int ans_render_wait(ans_renderer* r, ans_stats* stats) {
     return guarded([&] {
         if (!r) throw std::invalid_argument("renderer is null");
         Engine& e = *r->engine;
-        hipCheck(hipMemcpyAsync(e.ctlHost, e.ctl.p, 32, hipMemcpyDeviceToHost, e.stream), "hipMemcpyAsync ctl");
-        hipCheck(hipMemsetAsync(&e.ctl.p->sticky, 0, 4, e.stream), "hipMemsetAsync");
-        hipCheck(hipStreamSynchronize(e.stream), "hipStreamSynchronize");
-        const bool overflow = e.ctlHost->sticky != 0;
-        e.last.visible = e.ctlHost->visible;
-        e.last.pairs = e.ctlHost->pairs;
-        e.last.retries = 0;
-        e.fillStats(stats);
+        const bool overflow = e.readBack(e.frame);
+        e.frame.retries = 0;
+        e.fillStats(e.frame, stats);
         if (overflow) {
-            e.reservePairs(static_cast<size_t>(e.ctlHost->pairs) + e.ctlHost->pairs / 4 + 1024);
             g_lastError = "pair count exceeded the pair buffers; buffers grown, render again";
             return ANS_ERR_CAPACITY;
         }
@@ -456,11 +522,11 @@ int ans_render_wait(ans_renderer* r, ans_stats* stats) {
 int ans_frame_counts(ans_renderer* r, int64_t* n, int64_t* visible, int64_t* pairs, int64_t* tiles) {
     return guarded([&] {
         if (!r) throw std::invalid_argument("renderer is null");
-        const Engine& e = *r->engine;
-        if (n) *n = e.last.n;
-        if (visible) *visible = e.last.visible;
-        if (pairs) *pairs = e.last.pairs;
-        if (tiles) *tiles = e.last.tiles;
+        const FrameState& f = r->engine->frame;
+        if (n) *n = static_cast<int64_t>(f.n);
+        if (visible) *visible = f.visible;
+        if (pairs) *pairs = f.pairs;
+        if (tiles) *tiles = static_cast<int64_t>(f.tiles);
         return ANS_OK;
     });
 }
@@ -469,55 +535,7 @@ int ans_frame_dump(ans_renderer* r, uint32_t* splat_bits, int32_t* rect, uint64_
                    uint32_t* ranges) {
     return guarded([&] {
         if (!r) throw std::invalid_argument("renderer is null");
-        Engine& e = *r->engine;
-        const size_t n = e.last.n, P = e.last.pairs, T = e.last.tiles;
-        hipCheck(hipStreamSynchronize(e.stream), "hipStreamSynchronize");
-        std::vector<float4> r0(n), r1(n), r2(n);
-        std::vector<uint2> rc(n);
-        std::vector<uint32_t> tt(n);
-        auto down = [](void* dst, const void* src, size_t bytes) {
-            if (bytes) hipCheck(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
-        };
-        down(tt.data(), e.tilesTouched.p, 4 * n);
-        if (splat_bits || keys) {
-            down(r0.data(), e.rec0.p, 16 * n);
-            down(r1.data(), e.rec1.p, 16 * n);
-            down(r2.data(), e.rec2.p, 16 * n);
-        }
-        if (rect) down(rc.data(), e.rect.p, 8 * n);
-        for (size_t i = 0; i < n; ++i) {
-            const bool vis = tt[i] > 0;
-            if (splat_bits) {
-                const float v[10] = {r0[i].x, r0[i].y, r0[i].z, r0[i].w, r1[i].x, r1[i].y, r2[i].w, r2[i].x,
-                                     r2[i].y, r2[i].z};
-                if (vis) std::memcpy(splat_bits + 10 * i, v, sizeof v);
-                else std::memset(splat_bits + 10 * i, 0, sizeof v);
-            }
-            if (rect) {
-                if (vis) {
-                    rect[4 * i + 0] = static_cast<int32_t>(rc[i].x & 0xffffu);
-                    rect[4 * i + 1] = static_cast<int32_t>(rc[i].x >> 16);
-                    rect[4 * i + 2] = static_cast<int32_t>(rc[i].y & 0xffffu);
-                    rect[4 * i + 3] = static_cast<int32_t>(rc[i].y >> 16);
-                } else {
-                    rect[4 * i + 0] = 0; rect[4 * i + 1] = 0; rect[4 * i + 2] = -1; rect[4 * i + 3] = -1;
-                }
-            }
-        }
-        std::vector<uint32_t> st(P), sg(P);
-        if (P && (keys || values)) {
-            down(st.data(), e.last.sortedTile, 4 * P);
-            down(sg.data(), e.last.sortedGauss, 4 * P);
-        }
-        for (size_t p = 0; p < P; ++p) {
-            if (values) values[p] = sg[p];
-            if (keys) {
-                uint32_t bits;
-                std::memcpy(&bits, &r2[sg[p]].w, 4);
-                keys[p] = (static_cast<uint64_t>(st[p]) << 32) | bits;
-            }
-        }
-        if (ranges) down(ranges, e.ranges.p, 8 * T);
+        dumpFrame(r->engine->stream, r->engine->frame, splat_bits, rect, keys, values, ranges);
         return ANS_OK;
     });
 }

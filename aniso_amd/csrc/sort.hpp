@@ -100,7 +100,7 @@ __global__ __launch_bounds__(kSortBlock) void k_radixUpsweep(const uint32_t* __r
 
 // Exclusive scan of one row hist[bin * stride, + len) in place by one workgroup (blockIdx.x =
 // bin); the row total goes to rowTotal[bin].
-__global__ __launch_bounds__(kSortBlock) void k_scanRows(uint32_t* __restrict__ hist, uint32_t stride, uint32_t len,
+static __global__ __launch_bounds__(kSortBlock) void k_scanRows(uint32_t* __restrict__ hist, uint32_t stride, uint32_t len,
                                                          uint32_t* __restrict__ rowTotal,
                                                          const FrameCtl* __restrict__ ctl) {
     __shared__ uint32_t sTmp[kSortWaves];

@@ -118,6 +118,61 @@ int ans_frame_counts(ans_renderer* r, int64_t* n, int64_t* visible, int64_t* pai
 int ans_frame_dump(ans_renderer* r, uint32_t* splat_bits, int32_t* rect, uint64_t* keys, uint32_t* values,
                    uint32_t* ranges);
 
+/* ---- Differentiable path (no counterpart in the reference, whose training calls a
+ * third-party rasterizer at pipeline/train_gsplat.py:96-102; this is what a
+ * torch.autograd.Function binds in its place). All pointers below are DEVICE pointers to
+ * contiguous float32 arrays; `stream` is the hipStream_t to run on (NULL: the default stream), so
+ * the kernels order with the caller's producers and consumers of those arrays. */
+
+/* Raw parameters as an optimizer holds them: means [n][3], log_scales [n][3], quats [n][4]
+ * (w, x, y, z; normalized inside), opacity_logits [n], sh [n][(sh_degree+1)^2][3]. */
+typedef struct ans_params {
+    int64_t n;
+    int32_t sh_degree;
+    const float* means;
+    const float* log_scales;
+    const float* quats;
+    const float* opacity_logits;
+    const float* sh;
+} ans_params;
+
+/* Gradient outputs, same shapes as ans_params; overwritten (not accumulated). */
+typedef struct ans_grads {
+    float* means;
+    float* log_scales;
+    float* quats;
+    float* opacity_logits;
+    float* sh;
+} ans_grads;
+
+typedef struct ans_diff ans_diff;
+
+int ans_diff_create(ans_diff** out);
+void ans_diff_destroy(ans_diff* d);
+
+/* Forward: renders the unclamped colour image [height][width][3] into `image` and keeps the
+ * frame's state (splat records, tile lists) for the backward pass under *frame. Waits for the
+ * pair count (as gpu_render.cu:329 does) and renders again if the pair buffers overflowed. */
+int ans_diff_forward(ans_diff* d, const ans_params* params, const ans_camera* cam, const ans_options* options,
+                     float* image, void* stream, int64_t* frame, ans_stats* stats);
+
+/* Backward for a frame from ans_diff_forward: gradients of L w.r.t. every raw parameter, given
+ * grad_image = dL/dimage [height][width][3]. The frame stays valid until released. */
+int ans_diff_backward(ans_diff* d, int64_t frame, const ans_params* params, const float* grad_image,
+                      const ans_grads* grads, void* stream);
+
+int ans_diff_release(ans_diff* d, int64_t frame);
+
+/* After ans_diff_backward on a frame: per-Gaussian gradients w.r.t. its splat record, copied to
+ * host [n][9] = dL/d{u, v, conic A, B, C, opacity, r, g, b} (u, v: the screen-space mean
+ * gradient that densification heuristics read). Valid until the next backward on the handle. */
+int ans_diff_splat_grads(ans_diff* d, int64_t frame, float* out);
+
+/* Per-tile lists of a frame (sizes from ans_diff_frame_counts), for gradient checks against a
+ * CPU reference evaluated on the same lists: values [pairs], ranges [tiles][2]. */
+int ans_diff_frame_counts(ans_diff* d, int64_t frame, int64_t* visible, int64_t* pairs, int64_t* tiles);
+int ans_diff_frame_dump(ans_diff* d, int64_t frame, uint32_t* values, uint32_t* ranges);
+
 /* Host activation used by ans_renderer_create_raw, exposed for parity tests. */
 void ans_activate(int64_t n, const float* log_scale_xyz, const float* opacity_logit, const float* quat_wxyz,
                   float* scale_xyz, float* opacity, float* rot_wxyz);

@@ -72,6 +72,12 @@ def lib():
         L.orc_frame_splat_bits.argtypes = [P, P]
         L.orc_activate.argtypes = [ctypes.c_int64, P, P, P, P, P, P]
         L.orc_mt19937_scene.argtypes = [ctypes.c_uint32, ctypes.c_int64, P, P, P, P, P]
+        L.orc_grad_run.restype = ctypes.c_int64
+        L.orc_grad_run.argtypes = [ctypes.c_int64, P, P, P, P, P, ctypes.POINTER(_Camera), ctypes.c_int,
+                                   ctypes.c_double, ctypes.c_int, ctypes.c_int, P, P, P, P, P, P, ctypes.c_int]
+        L.orc_grad_fd.argtypes = [ctypes.c_int64, P, P, P, P, P, ctypes.POINTER(_Camera), ctypes.c_int,
+                                  ctypes.c_double, ctypes.c_int, ctypes.c_int, P, P, ctypes.c_int64, ctypes.c_double,
+                                  P, P, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -233,3 +239,57 @@ def ref_render(scene: ActivatedScene, camera, sh_degree=3, splat_scale=1.0):
     if rc != 0:
         raise RuntimeError("reference render failed")
     return out, st
+
+
+# --- FP64 gradients (grad.cpp) ------------------------------------------------------------------
+
+PARAM_GROUPS = {"means": slice(0, 3), "log_scales": slice(3, 6), "quats": slice(6, 10), "opacity_logits": slice(10, 11),
+                "sh_band0": slice(11, 14), "sh_band1": slice(14, 23), "sh_band2": slice(23, 38),
+                "sh_band3": slice(38, 59)}
+
+
+def _raw64(raw):
+    f = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    sh = raw.sh
+    if sh.shape[1] != 16:
+        full = np.zeros((raw.n, 16, 3), np.float64)
+        full[:, :sh.shape[1]] = sh
+        sh = full
+    return f(raw.means), f(raw.log_scales), f(raw.quats), f(raw.opacity_logits), f(sh)
+
+
+def grad_run(raw, camera, dLdC, sh_degree=3, splat_scale=1.0, tile=(16, 16), lists=None, threads=0,
+             splat_grad=False):
+    """FP64 forward image [H,W,3] and analytic gradient [n,59] of L = sum(dLdC * C) w.r.t. the raw
+    parameters (means, log-scales, quats, opacity logits, SH[16][3]). `lists` = (start[T+1],
+    concatenated per-tile lists) replaces the oracle's own binning."""
+    arrays = _raw64(raw)
+    n = raw.n
+    image = np.empty((camera.height, camera.width, 3), np.float64)
+    grad = np.empty((n, 59), np.float64)
+    g = np.ascontiguousarray(dLdC, dtype=np.float64)
+    ls_ptr = lst_ptr = None
+    if lists is not None:
+        start = np.ascontiguousarray(lists[0], np.uint32)
+        lst = np.ascontiguousarray(lists[1], np.uint32)
+        ls_ptr, lst_ptr = _ptr(start), _ptr(lst)
+    sg = np.empty((n, 9), np.float64) if splat_grad else None
+    vis = lib().orc_grad_run(n, *[_ptr(a) for a in arrays], ctypes.byref(_cam(camera)), sh_degree, splat_scale,
+                             tile[0], tile[1], ls_ptr, lst_ptr, _ptr(g), _ptr(image), _ptr(grad),
+                             _ptr(sg) if splat_grad else None, threads)
+    if splat_grad:
+        return image, grad, vis, sg
+    return image, grad, vis
+
+
+def grad_fd(raw, camera, dLdC, idx, sh_degree=3, splat_scale=1.0, tile=(16, 16), h_rel=1e-6, threads=0):
+    """FP64 central differences [m,59] of L = sum(dLdC * C) for Gaussians idx, and flags [m,59]
+    (1: the perturbation changed the blend structure, 2: culled)."""
+    arrays = _raw64(raw)
+    idx = np.ascontiguousarray(idx, np.int64)
+    fd = np.empty((len(idx), 59), np.float64)
+    flags = np.empty((len(idx), 59), np.uint8)
+    g = np.ascontiguousarray(dLdC, dtype=np.float64)
+    lib().orc_grad_fd(raw.n, *[_ptr(a) for a in arrays], ctypes.byref(_cam(camera)), sh_degree, splat_scale, tile[0],
+                      tile[1], _ptr(g), _ptr(idx), len(idx), h_rel, _ptr(fd), _ptr(flags), threads)
+    return fd, flags
