@@ -35,6 +35,7 @@ def main():
     ap.add_argument("--workload", default="synthetic-741883-1280x720")
     ap.add_argument("--skip-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--skip-extra", action="store_true", help="only the headline forward workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -101,6 +102,11 @@ def main():
         "kernel_ms": round(blend, 4),
     }
 
+    extra = {}
+    if not args.skip_extra:
+        extra["fwd_bwd"] = bench_fwd_bwd(raw, cam, max(5, args.steps // 4), args.warmup)
+        extra["fwd_clustered_2.2M"] = bench_forward("clustered-2.2M-1280x720", max(10, args.steps // 2), args.warmup)
+
     cpu = None
     if not args.skip_cpu:
         cpu = cpu_baseline(raw, cam, args.cpu_seconds)
@@ -127,8 +133,77 @@ def main():
         "frame_hbm": {"algorithmic_bytes": b_fwd, "achieved_GBs": round(b_fwd / (ms * 1e-3) / 1e9, 1)},
         "vs_baseline_note": "value / 2.3 ms, the reference's published RTX 5090 forward time for its "
                             "741,883-Gaussian SH3 scene at 1280 wide (README.md:26-28); lower is better",
+        **extra,
     }
     print(json.dumps(out), flush=True)
+
+
+def bench_forward(workload, steps, warmup):
+    """Forward ms/frame of another synthetic workload, same protocol as the headline line."""
+    import torch
+
+    from aniso_amd import _lib, scenes
+    from aniso_amd.renderer import GpuRenderer, RenderOptions, RenderStats, camera_c
+
+    wl = scenes.WORKLOADS[workload]
+    raw = scenes.synthetic_scene(wl["n"], wl["width"], wl["height"], clustered=wl["clustered"])
+    cam = scenes.pinhole_camera(wl["width"], wl["height"])
+    L = _lib.load()
+    with GpuRenderer(raw) as r:
+        st = RenderStats()
+        for _ in range(max(1, warmup)):
+            r.render(cam, st)
+        dev = torch.empty(cam.width * cam.height * 3, dtype=torch.uint8, device="cuda")
+        c_cam, c_opt = camera_c(cam), RenderOptions().c()
+        _lib.check(L.ans_render_wait(r._h, None))
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            _lib.check(L.ans_render_async(r._h, c_cam, c_opt, dev.data_ptr(), None))
+        _lib.check(L.ans_render_wait(r._h, None))
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+    return {"ms_per_frame": round(ms, 4), "gaussians": raw.n, "visible": st.visible, "pairs": st.tile_pairs,
+            "width": cam.width, "height": cam.height, "steps": steps,
+            "stages_ms": {"project": round(st.project_ms, 4), "sort": round(st.sort_ms, 4),
+                          "blend": round(st.blend_ms, 4)}}
+
+
+def bench_fwd_bwd(raw, cam, steps, warmup):
+    """One differentiable training step on the headline scene: forward (unclamped float image), L1
+    against a target rendered after perturbing means by N(0, 0.01^2) and SH DC by N(0, 0.05^2),
+    backward to every raw parameter. Timed over `steps` steps between device synchronizations."""
+    import torch
+
+    from aniso_amd import scenes
+    from aniso_amd.autograd import rasterize
+
+    dev = "cuda"
+    t = lambda a, g=False: torch.tensor(np.ascontiguousarray(a), dtype=torch.float32, device=dev, requires_grad=g)
+    params = [t(a, True) for a in (raw.means, raw.log_scales, raw.quats, raw.opacity_logits, raw.sh)]
+    rng = np.random.default_rng(0)
+    tm = raw.means + rng.normal(0, 0.01, raw.means.shape).astype(np.float32)
+    tsh = raw.sh.copy()
+    tsh[:, 0, :] += rng.normal(0, 0.05, tsh[:, 0, :].shape).astype(np.float32)
+    with torch.no_grad():
+        target = rasterize(t(tm), t(raw.log_scales), t(raw.quats), t(raw.opacity_logits), t(tsh), cam)
+
+    def step():
+        for p in params:
+            p.grad = None
+        img = rasterize(*params, cam)
+        loss = (img - target).abs().mean()
+        loss.backward()
+        return loss
+
+    for _ in range(max(1, warmup)):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    return {"ms_per_step": round(ms, 4), "steps": steps, "loss": "L1 mean vs perturbed-scene target",
+            "includes": "forward + loss + backward to means, log-scales, quats, opacity logits, SH"}
 
 
 def cpu_baseline(raw, cam, seconds):
