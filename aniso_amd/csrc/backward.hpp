@@ -3,62 +3,107 @@
 // forward these differentiate is the reference's (src/render.cpp:39-104, :193-221).
 //
 // Gradients are reduced without atomics: every (tile, Gaussian) pair is owned by one workgroup,
-// which sums its pixels' contributions in a fixed order and writes one 9-float partial per pair
-// into the pair's slot in the unsorted depth-rank order; each Gaussian's slots are contiguous
-// ([pairStart, pairStart + tiles touched)), so one lane per Gaussian sums them in order.
+// which sums its pixels' contributions in a fixed order and writes one partial record per pair
+// into the pair's slot; slots are numbered in Gaussian index order, each Gaussian's tiles
+// row-major ([slotStart, slotStart + tiles touched)), so the per-Gaussian kernel reads one
+// contiguous run per workgroup and sums each Gaussian's records in order.
 #pragma once
 
 #include "forward.hpp"
 
 namespace ans {
 
-template <int CTRL, int ROW_MASK, int BANK_MASK>
+template <int CTRL>
 __device__ inline float dppAdd(float v) {
-    const int moved = __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, BANK_MASK, false);
-    return v + __int_as_float(moved);
+    return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
 }
 
-// Sum over the 64 lanes, valid in lane 63: inclusive scan within rows of 16 (row_shr 1, 2, 4, 8),
-// then row_bcast:15 and row_bcast:31 carry the row totals up.
-__device__ inline float waveSumLast(float v) {
-    v = dppAdd<0x111, 0xf, 0xf>(v);
-    v = dppAdd<0x112, 0xf, 0xf>(v);
-    v = dppAdd<0x114, 0xf, 0xe>(v);
-    v = dppAdd<0x118, 0xf, 0xc>(v);
-    v = dppAdd<0x142, 0xa, 0xf>(v);
-    v = dppAdd<0x143, 0xc, 0xf>(v);
-    return v;
+__device__ inline float addSwapped(unsigned a, unsigned b) { return __uint_as_float(a) + __uint_as_float(b); }
+
+// Sum over the 64 lanes, in every lane: a butterfly of full-mask DPP adds within each 16-lane
+// row (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror, row_mirror), then gfx950's
+// permlane16/permlane32 swaps across rows. Lane 63 is read by the callers; the order of the
+// additions is fixed, so the result is deterministic.
+__device__ inline float waveSum(float v) {
+    v = dppAdd<0xB1>(v);
+    v = dppAdd<0x4E>(v);
+    v = dppAdd<0x141>(v);
+    v = dppAdd<0x140>(v);
+    const auto r16 = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    v = addSwapped(r16[0], r16[1]);
+    const auto r32 = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return addSwapped(r32[0], r32[1]);
 }
 
-constexpr int kMaxCheckpoints = 16;
+// Sums over the 64 lanes of two splats' 9 gradient components at once (a reduce-scatter):
+// permlane32_swap leaves lanes 0-31 holding splat 1's pairwise sums and lanes 32-63 splat 2's;
+// permlane16_swap pairs components (k, k + 5) so that rows 0..3 of register k hold
+// (splat 1, k), (splat 1, k + 5), (splat 2, k), (splat 2, k + 5); a row butterfly of DPP adds
+// (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror, row_mirror) completes each row.
+// Component 4 has no partner: its rows 1 and 3 repeat rows 0 and 2. The last lane of each row
+// then writes its value to out[(splat 1 or 2) row base + k (+ 5)], where the caller's row of 10
+// floats takes component 4's repeat in slot 9. The order of additions is fixed.
+__device__ inline void pairSumStore(float (&d1)[9], float (&d2)[9], float* rowBase) {
+    float t[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+        const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(d1[c]), __float_as_uint(d2[c]), false, false);
+        t[c] = addSwapped(r[0], r[1]);
+    }
+    float u[5];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(t[k]), __float_as_uint(t[k + 5]), false, false);
+        u[k] = addSwapped(r[0], r[1]);
+    }
+    {
+        const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(t[4]), __float_as_uint(t[4]), false, false);
+        u[4] = addSwapped(r[0], r[1]);
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) u[k] = dppAdd<0xB1>(u[k]);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) u[k] = dppAdd<0x4E>(u[k]);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) u[k] = dppAdd<0x141>(u[k]);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) u[k] = dppAdd<0x140>(u[k]);
+    if ((__lane_id() & 15) == 15) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) rowBase[k] = u[k];
+    }
+}
 
-// Adjoint of k_blend for one tile. Pass 1 re-runs the front-to-back walk to find each pixel's last
-// contributing splat and its final transmittance, recording the transmittance at the start of
-// every ckEvery-th batch in LDS. Pass 2 walks the batches back to front: within a batch,
-// T_i = T_{i+1} / (1 - alpha_i) (the same alpha the forward computed, so 1 - alpha is the same
-// float), and at each checkpointed batch start T is reset to the recorded forward value, so
-// rounding cannot accumulate across batches. Per contributing splat i of a pixel with image
-// gradient g = dL/dC:
+constexpr int kPartialStride = 12;  // floats per pair record: u v A B C opacity r g b, 3 pad
+
+// Adjoint of k_blend for one tile, one lane per pixel as in the forward. The forward left, per
+// pixel, its final transmittance T and the list position of the last splat it blended; the walk
+// goes back to front over the batches up to the tile's furthest such position, recovering each
+// splat's transmittance as T_i = T_{i+1} / (1 - alpha_i) from the same alpha the forward
+// computed. Per contributing splat i of a pixel with image gradient g = dL/dC:
 //   dL/drgb_i = g alpha_i T_i,  dL/dalpha_i = T_i g . (rgb_i - accum),  accum <- alpha_i rgb_i +
 //   (1 - alpha_i) accum  (accum = colour behind i per unit of the transmittance after i);
 //   alpha = o G (no gradient where the 0.99 clamp is active), G = exp(power),
 //   power = -(A dx^2 + C dy^2) / 2 - B dx dy, dx = px - u, dy = py - v.
-// partial[slot * 9 + {u, v, A, B, C, opacity, r, g, b}].
+// Each wavefront visits only the splats whose culling test (the forward's) admits its 8 x 8
+// block, two at a time: pairSumStore reduces the two splats' per-pixel contributions over the
+// wavefront, and the wavefronts' sums are added in fixed order into one partial record per pair:
+// partial[slot * 12 + {u, v, A, B, C, opacity, r, g, b}].
 template <int TW, int TH, int BATCH>
 __global__ __launch_bounds__(TW* TH) void k_blendBackward(
     const uint2* __restrict__ ranges, const uint32_t* __restrict__ sortedGauss, const float4* __restrict__ rec0,
     const float4* __restrict__ rec1, const float4* __restrict__ rec2, const uint2* __restrict__ rect,
-    const uint32_t* __restrict__ pairStart, int width, int height, int tilesX, const float* __restrict__ gradImage,
-    float* __restrict__ partial, const FrameCtl* __restrict__ ctl) {
+    const uint32_t* __restrict__ slotStart, const uint2* __restrict__ pixState, int width, int height, int tilesX,
+    const float* __restrict__ gradImage, float* __restrict__ partial, const FrameCtl* __restrict__ ctl) {
     constexpr int kThreads = TW * TH;
     constexpr int kWaves = kThreads / kWave;
     constexpr int kWavesX = TW / 8;
     __shared__ float4 s0[BATCH], s1[BATCH], s2[BATCH];
-    __shared__ uint32_t sGauss[BATCH];
     __shared__ uint8_t sMask[BATCH];
     __shared__ uint8_t sList[kWaves][BATCH];
-    __shared__ float sGrad[kWaves][BATCH][9];
-    __shared__ float sCk[kMaxCheckpoints][kThreads];
+    __shared__ uint32_t sSlot[BATCH];
+    __shared__ float sGrad[kWaves][BATCH + 1][10];  // row BATCH: the partner of an unpaired splat
+    __shared__ uint32_t sMaxLast;
 
     if (ctl->overflow) return;
     const int tile = blockIdx.x;
@@ -73,11 +118,9 @@ __global__ __launch_bounds__(TW* TH) void k_blendBackward(
 
     const uint2 range = ranges[tile];
     const uint32_t len = range.y - range.x;
-    const int nBatches = static_cast<int>((len + BATCH - 1) / BATCH);
-    const int ckEvery = max(1, (nBatches + kMaxCheckpoints - 1) / kMaxCheckpoints);
 
-    // Stages batch b: records, Gaussian ids and the per-wavefront culling mask; then this
-    // wavefront's list of splats to visit. Returns the number of splats in the batch.
+    // Stages batch b: records, each pair's partial slot and the per-wavefront culling mask; then
+    // this wavefront's list of splats to visit. Returns the number of splats in the batch.
     auto stage = [&](int b, int& count) -> int {
         const uint32_t start = range.x + static_cast<uint32_t>(b) * BATCH;
         const int cnt = min(static_cast<uint32_t>(BATCH), range.y - start);
@@ -85,21 +128,24 @@ __global__ __launch_bounds__(TW* TH) void k_blendBackward(
             const uint32_t g = sortedGauss[start + k];
             const float4 a = rec0[g];
             const float4 bb = rec1[g];
+            const uint2 rp = rect[g];
+            const uint32_t first = slotStart[g];
             s0[k] = a;
             s1[k] = bb;
             s2[k] = rec2[g];
-            sGauss[k] = g;
+            const uint32_t x0 = rp.x & 0xffffu, y0 = rp.x >> 16, w = (rp.y & 0xffffu) - x0 + 1;
+            sSlot[k] = first + (static_cast<uint32_t>(tileY) - y0) * w + (static_cast<uint32_t>(tileX) - x0);
             uint32_t m = 0;
             if (bb.y >= 1.0f / 255.0f) {
                 const float kq = fmaxf(2.0f * __logf(255.0f * bb.y), 0.0f) * 1.05f + 0.1f;
                 const float rB_C = -a.w / bb.x, rB_A = -a.w / a.z;
 #pragma unroll
-                for (int w = 0; w < kWaves; ++w) {
-                    const float x0 = tileX * TW + (w % kWavesX) * 8 + 0.5f;
-                    const float y0 = tileY * TH + (w / kWavesX) * 8 + 0.5f;
-                    const float dx0 = x0 - a.x, dy0 = y0 - a.y;
+                for (int wv = 0; wv < kWaves; ++wv) {
+                    const float x0f = tileX * TW + (wv % kWavesX) * 8 + 0.5f;
+                    const float y0f = tileY * TH + (wv / kWavesX) * 8 + 0.5f;
+                    const float dx0 = x0f - a.x, dy0 = y0f - a.y;
                     if (quadMinOverBox(a.z, a.w, bb.x, rB_C, rB_A, dx0, dx0 + 7.0f, dy0, dy0 + 7.0f) <= kq)
-                        m |= 1u << w;
+                        m |= 1u << wv;
                 }
             }
             sMask[k] = static_cast<uint8_t>(m);
@@ -117,41 +163,22 @@ __global__ __launch_bounds__(TW* TH) void k_blendBackward(
         return cnt;
     };
 
-    // Pass 1: forward walk.
+    // The forward's per-pixel end state, and the furthest last-blended position in the tile.
     float T = 1.0f;
-    bool done = !inside;
-    uint32_t lastContrib = 0;  // list position + 1 of this pixel's last contributing splat
-    int nbDone = 0;
-    for (int b = 0; b < nBatches; ++b) {
-        if (__syncthreads_count(done) == kThreads) break;
-        if (b % ckEvery == 0) sCk[b / ckEvery][tid] = T;
-        int count;
-        stage(b, count);
-        nbDone = b + 1;
-        if (__ballot(!done) == 0) continue;
-        for (int i = 0; i < count; ++i) {
-            const int j = sList[wave][i];
-            const float4 a = s0[j];
-            const float4 bb = s1[j];
-            if (!done) {
-                const float dx = px - a.x, dy = py - a.y;
-                const float power = -0.5f * (a.z * dx * dx + bb.x * dy * dy) - a.w * dx * dy;
-                if (!(power > 0.0f)) {
-                    const float alpha = fminf(0.99f, bb.y * __expf(power));
-                    if (alpha >= 1.0f / 255.0f) {
-                        const float nextT = T * (1.0f - alpha);
-                        if (nextT < 1e-4f) {
-                            done = true;
-                        } else {
-                            T = nextT;
-                            lastContrib = static_cast<uint32_t>(b) * BATCH + j + 1;
-                        }
-                    }
-                }
-            }
-            if ((i & 7) == 7 && __ballot(!done) == 0) break;
-        }
+    uint32_t lastContrib = 0;  // list position + 1 of this pixel's last blended splat
+    if (inside) {
+        const uint2 st = pixState[static_cast<size_t>(y) * width + x];
+        T = __uint_as_float(st.x);
+        lastContrib = st.y;
     }
+    if (tid == 0) sMaxLast = 0;
+    uint32_t m = lastContrib;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) m = max(m, static_cast<uint32_t>(__shfl_xor(static_cast<int>(m), d, kWave)));
+    __syncthreads();
+    if (lane == 0) atomicMax(&sMaxLast, m);
+    __syncthreads();
+    const int nbDone = static_cast<int>((sMaxLast + BATCH - 1) / BATCH);
 
     // Pass 2: reverse walk.
     float gr = 0.0f, gg = 0.0f, gb = 0.0f;
@@ -163,75 +190,82 @@ __global__ __launch_bounds__(TW* TH) void k_blendBackward(
     }
     float ar = 0.0f, ag = 0.0f, ab = 0.0f;  // accum
     float Tcur = T;
+    // This pixel's gradient contributions from splat j of batch b (zero if it does not blend
+    // there); steps T and accum back past it. True if it contributed.
+    auto visit = [&](int b, int j, float (&d)[9]) -> bool {
+#pragma unroll
+        for (int c = 0; c < 9; ++c) d[c] = 0.0f;
+        if (static_cast<uint32_t>(b) * BATCH + j >= lastContrib) return false;
+        const float4 a = s0[j];
+        const float4 bb = s1[j];
+        const float dx = px - a.x, dy = py - a.y;
+        const float power = -0.5f * (a.z * dx * dx + bb.x * dy * dy) - a.w * dx * dy;
+        if (power > 0.0f) return false;
+        const float G = __expf(power);
+        const float raw = bb.y * G;
+        const float alpha = fminf(0.99f, raw);
+        if (!(alpha >= 1.0f / 255.0f)) return false;
+        const float4 c = s2[j];
+        const float oneMinus = 1.0f - alpha;
+        const float Ti = Tcur * __builtin_amdgcn_rcpf(oneMinus);
+        const float w = alpha * Ti;
+        d[6] = gr * w;
+        d[7] = gg * w;
+        d[8] = gb * w;
+        const float gAlpha = Ti * (gr * (c.x - ar) + gg * (c.y - ag) + gb * (c.z - ab));
+        ar = alpha * c.x + oneMinus * ar;
+        ag = alpha * c.y + oneMinus * ag;
+        ab = alpha * c.z + oneMinus * ab;
+        if (raw <= 0.99f) {
+            d[5] = gAlpha * G;
+            const float gPow = gAlpha * raw;
+            d[0] = gPow * (a.z * dx + a.w * dy);
+            d[1] = gPow * (bb.x * dy + a.w * dx);
+            d[2] = gPow * (-0.5f * dx * dx);
+            d[3] = gPow * (-dx * dy);
+            d[4] = gPow * (-0.5f * dy * dy);
+        }
+        Tcur = Ti;
+        return true;
+    };
+    const int rowHalf = (lane >> 4) & 1;  // rows 1 and 3 write components 5..9
+    const bool secondSplat = lane >= 32;  // rows 2 and 3 write the second splat
     for (int b = nbDone - 1; b >= 0; --b) {
         __syncthreads();
-        for (int k = tid; k < kWaves * BATCH * 9; k += kThreads) (&sGrad[0][0][0])[k] = 0.0f;
+        for (int k = tid; k < kWaves * (BATCH + 1) * 10; k += kThreads) (&sGrad[0][0][0])[k] = 0.0f;
         int count;
         const int cnt = stage(b, count);
         if (__ballot(lastContrib > static_cast<uint32_t>(b) * BATCH) != 0) {
-            for (int i = count - 1; i >= 0; --i) {
-                const int j = sList[wave][i];
-                const float4 a = s0[j];
-                const float4 bb = s1[j];
-                float d[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-                bool contrib = false;
-                if (static_cast<uint32_t>(b) * BATCH + j < lastContrib) {
-                    const float dx = px - a.x, dy = py - a.y;
-                    const float power = -0.5f * (a.z * dx * dx + bb.x * dy * dy) - a.w * dx * dy;
-                    if (!(power > 0.0f)) {
-                        const float G = __expf(power);
-                        const float raw = bb.y * G;
-                        const float alpha = fminf(0.99f, raw);
-                        if (alpha >= 1.0f / 255.0f) {
-                            contrib = true;
-                            const float4 c = s2[j];
-                            const float Ti = Tcur / (1.0f - alpha);
-                            const float w = alpha * Ti;
-                            d[6] = gr * w;
-                            d[7] = gg * w;
-                            d[8] = gb * w;
-                            const float gAlpha = Ti * (gr * (c.x - ar) + gg * (c.y - ag) + gb * (c.z - ab));
-                            ar = alpha * c.x + (1.0f - alpha) * ar;
-                            ag = alpha * c.y + (1.0f - alpha) * ag;
-                            ab = alpha * c.z + (1.0f - alpha) * ab;
-                            if (raw <= 0.99f) {
-                                d[5] = gAlpha * G;
-                                const float gPow = gAlpha * raw;
-                                d[0] = gPow * (a.z * dx + a.w * dy);
-                                d[1] = gPow * (bb.x * dy + a.w * dx);
-                                d[2] = gPow * (-0.5f * dx * dx);
-                                d[3] = gPow * (-dx * dy);
-                                d[4] = gPow * (-0.5f * dy * dy);
-                            }
-                            Tcur = Ti;
-                        }
-                    }
-                }
-                if (__ballot(contrib) != 0) {
+            for (int i = count - 1; i >= 0; i -= 2) {
+                const int j1 = sList[wave][i];
+                const int j2 = i >= 1 ? static_cast<int>(sList[wave][i - 1]) : BATCH;
+                float d1[9], d2[9];
+                bool any = visit(b, j1, d1);
+                if (i >= 1) {
+                    any |= visit(b, j2, d2);
+                } else {
 #pragma unroll
-                    for (int c = 0; c < 9; ++c) {
-                        const float s = waveSumLast(d[c]);
-                        if (lane == kWave - 1) sGrad[wave][j][c] = s;
-                    }
+                    for (int c = 0; c < 9; ++c) d2[c] = 0.0f;
                 }
+                if (__ballot(any) != 0)
+                    pairSumStore(d1, d2, &sGrad[wave][secondSplat ? j2 : j1][rowHalf * 5]);
             }
         }
-        if (b % ckEvery == 0) Tcur = sCk[b / ckEvery][tid];
         __syncthreads();
         // One partial per pair of this batch: the wavefronts' sums in fixed order.
         for (int k = tid; k < cnt; k += kThreads) {
-            const uint32_t g = sGauss[k];
-            const uint2 rp = rect[g];
-            const uint32_t x0 = rp.x & 0xffffu, y0 = rp.x >> 16, w = (rp.y & 0xffffu) - x0 + 1;
-            const uint32_t slot = pairStart[g] + (static_cast<uint32_t>(tileY) - y0) * w + (static_cast<uint32_t>(tileX) - x0);
-            float* out = partial + static_cast<size_t>(slot) * 9;
+            float s[12];
 #pragma unroll
             for (int c = 0; c < 9; ++c) {
-                float s = 0.0f;
+                s[c] = 0.0f;
 #pragma unroll
-                for (int wv = 0; wv < kWaves; ++wv) s += sGrad[wv][k][c];
-                out[c] = s;
+                for (int wv = 0; wv < kWaves; ++wv) s[c] += sGrad[wv][k][c];
             }
+            s[9] = s[10] = s[11] = 0.0f;
+            float4* out = reinterpret_cast<float4*>(partial + static_cast<size_t>(sSlot[k]) * kPartialStride);
+            out[0] = make_float4(s[0], s[1], s[2], s[3]);
+            out[1] = make_float4(s[4], s[5], s[6], s[7]);
+            out[2] = make_float4(s[8], s[9], s[10], s[11]);
         }
     }
     // Pairs after the point where every pixel had stopped get zero partials.
@@ -239,10 +273,9 @@ __global__ __launch_bounds__(TW* TH) void k_blendBackward(
         const uint32_t g = sortedGauss[range.x + k];
         const uint2 rp = rect[g];
         const uint32_t x0 = rp.x & 0xffffu, y0 = rp.x >> 16, w = (rp.y & 0xffffu) - x0 + 1;
-        const uint32_t slot = pairStart[g] + (static_cast<uint32_t>(tileY) - y0) * w + (static_cast<uint32_t>(tileX) - x0);
-        float* out = partial + static_cast<size_t>(slot) * 9;
-#pragma unroll
-        for (int c = 0; c < 9; ++c) out[c] = 0.0f;
+        const uint32_t slot = slotStart[g] + (static_cast<uint32_t>(tileY) - y0) * w + (static_cast<uint32_t>(tileX) - x0);
+        float4* out = reinterpret_cast<float4*>(partial + static_cast<size_t>(slot) * kPartialStride);
+        out[0] = out[1] = out[2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
 }
 
@@ -278,7 +311,7 @@ __host__ __device__ inline void shBasisGrad(int degree, float x, float y, float 
 // Adjoint of projectGaussian (raw parameters): from gs = dL/d{u, v, A, B, C, opacity, r, g, b}
 // to the gradients of the mean (gm), log-scales (gls), raw quaternion (gq), opacity logit and the
 // SH coefficients (gsh, [coefficient][channel], zero past the evaluated degree). shi holds the
-// Gaussian's SH, [coefficient][channel].
+// Gaussian's SH, [coefficient][channel]; gsh may be the same array.
 __host__ __device__ inline void gaussAdjoint(const GaussProj& g, const CamParams& cam, const float gs[9],
                                              const float* shi, float gm[3], float gls[3], float gq[4], float* glogit,
                                              float* gsh) {
@@ -354,8 +387,8 @@ __host__ __device__ inline void gaussAdjoint(const GaussProj& g, const CamParams
         if (k < g.shCount) {
             float gbk = 0.0f;
             for (int c = 0; c < 3; ++c) {
+                gbk += grgb[c] * shi[k * 3 + c];  // read before the write below: gsh may alias shi
                 gsh[k * 3 + c] = grgb[c] * g.basis[k];
-                gbk += grgb[c] * shi[k * 3 + c];
             }
             for (int j = 0; j < 3; ++j) gdir[j] += gbk * bgrad[k][j];
         }
@@ -366,44 +399,165 @@ __host__ __device__ inline void gaussAdjoint(const GaussProj& g, const CamParams
     for (int j = 0; j < 3; ++j) gm[j] += (gdir[j] - dd * g.dir[j]) / g.dlen;
 }
 
-// Per Gaussian: sums its pair partials in slot order, then runs gaussAdjoint to the gradients of
-// means, log-scales, quaternions (w, x, y, z), opacity logits and SH [coefficient][channel].
-// Culled Gaussians get zero gradients.
-static __global__ __launch_bounds__(256) void k_gaussBackward(int n, const float* __restrict__ pos,
-                                                              const float* __restrict__ logScale,
-                                                              const float4* __restrict__ quat,
-                                                              const float* __restrict__ logit,
-                                                              const float* __restrict__ sh, CamParams cam,
-                                                              const uint32_t* __restrict__ tilesTouched,
-                                                              const uint32_t* __restrict__ pairStart,
-                                                              const float* __restrict__ partial,
-                                                              float* __restrict__ gMeans, float* __restrict__ gScales,
-                                                              float* __restrict__ gQuats, float* __restrict__ gLogit,
-                                                              float* __restrict__ gSh,
-                                                              const FrameCtl* __restrict__ ctl) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n || ctl->overflow) return;
-    const int K3 = cam.shCoeffs * 3;
-    float* gshi = gSh + static_cast<size_t>(i) * K3;
-    const uint32_t tt = tilesTouched[i];
-    GaussProj g;
-    if (tt == 0 || !projectGaussian<false, true>(i, n, pos, logScale, quat, logit, sh, cam, g)) {
-        for (int j = 0; j < 3; ++j) gMeans[3 * i + j] = 0.0f, gScales[3 * i + j] = 0.0f;
-        for (int j = 0; j < 4; ++j) gQuats[4 * i + j] = 0.0f;
-        gLogit[i] = 0.0f;
-        for (int j = 0; j < K3; ++j) gshi[j] = 0.0f;
-        return;
-    }
-    float gs[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    const float* src = partial + static_cast<size_t>(pairStart[i]) * 9;
-    for (uint32_t k = 0; k < tt; ++k)
+// Pair slots in Gaussian index order: per block of 256 Gaussians, the sum of their tile counts.
+static __global__ __launch_bounds__(256) void k_slotBlockSums(uint32_t n, const uint32_t* __restrict__ tilesTouched,
+                                                              uint32_t* __restrict__ blockSum) {
+    __shared__ uint32_t sTmp[256 / kWave];
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    uint32_t total;
+    blockExclusiveScan<256>(i < n ? tilesTouched[i] : 0u, sTmp, &total);
+    if (threadIdx.x == 0) blockSum[blockIdx.x] = total;
+}
+
+// One workgroup: exclusive scan of the per-block sums in place.
+static __global__ __launch_bounds__(256) void k_scanBlockSums(uint32_t* __restrict__ blockSum, uint32_t blocks) {
+    __shared__ uint32_t sTmp[256 / kWave];
+    constexpr int kPer = 8;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < blocks; base += 256 * kPer) {
+        const uint32_t first = base + threadIdx.x * kPer;
+        uint32_t v[kPer], sum = 0;
 #pragma unroll
-        for (int c = 0; c < 9; ++c) gs[c] += src[9 * k + c];
-    float gm[3], gls[3], gq[4], gl;
-    gaussAdjoint(g, cam, gs, sh + static_cast<size_t>(i) * K3, gm, gls, gq, &gl, gshi);
-    for (int j = 0; j < 3; ++j) gMeans[3 * i + j] = gm[j], gScales[3 * i + j] = gls[j];
-    for (int j = 0; j < 4; ++j) gQuats[4 * i + j] = gq[j];
-    gLogit[i] = gl;
+        for (int k = 0; k < kPer; ++k) {
+            v[k] = first + k < blocks ? blockSum[first + k] : 0u;
+            sum += v[k];
+        }
+        uint32_t total;
+        uint32_t run = carry + blockExclusiveScan<256>(sum, sTmp, &total);
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            if (first + k < blocks) blockSum[first + k] = run;
+            run += v[k];
+        }
+        carry += total;
+    }
+}
+
+// slotStart[i]: the first pair slot of Gaussian i (exclusive scan of tile counts in index order).
+static __global__ __launch_bounds__(256) void k_slotStart(uint32_t n, const uint32_t* __restrict__ tilesTouched,
+                                                          const uint32_t* __restrict__ blockBase,
+                                                          uint32_t* __restrict__ slotStart) {
+    __shared__ uint32_t sTmp[256 / kWave];
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    uint32_t total;
+    const uint32_t excl = blockExclusiveScan<256>(i < n ? tilesTouched[i] : 0u, sTmp, &total);
+    if (i < n) slotStart[i] = blockBase[blockIdx.x] + excl;
+}
+
+constexpr int kShRow = 49;  // LDS row stride of one Gaussian's SH (odd: row-per-lane reads are conflict-free)
+
+// Row of flat element e in rows of k3 floats, e < 64 * 48: (e + 0.5) / k3 lies at least 0.5 / 48
+// from an integer and the float product is within 2e-4 of it, so truncation gives e / k3.
+__device__ inline int rowOf(int e, float rcpK3) {
+    return static_cast<int>((static_cast<float>(e) + 0.5f) * rcpK3);
+}
+
+// Per Gaussian, one wavefront per 64 consecutive Gaussians (index order). The wavefront's pair
+// records are one contiguous run [slotStart of its first Gaussian, end of its last); it streams
+// them 64 at a time through LDS (each lane loads one 48-byte record, the next chunk prefetched
+// while the current one is summed) and each lane sums its own Gaussian's records in slot order.
+// The SH of the 64 Gaussians is one contiguous block: it is loaded coalesced (every load issued
+// up front), transposed through LDS to one row per lane, and the SH gradients go back the same
+// way. gaussAdjoint then gives the gradients of means, log-scales, quaternions (w, x, y, z),
+// opacity logits and SH [coefficient][channel]; the summed splat-record gradient goes to
+// splatGrad. Culled Gaussians get zero gradients.
+static __global__ __launch_bounds__(kWave) void k_gaussBackward(
+    int n, const float* __restrict__ pos, const float* __restrict__ logScale, const float4* __restrict__ quat,
+    const float* __restrict__ logit, const float* __restrict__ sh, CamParams cam,
+    const uint32_t* __restrict__ tilesTouched, const uint32_t* __restrict__ slotStart,
+    const float* __restrict__ partial, float* __restrict__ splatGrad, float* __restrict__ gMeans,
+    float* __restrict__ gScales, float* __restrict__ gQuats, float* __restrict__ gLogit, float* __restrict__ gSh,
+    const FrameCtl* __restrict__ ctl) {
+    __shared__ float4 sRec[kWave][3];
+    __shared__ float sSh[kWave * kShRow];
+    if (ctl->overflow) return;
+    const int lane = threadIdx.x;
+    const int g0 = blockIdx.x * kWave;
+    const int rows = min(kWave, n - g0);
+    const int i = g0 + lane;
+    const bool valid = lane < rows;
+    const uint32_t tt = valid ? tilesTouched[i] : 0u;
+    const uint32_t s0 = valid ? slotStart[i] : 0u;
+    const int K3 = cam.shCoeffs * 3;  // at most 48
+    const int total = rows * K3;
+    const float* src = sh + static_cast<size_t>(g0) * K3;
+    float v[48];
+#pragma unroll
+    for (int j = 0; j < 48; ++j) {
+        const int e = lane + j * kWave;
+        v[j] = e < total ? src[e] : 0.0f;
+    }
+    const uint32_t r0 = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(s0)));
+    const uint32_t r1 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(s0 + tt), rows - 1));
+    const float4* rec = reinterpret_cast<const float4*>(partial);
+    const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float4 a0 = zero, a1 = zero, a2 = zero;
+    if (r0 + lane < r1) {
+        const size_t r = static_cast<size_t>(r0 + lane) * 3;
+        a0 = rec[r];
+        a1 = rec[r + 1];
+        a2 = rec[r + 2];
+    }
+    const float rcpK3 = 1.0f / static_cast<float>(K3);
+#pragma unroll
+    for (int j = 0; j < 48; ++j) {
+        const int e = lane + j * kWave;
+        if (e < total) {
+            const int row = rowOf(e, rcpK3);
+            sSh[row * kShRow + (e - row * K3)] = v[j];
+        }
+    }
+
+    float gs[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t c0 = r0; c0 < r1; c0 += kWave) {
+        waveSync();
+        sRec[lane][0] = a0;
+        sRec[lane][1] = a1;
+        sRec[lane][2] = a2;
+        const uint32_t nx = c0 + kWave + lane;
+        if (nx < r1) {
+            const size_t r = static_cast<size_t>(nx) * 3;
+            a0 = rec[r];
+            a1 = rec[r + 1];
+            a2 = rec[r + 2];
+        }
+        waveSync();
+        const uint32_t a = max(s0, c0), b = min(s0 + tt, c0 + kWave);
+        for (uint32_t k = a; k < b; ++k) {
+            const float4 x = sRec[k - c0][0], y = sRec[k - c0][1], z = sRec[k - c0][2];
+            gs[0] += x.x; gs[1] += x.y; gs[2] += x.z; gs[3] += x.w;
+            gs[4] += y.x; gs[5] += y.y; gs[6] += y.z; gs[7] += y.w;
+            gs[8] += z.x;
+        }
+    }
+    waveSync();
+    float* row = sSh + lane * kShRow;
+    if (valid) {
+        for (int c = 0; c < 9; ++c) splatGrad[static_cast<size_t>(i) * 9 + c] = gs[c];
+        GaussProj g;
+        if (tt > 0 && projectGaussian<false, true>(i, pos, logScale, quat, logit, row, 1, cam, g)) {
+            float gm[3], gls[3], gq[4], gl;
+            gaussAdjoint(g, cam, gs, row, gm, gls, gq, &gl, row);
+            for (int j = 0; j < 3; ++j) gMeans[3 * i + j] = gm[j], gScales[3 * i + j] = gls[j];
+            reinterpret_cast<float4*>(gQuats)[i] = make_float4(gq[0], gq[1], gq[2], gq[3]);
+            gLogit[i] = gl;
+        } else {
+            for (int j = 0; j < 3; ++j) gMeans[3 * i + j] = 0.0f, gScales[3 * i + j] = 0.0f;
+            reinterpret_cast<float4*>(gQuats)[i] = zero;
+            gLogit[i] = 0.0f;
+            for (int j = 0; j < K3; ++j) row[j] = 0.0f;
+        }
+    }
+    waveSync();
+    float* dst = gSh + static_cast<size_t>(g0) * K3;
+#pragma unroll
+    for (int j = 0; j < 48; ++j) {
+        const int e = lane + j * kWave;
+        if (e < total) {
+            const int r = rowOf(e, rcpK3);
+            dst[e] = sSh[r * kShRow + (e - r * K3)];
+        }
+    }
 }
 
 }  // namespace ans

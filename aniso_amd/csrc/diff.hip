@@ -20,21 +20,29 @@ static void launchBlendBackward(Engine& e, FrameState& f, const float* gradImage
     const CamParams& cp = f.cp;
     const uint32_t tiles = static_cast<uint32_t>(cp.tilesX) * cp.tilesY;
     k_blendBackward<TW, TH, kBatch><<<tiles, TW * TH, 0, e.stream>>>(
-        f.ranges.p, f.sortedGauss, f.rec0.p, f.rec1.p, f.rec2.p, f.rect.p, f.pairStart.p, cp.width, cp.height,
-        cp.tilesX, gradImage, e.partial.p, f.ctl.p);
+        f.ranges.p, f.sortedGauss, f.rec0.p, f.rec1.p, f.rec2.p, f.rect.p, e.slotStart.p, f.pixState.p, cp.width,
+        cp.height, cp.tilesX, gradImage, e.partial.p, f.ctl.p);
 }
 
 void Engine::enqueueBackward(FrameState& f, const SceneDev& s, const float* gradImage, float* gMeans,
                              float* gScales, float* gQuats, float* gOpacity, float* gSh) {
-    partial.reserve(std::max<size_t>(f.pairs, 1) * 9);
-    if (f.n > 0) {
-        if (f.opts.tile_w == 8) launchBlendBackward<8, 8>(*this, f, gradImage);
-        else if (f.opts.tile_w == 32) launchBlendBackward<32, 16>(*this, f, gradImage);
-        else launchBlendBackward<16, 16>(*this, f, gradImage);
-        k_gaussBackward<<<static_cast<uint32_t>((f.n + 255) / 256), 256, 0, stream>>>(
-            static_cast<int>(f.n), s.pos, s.scale, s.rot, s.opacity, s.sh, f.cp, f.tilesTouched.p, f.pairStart.p,
-            partial.p, gMeans, gScales, gQuats, gOpacity, gSh, f.ctl.p);
-    }
+    if (f.n == 0) return;
+    partial.reserve(std::max<size_t>(f.pairs, 1) * kPartialStride);
+    splatGrad.reserve(f.n * 9);
+    slotStart.reserve(f.n);
+    const uint32_t n = static_cast<uint32_t>(f.n);
+    const uint32_t grid = (n + 255) / 256;
+    blockPairs.reserve(grid);
+    // Pair slots in Gaussian index order.
+    k_slotBlockSums<<<grid, 256, 0, stream>>>(n, f.tilesTouched.p, blockPairs.p);
+    k_scanBlockSums<<<1, 256, 0, stream>>>(blockPairs.p, grid);
+    k_slotStart<<<grid, 256, 0, stream>>>(n, f.tilesTouched.p, blockPairs.p, slotStart.p);
+    if (f.opts.tile_w == 8) launchBlendBackward<8, 8>(*this, f, gradImage);
+    else if (f.opts.tile_w == 32) launchBlendBackward<32, 16>(*this, f, gradImage);
+    else launchBlendBackward<16, 16>(*this, f, gradImage);
+    k_gaussBackward<<<(n + kWave - 1) / kWave, kWave, 0, stream>>>(static_cast<int>(n), s.pos, s.scale, s.rot, s.opacity, s.sh, f.cp,
+                                                   f.tilesTouched.p, slotStart.p, partial.p, splatGrad.p, gMeans,
+                                                   gScales, gQuats, gOpacity, gSh, f.ctl.p);
     hipCheck(hipGetLastError(), "kernel launch");
 }
 
@@ -136,6 +144,7 @@ int ans_diff_forward(ans_diff* d, const ans_params* params, const ans_camera* ca
         } else {
             f = std::make_unique<FrameState>();
         }
+        f->savePixels = true;
         e.renderSync(*f, s, cp, o, nullptr, image);
         e.fillStats(*f, stats);
         const int64_t id = d->next++;
@@ -154,6 +163,8 @@ int ans_diff_backward(ans_diff* d, int64_t frame, const ans_params* params, cons
         if (!grad_image || !grads || (s.n > 0 && (!grads->means || !grads->log_scales || !grads->quats ||
                                                   !grads->opacity_logits || !grads->sh)))
             throw std::invalid_argument("gradient pointers must not be null");
+        if (reinterpret_cast<uintptr_t>(grads->quats) % 16 != 0)
+            throw std::invalid_argument("grads quats must be 16-byte aligned");
         Engine& e = *d->engine;
         e.stream = static_cast<hipStream_t>(stream);  // NULL: the default stream, like any HIP call
         e.enqueueBackward(f, s, grad_image, grads->means, grads->log_scales, grads->quats, grads->opacity_logits,
@@ -179,20 +190,8 @@ int ans_diff_splat_grads(ans_diff* d, int64_t frame, float* out) {
         if (!out) throw std::invalid_argument("out is null");
         Engine& e = *d->engine;
         hipCheck(hipStreamSynchronize(e.stream), "hipStreamSynchronize");
-        std::vector<uint32_t> tt(f.n), ps(f.n);
-        std::vector<float> part(static_cast<size_t>(f.pairs) * 9);
-        if (f.n) {
-            hipCheck(hipMemcpy(tt.data(), f.tilesTouched.p, 4 * f.n, hipMemcpyDeviceToHost), "hipMemcpy");
-            hipCheck(hipMemcpy(ps.data(), f.pairStart.p, 4 * f.n, hipMemcpyDeviceToHost), "hipMemcpy");
-        }
-        if (!part.empty())
-            hipCheck(hipMemcpy(part.data(), e.partial.p, part.size() * 4, hipMemcpyDeviceToHost), "hipMemcpy");
-        for (size_t i = 0; i < f.n; ++i)
-            for (int c = 0; c < 9; ++c) {
-                float s = 0.0f;
-                for (uint32_t k = 0; k < tt[i]; ++k) s += part[(static_cast<size_t>(ps[i]) + k) * 9 + c];
-                out[9 * i + c] = s;
-            }
+        if (f.n && e.splatGrad.cap >= f.n * 9)
+            hipCheck(hipMemcpy(out, e.splatGrad.p, f.n * 9 * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy");
         return ANS_OK;
     });
 }

@@ -5,6 +5,7 @@
 #include "forward.hpp"
 
 #include <cstddef>
+#include <memory>
 #include <stdexcept>
 #include <string>
 
@@ -59,10 +60,13 @@ struct FrameState {
     DevBuf<FrameCtl> ctl;
     DevBuf<float4> rec0, rec1, rec2;
     DevBuf<uint2> rect;
-    DevBuf<uint32_t> tilesTouched, pairStart;
+    DevBuf<uint32_t> tilesTouched;
+    DevBuf<uint32_t> order;  // Gaussians by depth (rank -> index)
     DevBuf<uint32_t> tileA, gaussA, tileB, gaussB;
     size_t pairCap = 0;
     DevBuf<uint2> ranges;
+    bool savePixels = false;  // the blend records pixState (frames the backward pass will read)
+    DevBuf<uint2> pixState;   // per pixel: final transmittance bits, last blended list position + 1
 
     size_t n = 0, tiles = 0;
     uint32_t visible = 0, pairs = 0;
@@ -90,7 +94,9 @@ struct Engine {
     DevBuf<uint32_t> rowTotal;      // per-digit totals of one pass
     DevBuf<uint32_t> blockPairs;    // pairs per expansion block, scanned in place
     DevBuf<uint32_t> blockVisible;  // visible Gaussians per expansion block
-    DevBuf<float> partial;          // backward: per-pair partial gradients, 9 floats per pair
+    DevBuf<float> partial;          // backward: per-pair partial gradients, 12-float records
+    DevBuf<float> splatGrad;        // backward: per-Gaussian splat-record gradients, 9 floats
+    DevBuf<uint32_t> slotStart;     // backward: first pair slot per Gaussian, index order
     FrameState frame;
 
     Engine();
@@ -115,4 +121,39 @@ struct Engine {
                          float* gQuats, float* gOpacity, float* gSh);
 };
 
+// Runs fn, turning exceptions into the C ABI's status codes and ans_last_error().
+template <class Fn>
+int guarded(Fn&& fn) {
+    try {
+        g_lastError.clear();
+        return fn();
+    } catch (const std::invalid_argument& e) {
+        g_lastError = e.what();
+        return ANS_ERR_ARGUMENT;
+    } catch (const std::exception& e) {
+        g_lastError = e.what();
+        return ANS_ERR_HIP;
+    } catch (...) {
+        g_lastError = "unknown error";
+        return ANS_ERR_HIP;
+    }
+}
+
+struct ViewBatch;  // batched multi-view state (batch.hip)
+struct ViewBatchDeleter {
+    void operator()(ViewBatch* b) const;
+};
+
 }  // namespace ans
+
+// The render handle: the scene in device memory (planar SH), the engine, and the batched
+// multi-view state once used.
+struct ans_renderer {
+    ans::SceneDev scene;
+    ans::DevBuf<float> pos, scale, opacity, sh;
+    ans::DevBuf<float4> rot;
+    ans::DevBuf<uint8_t> image8;
+    ans::DevBuf<float> imagef;
+    std::unique_ptr<ans::Engine> engine;
+    std::unique_ptr<ans::ViewBatch, ans::ViewBatchDeleter> batch;
+};

@@ -129,10 +129,13 @@ struct GaussProj {
 // activated here in FP32. Deliberate deviations shared with the CPU oracle: the near test is
 // !(z >= 0.2) so a NaN depth is culled, and a non-finite centre or a radius of 2^24 pixels or more
 // is culled. Returns false if culled.
+// shBase points at this Gaussian's coefficient 0 (red); coefficient * 3 + channel j sits at
+// shBase[j * shStride] (PLANAR: shStride = n; otherwise 1 and shBase 16-byte aligned).
 template <bool PLANAR, bool RAW>
-__host__ __device__ inline bool projectGaussian(int i, int n, const float* __restrict__ pos, const float* __restrict__ scale,
-                                       const float4* __restrict__ rot, const float* __restrict__ opacity,
-                                       const float* __restrict__ sh, const CamParams& cam, GaussProj& g) {
+__host__ __device__ inline bool projectGaussian(int i, const float* __restrict__ pos, const float* __restrict__ scale,
+                                                const float4* __restrict__ rot, const float* __restrict__ opacity,
+                                                const float* __restrict__ shBase, size_t shStride, const CamParams& cam,
+                                                GaussProj& g) {
     const F3 m = reinterpret_cast<const F3*>(pos)[i];
     g.px = cam.R[0] * m.x + cam.R[1] * m.y + cam.R[2] * m.z + cam.t[0];
     g.py = cam.R[3] * m.x + cam.R[4] * m.y + cam.R[5] * m.z + cam.t[1];
@@ -216,8 +219,7 @@ __host__ __device__ inline bool projectGaussian(int i, int n, const float* __res
     g.dir[1] = g.dir[1] / g.dlen;
     g.dir[2] = g.dir[2] / g.dlen;
     g.shCount = shBasis(cam.shDegree, g.dir[0], g.dir[1], g.dir[2], g.basis);
-    const float* shBase = PLANAR ? sh + i : sh + static_cast<size_t>(i) * cam.shCoeffs * 3;
-    g.rgbRaw = shAccumulate<PLANAR>(shBase, PLANAR ? static_cast<size_t>(n) : 1, g.shCount, g.basis);
+    g.rgbRaw = shAccumulate<PLANAR>(shBase, shStride, g.shCount, g.basis);
     // std::max(0, v) semantics (NaN -> 0).
     g.rgb.x = (0.0f < g.rgbRaw.x) ? g.rgbRaw.x : 0.0f;
     g.rgb.y = (0.0f < g.rgbRaw.y) ? g.rgbRaw.y : 0.0f;
@@ -247,7 +249,9 @@ __global__ __launch_bounds__(256) void k_preprocess(int n, const float* __restri
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     GaussProj g;
-    if (!projectGaussian<PLANAR, RAW>(i, n, pos, scale, rot, opacity, sh, cam, g)) {
+    const float* shBase = PLANAR ? sh + i : sh + static_cast<size_t>(i) * cam.shCoeffs * 3;
+    if (!projectGaussian<PLANAR, RAW>(i, pos, scale, rot, opacity, shBase, PLANAR ? static_cast<size_t>(n) : 1, cam,
+                                      g)) {
         tilesTouched[i] = 0;
         depthKey[i] = 0xffffffffu;
         return;
@@ -273,12 +277,22 @@ __global__ __launch_bounds__(256) void k_preprocess(int n, const float* __restri
 constexpr int kDupBlock = 256;                       // 4 wavefronts x 64 depth ranks
 constexpr int kDupTile = kDupBlock;                  // depth ranks per block
 
+// Batched views (blockIdx.y = view) offset per-Gaussian arrays by view * gaussStride, per-block
+// totals by view * blockStride, pair arrays by view * pairStride and use ctl[view]; a single frame
+// is gridDim.y = 1.
+
 // Pairs and visible Gaussians per block of 256 depth ranks.
 static __global__ __launch_bounds__(kDupBlock) void k_pairTotals(const uint32_t* __restrict__ sortedIdx,
-                                                          const uint32_t* __restrict__ tilesTouched, uint32_t n,
-                                                          uint32_t* __restrict__ blockPairs,
-                                                          uint32_t* __restrict__ blockVisible) {
+                                                                 const uint32_t* __restrict__ tilesTouched, uint32_t n,
+                                                                 uint32_t* __restrict__ blockPairs,
+                                                                 uint32_t* __restrict__ blockVisible,
+                                                                 uint32_t gaussStride, uint32_t blockStride) {
     __shared__ uint32_t sTmp[kDupBlock / kWave];
+    const size_t go = static_cast<size_t>(blockIdx.y) * gaussStride;
+    sortedIdx += go;
+    tilesTouched += go;
+    blockPairs += static_cast<size_t>(blockIdx.y) * blockStride;
+    blockVisible += static_cast<size_t>(blockIdx.y) * blockStride;
     const uint32_t r = blockIdx.x * kDupTile + threadIdx.x;
     const uint32_t c = r < n ? tilesTouched[sortedIdx[r]] : 0u;
     uint32_t total, visible;
@@ -294,9 +308,13 @@ static __global__ __launch_bounds__(kDupBlock) void k_pairTotals(const uint32_t*
 // gpu_render.cu:324-329 reads back to the host) and the visible count, and flags a pair total
 // above the pair buffers' capacity.
 static __global__ __launch_bounds__(kDupBlock) void k_scanPairs(uint32_t* __restrict__ blockPairs,
-                                                         const uint32_t* __restrict__ blockVisible, uint32_t blocks,
-                                                         uint32_t cap, FrameCtl* __restrict__ ctl) {
+                                                                const uint32_t* __restrict__ blockVisible,
+                                                                uint32_t blocks, uint32_t cap,
+                                                                FrameCtl* __restrict__ ctl, uint32_t blockStride) {
     __shared__ uint32_t sTmp[kDupBlock / kWave];
+    blockPairs += static_cast<size_t>(blockIdx.y) * blockStride;
+    blockVisible += static_cast<size_t>(blockIdx.y) * blockStride;
+    ctl += blockIdx.y;
     constexpr int kPer = 8;
     uint64_t carry = 0;
     uint32_t visible = 0;
@@ -347,14 +365,27 @@ __device__ inline int waveInclusiveMax(int v) {
 // pairs (tile id, Gaussian index) contiguously, each Gaussian's tiles row-major (the order
 // gpu_render.cu:155-156 writes them), 64 pairs per step with one pair per lane. A step finds
 // the Gaussian owning each slot by marking the slots where Gaussians start (LDS head flags) and
-// a wavefront max-scan. pairStart[g] records each Gaussian's first slot, the contiguous range
-// the backward pass reduces over.
+// a wavefront max-scan.
+// tilesXView (batched views, else null) holds each view's tile-grid width.
 static __global__ __launch_bounds__(kDupBlock) void k_expandPairs(
     const uint32_t* __restrict__ sortedIdx, const uint32_t* __restrict__ tilesTouched, const uint2* __restrict__ rect,
-    uint32_t n, int tilesX, const uint32_t* __restrict__ blockBase, uint32_t* __restrict__ pairStart,
-    uint32_t* __restrict__ tileOut, uint32_t* __restrict__ gaussOut, const FrameCtl* __restrict__ ctl) {
+    uint32_t n, int tilesX, const uint32_t* __restrict__ blockBase,
+    uint32_t* __restrict__ tileOut, uint32_t* __restrict__ gaussOut, const FrameCtl* __restrict__ ctl,
+    uint32_t gaussStride, uint32_t blockStride, uint32_t pairStride, const int* __restrict__ tilesXView) {
     __shared__ uint32_t sWave[kDupBlock / kWave];
     __shared__ int sHead[kDupBlock / kWave][kWave];
+    {
+        const uint32_t view = blockIdx.y;
+        ctl += view;
+        const size_t go = static_cast<size_t>(view) * gaussStride, po = static_cast<size_t>(view) * pairStride;
+        sortedIdx += go;
+        tilesTouched += go;
+        rect += go;
+        blockBase += static_cast<size_t>(view) * blockStride;
+        tileOut += po;
+        gaussOut += po;
+        if (tilesXView) tilesX = tilesXView[view];
+    }
     if (ctl->overflow) return;
     const int wave = threadIdx.x / kWave, lane = __lane_id();
     const uint32_t r = blockIdx.x * kDupTile + threadIdx.x;
@@ -372,7 +403,6 @@ static __global__ __launch_bounds__(kDupBlock) void k_expandPairs(
     __syncthreads();
     uint32_t out = blockBase[blockIdx.x];
     for (int w = 0; w < wave; ++w) out += sWave[w];
-    if (cnt) pairStart[g] = out + excl;
 
     const uint32_t x0 = rp.x & 0xffffu, y0 = rp.x >> 16;
     const uint32_t w = (rp.y & 0xffffu) - x0 + 1;
@@ -413,7 +443,12 @@ static __global__ __launch_bounds__(kDupBlock) void k_expandPairs(
 // Start and end of each tile's run in the sorted pairs (gpu_render.cu:164-177); ranges were
 // cleared to [0, 0) by the frame init.
 static __global__ __launch_bounds__(256) void k_tileRanges(const uint32_t* __restrict__ sortedTile,
-                                                    const FrameCtl* __restrict__ ctl, uint2* __restrict__ ranges) {
+                                                           const FrameCtl* __restrict__ ctl,
+                                                           uint2* __restrict__ ranges, uint32_t pairStride,
+                                                           uint32_t tileStride) {
+    ctl += blockIdx.y;
+    sortedTile += static_cast<size_t>(blockIdx.y) * pairStride;
+    ranges += static_cast<size_t>(blockIdx.y) * tileStride;
     if (ctl->overflow) return;
     const uint32_t total = ctl->pairs;
     const uint32_t p = blockIdx.x * 256 + threadIdx.x;
@@ -451,13 +486,14 @@ __device__ inline float quadMinOverBox(float A, float B, float C, float rB_C, fl
 // block cannot change any of its pixels, so the block's wavefront never visits it. Each
 // wavefront compacts the indices it must visit with ballots, walks them in order, and stops as
 // soon as all of its pixels are done; the workgroup stops when all wavefronts are.
+//
+// pixState (optional, for the backward pass): per pixel, the final transmittance's bits and the
+// position + 1 in the tile's list of the last splat blended into it (0: none).
 template <int TW, int TH, int BATCH, bool CULL>
-__global__ __launch_bounds__(TW* TH) void k_blend(const uint2* __restrict__ ranges,
-                                                  const uint32_t* __restrict__ sortedGauss,
-                                                  const float4* __restrict__ rec0, const float4* __restrict__ rec1,
-                                                  const float4* __restrict__ rec2, int width, int height, int tilesX,
-                                                  uint8_t* __restrict__ out8, float* __restrict__ outf,
-                                                  const FrameCtl* __restrict__ ctl) {
+__device__ inline void blendTile(int tileX, int tileY, uint2 range, const uint32_t* __restrict__ sortedGauss,
+                                 const float4* __restrict__ rec0, const float4* __restrict__ rec1,
+                                 const float4* __restrict__ rec2, int width, int height, uint8_t* __restrict__ out8,
+                                 float* __restrict__ outf, uint2* __restrict__ pixState) {
     constexpr int kThreads = TW * TH;
     constexpr int kWaves = kThreads / kWave;
     constexpr int kWavesX = TW / 8;
@@ -466,9 +502,6 @@ __global__ __launch_bounds__(TW* TH) void k_blend(const uint2* __restrict__ rang
     __shared__ uint8_t sMask[BATCH];
     __shared__ uint8_t sList[kWaves][BATCH];
 
-    if (ctl->overflow) return;
-    const int tile = blockIdx.x;
-    const int tileX = tile % tilesX, tileY = tile / tilesX;
     const int tid = threadIdx.x;
     const int wave = tid / kWave, lane = __lane_id();
     const int bx = tileX * TW + (wave % kWavesX) * 8, by = tileY * TH + (wave / kWavesX) * 8;
@@ -477,9 +510,9 @@ __global__ __launch_bounds__(TW* TH) void k_blend(const uint2* __restrict__ rang
     const float px = x + 0.5f, py = y + 0.5f;
     const uint64_t below = lanesBelow();
 
-    const uint2 range = ranges[tile];
     float T = 1.0f;
     float cr = 0.0f, cg = 0.0f, cb = 0.0f;
+    uint32_t last = 0;
     bool done = !inside;
     for (uint32_t start = range.x; start < range.y; start += BATCH) {
         if (__syncthreads_count(done) == kThreads) break;
@@ -543,6 +576,7 @@ __global__ __launch_bounds__(TW* TH) void k_blend(const uint2* __restrict__ rang
                             cg = __builtin_fmaf(c.y, w, cg);
                             cb = __builtin_fmaf(c.z, w, cb);
                             T = nextT;
+                            last = start - range.x + static_cast<uint32_t>(j) + 1;
                         }
                     }
                 }
@@ -562,7 +596,21 @@ __global__ __launch_bounds__(TW* TH) void k_blend(const uint2* __restrict__ rang
             outf[3 * p + 1] = cg;
             outf[3 * p + 2] = cb;
         }
+        if (pixState) pixState[p] = make_uint2(__float_as_uint(T), last);
     }
+}
+
+template <int TW, int TH, int BATCH, bool CULL>
+__global__ __launch_bounds__(TW* TH) void k_blend(const uint2* __restrict__ ranges,
+                                                  const uint32_t* __restrict__ sortedGauss,
+                                                  const float4* __restrict__ rec0, const float4* __restrict__ rec1,
+                                                  const float4* __restrict__ rec2, int width, int height, int tilesX,
+                                                  uint8_t* __restrict__ out8, float* __restrict__ outf,
+                                                  uint2* __restrict__ pixState, const FrameCtl* __restrict__ ctl) {
+    if (ctl->overflow) return;
+    const int tile = blockIdx.x;
+    blendTile<TW, TH, BATCH, CULL>(tile % tilesX, tile / tilesX, ranges[tile], sortedGauss, rec0, rec1, rec2, width,
+                                   height, out8, outf, pixState);
 }
 
 }  // namespace ans

@@ -88,7 +88,7 @@ FrameState::FrameState() {
 void FrameState::reserve(size_t gaussians, size_t tileCount) {
     const size_t m = std::max<size_t>(gaussians, 1);
     rec0.reserve(m); rec1.reserve(m); rec2.reserve(m);
-    rect.reserve(m); tilesTouched.reserve(m); pairStart.reserve(m);
+    rect.reserve(m); tilesTouched.reserve(m); order.reserve(m);
     ranges.reserve(std::max<size_t>(tileCount, 1));
 }
 
@@ -166,13 +166,14 @@ void Engine::enqueueFrame(FrameState& f, const SceneDev& s, const CamParams& cp,
         const uint32_t* kin = depthKey.p;
         const uint32_t* vin = nullptr;
         uint32_t* kout[4] = {sortKA.p, sortKB.p, sortKA.p, nullptr};
-        uint32_t* vout[4] = {sortVA.p, sortVB.p, sortVA.p, sortVB.p};
+        uint32_t* vout[4] = {sortVA.p, sortVB.p, sortVA.p, f.order.p};
         for (int pass = 0; pass < 4; ++pass) {
-            k_radixUpsweep<kDepthItems><<<blocks, kSortBlock, 0, stream>>>(kin, nullptr, n, n, 8 * pass, 8, hist.p,
-                                                                           blocks, ctl);
-            k_scanRows<<<kRadix, kSortBlock, 0, stream>>>(hist.p, blocks, blocks, rowTotal.p, ctl);
+            k_radixUpsweep<kDepthItems><<<blocks, kSortBlock, 0, stream>>>(kin, false, n, n, 8 * pass, 8, hist.p,
+                                                                           blocks, ctl, 0, 0);
+            k_scanRows<<<kRadix, kSortBlock, 0, stream>>>(hist.p, blocks, blocks, rowTotal.p, ctl, false, n,
+                                                          kSortBlock * kDepthItems, 0);
             k_radixDownsweep<kDepthItems><<<blocks, kSortBlock, 0, stream>>>(
-                kin, vin, kout[pass], vout[pass], nullptr, n, n, 8 * pass, 8, hist.p, rowTotal.p, blocks, ctl);
+                kin, vin, kout[pass], vout[pass], false, n, n, 8 * pass, 8, hist.p, rowTotal.p, blocks, ctl, 0, 0);
             kin = kout[pass];
             vin = vout[pass];
         }
@@ -187,11 +188,12 @@ void Engine::enqueueFrame(FrameState& f, const SceneDev& s, const CamParams& cp,
     if (n > 0) {
         const uint32_t blocksDup = (n + kDupTile - 1) / kDupTile;
         const uint32_t cap = static_cast<uint32_t>(f.pairCap);
-        k_pairTotals<<<blocksDup, kDupBlock, 0, stream>>>(sortVB.p, f.tilesTouched.p, n, blockPairs.p,
-                                                          blockVisible.p);
-        k_scanPairs<<<1, kDupBlock, 0, stream>>>(blockPairs.p, blockVisible.p, blocksDup, cap, ctl);
-        k_expandPairs<<<blocksDup, kDupBlock, 0, stream>>>(sortVB.p, f.tilesTouched.p, f.rect.p, n, cp.tilesX,
-                                                           blockPairs.p, f.pairStart.p, f.tileA.p, f.gaussA.p, ctl);
+        k_pairTotals<<<blocksDup, kDupBlock, 0, stream>>>(f.order.p, f.tilesTouched.p, n, blockPairs.p,
+                                                          blockVisible.p, 0, 0);
+        k_scanPairs<<<1, kDupBlock, 0, stream>>>(blockPairs.p, blockVisible.p, blocksDup, cap, ctl, 0);
+        k_expandPairs<<<blocksDup, kDupBlock, 0, stream>>>(f.order.p, f.tilesTouched.p, f.rect.p, n, cp.tilesX,
+                                                           blockPairs.p, f.tileA.p, f.gaussA.p, ctl, 0, 0, 0,
+                                                           nullptr);
         const uint32_t blocksPair =
             static_cast<uint32_t>((f.pairCap + kSortBlock * kPairItems - 1) / (kSortBlock * kPairItems));
         uint32_t* tk[2] = {f.tileA.p, f.tileB.p};
@@ -200,17 +202,18 @@ void Engine::enqueueFrame(FrameState& f, const SceneDev& s, const CamParams& cp,
             const int src = pass & 1, dst = src ^ 1;
             const int shift = pass * digitBits;
             const int bits = std::min(digitBits, tileBits - shift);
-            k_radixUpsweep<kPairItems><<<blocksPair, kSortBlock, 0, stream>>>(tk[src], &ctl->pairs, 0, cap, shift,
-                                                                              bits, hist.p, blocksPair, ctl);
-            k_scanRows<<<1u << bits, kSortBlock, 0, stream>>>(hist.p, blocksPair, blocksPair, rowTotal.p, ctl);
+            k_radixUpsweep<kPairItems><<<blocksPair, kSortBlock, 0, stream>>>(tk[src], true, 0, cap, shift, bits,
+                                                                              hist.p, blocksPair, ctl, 0, 0);
+            k_scanRows<<<1u << bits, kSortBlock, 0, stream>>>(hist.p, blocksPair, blocksPair, rowTotal.p, ctl, true,
+                                                              cap, kSortBlock * kPairItems, 0);
             k_radixDownsweep<kPairItems><<<blocksPair, kSortBlock, 0, stream>>>(
-                tk[src], tv[src], tk[dst], tv[dst], &ctl->pairs, 0, cap, shift, bits, hist.p, rowTotal.p,
-                blocksPair, ctl);
+                tk[src], tv[src], tk[dst], tv[dst], true, 0, cap, shift, bits, hist.p, rowTotal.p, blocksPair, ctl,
+                0, 0);
         }
         f.sortedTile = tk[passes & 1];
         f.sortedGauss = tv[passes & 1];
         k_tileRanges<<<static_cast<uint32_t>((f.pairCap + 255) / 256), 256, 0, stream>>>(f.sortedTile, ctl,
-                                                                                          f.ranges.p);
+                                                                                          f.ranges.p, 0, 0);
     }
     hipCheck(hipEventRecord(ev[2], stream), "hipEventRecord");
     launchBlend(f, out8, outf);
@@ -222,14 +225,19 @@ template <int TW, int TH, int BATCH>
 void Engine::launchBlendB(FrameState& f, bool cull, uint8_t* out8, float* outf) {
     const CamParams& cp = f.cp;
     const uint32_t tiles = static_cast<uint32_t>(cp.tilesX) * cp.tilesY;
+    uint2* pix = nullptr;
+    if (f.savePixels) {
+        f.pixState.reserve(static_cast<size_t>(cp.width) * cp.height);
+        pix = f.pixState.p;
+    }
     if (cull)
         k_blend<TW, TH, BATCH, true><<<tiles, TW * TH, 0, stream>>>(f.ranges.p, f.sortedGauss, f.rec0.p, f.rec1.p,
                                                                      f.rec2.p, cp.width, cp.height, cp.tilesX, out8,
-                                                                     outf, f.ctl.p);
+                                                                     outf, pix, f.ctl.p);
     else
         k_blend<TW, TH, BATCH, false><<<tiles, TW * TH, 0, stream>>>(f.ranges.p, f.sortedGauss, f.rec0.p, f.rec1.p,
                                                                       f.rec2.p, cp.width, cp.height, cp.tilesX, out8,
-                                                                      outf, f.ctl.p);
+                                                                      outf, pix, f.ctl.p);
 }
 
 template <int TW, int TH>
@@ -350,33 +358,7 @@ void dumpFrame(hipStream_t stream, const FrameState& f, uint32_t* splat_bits, in
 
 using namespace ans;
 
-struct ans_renderer {
-    SceneDev scene;
-    DevBuf<float> pos, scale, opacity, sh;
-    DevBuf<float4> rot;
-    DevBuf<uint8_t> image8;
-    DevBuf<float> imagef;
-    std::unique_ptr<Engine> engine;
-};
-
 namespace {
-
-template <class Fn>
-int guarded(Fn&& fn) {
-    try {
-        g_lastError.clear();
-        return fn();
-    } catch (const std::invalid_argument& e) {
-        g_lastError = e.what();
-        return ANS_ERR_ARGUMENT;
-    } catch (const std::exception& e) {
-        g_lastError = e.what();
-        return ANS_ERR_HIP;
-    } catch (...) {
-        g_lastError = "unknown error";
-        return ANS_ERR_HIP;
-    }
-}
 
 void upload(void* dst, const void* src, size_t bytes) {
     if (bytes) hipCheck(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");

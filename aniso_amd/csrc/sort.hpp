@@ -10,6 +10,11 @@
 // Keys and values are 32-bit. Replaces cub::DeviceRadixSort::SortPairs (gpu_render.cu:338-344)
 // in a design that sorts 32-bit depth keys per Gaussian and 32-bit tile ids per pair instead of
 // 64-bit tile|depth keys per pair.
+//
+// Batched views: blockIdx.y is the view. Each view sorts its own segment (items at
+// view * itemStride, histograms at view * histStride, row totals at view * 256, control block
+// ctl[view]); a single frame is the case gridDim.y = 1. With countFromCtl the item count is the
+// view's pair count (ctl->pairs, at most cap) and blocks past it do no work.
 #pragma once
 
 #include "common.hpp"
@@ -68,26 +73,31 @@ __device__ inline uint64_t matchDigit(uint32_t digit, int bits, bool valid) {
     return peers;
 }
 
-__device__ inline uint32_t itemCount(const uint32_t* count, uint32_t nHost, uint32_t cap) {
-    return count ? min(*count, cap) : nHost;
+__device__ inline uint32_t itemCount(const FrameCtl* ctl, bool countFromCtl, uint32_t nHost, uint32_t cap) {
+    return countFromCtl ? min(ctl->pairs, cap) : nHost;
 }
 
-// Per-block histogram of bits [shift, shift + bits) into hist[bin * stride + block]. Every block
-// of the grid writes its whole column (zeros past the item count).
+// Per-block histogram of bits [shift, shift + bits) into hist[bin * stride + block], for the
+// blocks that hold items.
 template <int ITEMS>
-__global__ __launch_bounds__(kSortBlock) void k_radixUpsweep(const uint32_t* __restrict__ keys,
-                                                             const uint32_t* __restrict__ count, uint32_t nHost,
-                                                             uint32_t cap, int shift, int bits,
+__global__ __launch_bounds__(kSortBlock) void k_radixUpsweep(const uint32_t* __restrict__ keys, bool countFromCtl,
+                                                             uint32_t nHost, uint32_t cap, int shift, int bits,
                                                              uint32_t* __restrict__ hist, uint32_t stride,
-                                                             const FrameCtl* __restrict__ ctl) {
+                                                             const FrameCtl* __restrict__ ctl, uint32_t itemStride,
+                                                             uint32_t histStride) {
     __shared__ uint32_t h[kRadix];
+    const uint32_t view = blockIdx.y;
+    ctl += view;
     if (ctl->overflow) return;
-    const uint32_t n = itemCount(count, nHost, cap);
+    keys += static_cast<size_t>(view) * itemStride;
+    hist += static_cast<size_t>(view) * histStride;
+    const uint32_t n = itemCount(ctl, countFromCtl, nHost, cap);
+    const uint32_t base = blockIdx.x * (kSortBlock * ITEMS);
+    if (base >= n && blockIdx.x > 0) return;
     const int tid = threadIdx.x;
     const int radix = 1 << bits;
     for (int i = tid; i < radix; i += kSortBlock) h[i] = 0;
     __syncthreads();
-    const uint32_t base = blockIdx.x * (kSortBlock * ITEMS);
     const uint32_t mask = radix - 1;
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
@@ -98,13 +108,20 @@ __global__ __launch_bounds__(kSortBlock) void k_radixUpsweep(const uint32_t* __r
     for (int i = tid; i < radix; i += kSortBlock) hist[static_cast<size_t>(i) * stride + blockIdx.x] = h[i];
 }
 
-// Exclusive scan of one row hist[bin * stride, + len) in place by one workgroup (blockIdx.x =
-// bin); the row total goes to rowTotal[bin].
-static __global__ __launch_bounds__(kSortBlock) void k_scanRows(uint32_t* __restrict__ hist, uint32_t stride, uint32_t len,
-                                                         uint32_t* __restrict__ rowTotal,
-                                                         const FrameCtl* __restrict__ ctl) {
+// Exclusive scan of one row hist[bin * stride, + blocks holding items) in place by one workgroup
+// (blockIdx.x = bin); the row total goes to rowTotal[bin]. tileItems: items per sort block.
+static __global__ __launch_bounds__(kSortBlock) void k_scanRows(uint32_t* __restrict__ hist, uint32_t stride,
+                                                                uint32_t len, uint32_t* __restrict__ rowTotal,
+                                                                const FrameCtl* __restrict__ ctl, bool countFromCtl,
+                                                                uint32_t cap, uint32_t tileItems,
+                                                                uint32_t histStride) {
     __shared__ uint32_t sTmp[kSortWaves];
+    const uint32_t view = blockIdx.y;
+    ctl += view;
     if (ctl->overflow) return;
+    hist += static_cast<size_t>(view) * histStride;
+    rowTotal += view * kRadix;
+    if (countFromCtl) len = max(1u, min(len, (itemCount(ctl, true, 0, cap) + tileItems - 1) / tileItems));
     constexpr int kPer = 8;
     uint32_t* row = hist + static_cast<size_t>(blockIdx.x) * stride;
     uint32_t carry = 0;
@@ -135,9 +152,9 @@ static __global__ __launch_bounds__(kSortBlock) void k_scanRows(uint32_t* __rest
 template <int ITEMS>
 __global__ __launch_bounds__(kSortBlock) void k_radixDownsweep(
     const uint32_t* __restrict__ keysIn, const uint32_t* __restrict__ valsIn, uint32_t* __restrict__ keysOut,
-    uint32_t* __restrict__ valsOut, const uint32_t* __restrict__ count, uint32_t nHost, uint32_t cap, int shift,
-    int bits, const uint32_t* __restrict__ hist, const uint32_t* __restrict__ rowTotal, uint32_t stride,
-    const FrameCtl* __restrict__ ctl) {
+    uint32_t* __restrict__ valsOut, bool countFromCtl, uint32_t nHost, uint32_t cap, int shift, int bits,
+    const uint32_t* __restrict__ hist, const uint32_t* __restrict__ rowTotal, uint32_t stride,
+    const FrameCtl* __restrict__ ctl, uint32_t itemStride, uint32_t histStride) {
     constexpr int kTile = kSortBlock * ITEMS;
     __shared__ uint32_t sWaveHist[kSortWaves][kRadix];
     __shared__ uint32_t sBinStart[kRadix];
@@ -146,10 +163,21 @@ __global__ __launch_bounds__(kSortBlock) void k_radixDownsweep(
     __shared__ uint32_t sVals[kTile];
     __shared__ uint32_t sScan[kSortWaves];
 
+    const uint32_t view = blockIdx.y;
+    ctl += view;
     if (ctl->overflow) return;
-    const uint32_t n = itemCount(count, nHost, cap);
+    const uint32_t n = itemCount(ctl, countFromCtl, nHost, cap);
     const uint32_t base = blockIdx.x * kTile;
     if (base >= n) return;
+    {
+        const size_t io = static_cast<size_t>(view) * itemStride;
+        keysIn += io;
+        if (valsIn) valsIn += io;
+        if (keysOut) keysOut += io;
+        valsOut += io;
+        hist += static_cast<size_t>(view) * histStride;
+        rowTotal += view * kRadix;
+    }
     const int tid = threadIdx.x;
     const int lane = __lane_id();
     const int wave = tid / kWave;

@@ -125,7 +125,8 @@ int ans_frame_dump(ans_renderer* r, uint32_t* splat_bits, int32_t* rect, uint64_
  * the kernels order with the caller's producers and consumers of those arrays. */
 
 /* Raw parameters as an optimizer holds them: means [n][3], log_scales [n][3], quats [n][4]
- * (w, x, y, z; normalized inside), opacity_logits [n], sh [n][(sh_degree+1)^2][3]. */
+ * (w, x, y, z; normalized inside), opacity_logits [n], sh [n][(sh_degree+1)^2][3]; quats and
+ * sh 16-byte aligned. */
 typedef struct ans_params {
     int64_t n;
     int32_t sh_degree;
@@ -136,7 +137,8 @@ typedef struct ans_params {
     const float* sh;
 } ans_params;
 
-/* Gradient outputs, same shapes as ans_params; overwritten (not accumulated). */
+/* Gradient outputs, same shapes as ans_params; overwritten (not accumulated). quats must be
+ * 16-byte aligned. */
 typedef struct ans_grads {
     float* means;
     float* log_scales;
