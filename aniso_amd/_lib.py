@@ -26,6 +26,7 @@ class Options_(ctypes.Structure):
 
 
 FLAG_NO_CULL = 1
+FLAG_NO_GRAPH = 2
 
 
 class Stats_(ctypes.Structure):
@@ -61,6 +62,12 @@ SIGNATURES = {
     "ans_frame_counts": (ctypes.c_int, [P, P, P, P, P]),
     "ans_frame_dump": (ctypes.c_int, [P, P, P, P, P, P]),
     "ans_activate": (None, [ctypes.c_int64, P, P, P, P, P, P]),
+    "ans_render_views": (ctypes.c_int, [P, ctypes.c_int32, ctypes.POINTER(Camera_), ctypes.POINTER(Options_), P, P,
+                                        ctypes.POINTER(Stats_)]),
+    "ans_views_autotune": (ctypes.c_int, [P, ctypes.c_int32, ctypes.POINTER(Camera_), ctypes.POINTER(Options_), P, P,
+                                          ctypes.c_int32, ctypes.POINTER(Options_), ctypes.POINTER(ctypes.c_double)]),
+    "ans_views_frame_counts": (ctypes.c_int, [P, ctypes.c_int32, P, P, P]),
+    "ans_views_frame_dump": (ctypes.c_int, [P, ctypes.c_int32, P, P]),
     "ans_diff_create": (ctypes.c_int, [ctypes.POINTER(P)]),
     "ans_diff_destroy": (None, [P]),
     "ans_diff_forward": (ctypes.c_int, [P, ctypes.POINTER(Params_), ctypes.POINTER(Camera_), ctypes.POINTER(Options_),

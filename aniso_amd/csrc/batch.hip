@@ -1,11 +1,484 @@
-// Batched multi-view forward: many cameras of one scene per pass of batched launches.
+// Batched multi-view forward: many cameras of one scene rendered by one sequence of batched
+// launches (blockIdx.y = view, or (view, tile) work items for the blend), replayed as a captured
+// HIP graph. There is no batched renderer in the reference; its nearest caller renders a camera
+// path one frame at a time (src/main.cpp:234-244), each frame a full GpuRenderer::render with its
+// own host round trip (src/gpu_render.cu:324-329).
+//
+// Per view the pipeline is the single-frame one (views.hpp, forward.hpp, sort.hpp) run on that
+// view's segment of a shared arena, so each view's splat records, lists, ranges and image equal
+// a single-frame render of that camera with the same options.
 #include "aniso_amd.h"
 #include "engine.hpp"
+#include "views.hpp"
+
+#include <algorithm>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <tuple>
+#include <vector>
 
 namespace ans {
 
-struct ViewBatch {};
+namespace {
+
+constexpr int kMaxViews = 256;
+
+size_t alignUp(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Byte offsets of the per-view table: CamParams[V], uint8_t*[V] outputs, int tilesX[V],
+// int tiles[V].
+struct TableLayout {
+    size_t cams, outs, tilesX, tiles, bytes;
+    explicit TableLayout(int v) {
+        cams = 0;
+        outs = alignUp(sizeof(CamParams) * v, 16);
+        tilesX = alignUp(outs + sizeof(uint8_t*) * v, 16);
+        tiles = alignUp(tilesX + sizeof(int) * v, 16);
+        bytes = alignUp(tiles + sizeof(int) * v, 16);
+    }
+};
+
+}  // namespace
+
+struct ViewBatch {
+    // Grow-only arena. Cached graphs bake its addresses, so growth of any buffer drops them.
+    DevBuf<uint8_t> table;
+    DevBuf<FrameCtl> ctl;  // one per view, plus one for the item-order sort
+    DevBuf<float4> rec0, rec1, rec2;
+    DevBuf<uint2> rect;
+    DevBuf<uint32_t> tilesTouched, depthKey, order, kA, vA, kB, vB;
+    DevBuf<uint32_t> hist, rowTotal, blockPairs, blockVisible;
+    DevBuf<uint32_t> tileA, gaussA, tileB, gaussB;
+    DevBuf<uint2> ranges;
+    DevBuf<uint32_t> itemKey, itemK2, itemV2, itemOrder;
+
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[2] = {};
+    uint8_t* tableHost = nullptr;  // pinned staging of the per-view table
+    FrameCtl* ctlHost = nullptr;   // pinned copy of the control blocks
+    size_t pairCap = 0;            // pairs per view
+    uint64_t generation = 0;       // bumped when the arena moves
+
+    using GraphKey = std::tuple<int, int64_t, int, int, int, bool, uint32_t, uint32_t, size_t, uint64_t>;
+    std::map<GraphKey, hipGraphExec_t> graphs;
+
+    // The last call, for stats and dumps.
+    int views = 0;
+    uint32_t maxTiles = 0;
+    std::vector<int> tilesPerView;
+    std::vector<FrameCtl> lastCtl;
+    const uint32_t* sortedGauss = nullptr;
+    float lastMs = 0.0f;
+    int lastRetries = 0;
+
+    ViewBatch() {
+        hipCheck(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+        for (auto& e : ev) hipCheck(hipEventCreate(&e), "hipEventCreate");
+        hipCheck(hipHostMalloc(reinterpret_cast<void**>(&tableHost), TableLayout(kMaxViews).bytes,
+                               hipHostMallocDefault),
+                 "hipHostMalloc");
+        hipCheck(hipHostMalloc(reinterpret_cast<void**>(&ctlHost), sizeof(FrameCtl) * (kMaxViews + 1),
+                               hipHostMallocDefault),
+                 "hipHostMalloc");
+    }
+    ~ViewBatch() {
+        dropGraphs();
+        if (tableHost) (void)hipHostFree(tableHost);
+        if (ctlHost) (void)hipHostFree(ctlHost);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    void dropGraphs() {
+        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
+    }
+};
 
 void ViewBatchDeleter::operator()(ViewBatch* b) const { delete b; }
 
+namespace {
+
+// Shapes of one batched call.
+struct Layout {
+    int views;
+    uint32_t n;
+    ans_options opts;
+    bool cull;
+    uint32_t maxTiles, totalTiles, items;
+    int tileBits, passes, digitBits;
+    uint32_t blocksD, blocksDup, blocksP, blocksI;
+    size_t pairCap;
+};
+
+template <class T>
+bool grow(DevBuf<T>& b, size_t n) {
+    if (n <= b.cap) return false;
+    b.reserve(n);
+    return true;
+}
+
+void reserveArena(ViewBatch& B, const Layout& L) {
+    const size_t V = L.views, n = std::max<uint32_t>(L.n, 1), vn = V * n;
+    bool moved = false;
+    moved |= grow(B.table, TableLayout(L.views).bytes);
+    moved |= grow(B.ctl, V + 1);
+    moved |= grow(B.rec0, vn) | grow(B.rec1, vn) | grow(B.rec2, vn) | grow(B.rect, vn);
+    moved |= grow(B.tilesTouched, vn) | grow(B.depthKey, vn) | grow(B.order, vn);
+    moved |= grow(B.kA, vn) | grow(B.vA, vn) | grow(B.kB, vn) | grow(B.vB, vn);
+    const size_t histD = V * kRadix * size_t(L.blocksD), histP = V * kRadix * size_t(L.blocksP),
+                 histI = size_t(kRadix) * L.blocksI;
+    moved |= grow(B.hist, std::max({histD, histP, histI}));
+    moved |= grow(B.rowTotal, (V + 1) * kRadix);
+    moved |= grow(B.blockPairs, V * L.blocksDup) | grow(B.blockVisible, V * L.blocksDup);
+    const size_t vp = V * L.pairCap;
+    moved |= grow(B.tileA, vp) | grow(B.gaussA, vp) | grow(B.tileB, vp) | grow(B.gaussB, vp);
+    moved |= grow(B.ranges, size_t(L.items));
+    moved |= grow(B.itemKey, L.items) | grow(B.itemK2, L.items) | grow(B.itemV2, L.items) |
+             grow(B.itemOrder, L.items);
+    if (moved) {
+        B.dropGraphs();
+        ++B.generation;
+        // A fresh control array: zero it once (the per-call init leaves sticky alone).
+        hipCheck(hipMemset(B.ctl.p, 0, sizeof(FrameCtl) * (V + 1)), "hipMemset ctl");
+    }
+}
+
+template <int TW, int TH, int BATCH>
+void launchBlendViews(ViewBatch& B, const Layout& L, const TableLayout& T, const uint32_t* sortedGauss,
+                      hipStream_t st) {
+    const CamParams* cams = reinterpret_cast<const CamParams*>(B.table.p + T.cams);
+    uint8_t* const* outs = reinterpret_cast<uint8_t* const*>(B.table.p + T.outs);
+    if (L.cull)
+        k_blendViews<TW, TH, BATCH, true><<<L.totalTiles, TW * TH, 0, st>>>(
+            B.itemOrder.p, L.maxTiles, B.ranges.p, sortedGauss, static_cast<uint32_t>(L.pairCap), B.rec0.p, B.rec1.p,
+            B.rec2.p, L.n, cams, outs, B.ctl.p);
+    else
+        k_blendViews<TW, TH, BATCH, false><<<L.totalTiles, TW * TH, 0, st>>>(
+            B.itemOrder.p, L.maxTiles, B.ranges.p, sortedGauss, static_cast<uint32_t>(L.pairCap), B.rec0.p, B.rec1.p,
+            B.rec2.p, L.n, cams, outs, B.ctl.p);
+}
+
+template <int TW, int TH>
+void launchBlendViewsT(ViewBatch& B, const Layout& L, const TableLayout& T, const uint32_t* sortedGauss,
+                       hipStream_t st) {
+    if (L.opts.batch_depth == 64) launchBlendViews<TW, TH, 64>(B, L, T, sortedGauss, st);
+    else if (L.opts.batch_depth == 128) launchBlendViews<TW, TH, 128>(B, L, T, sortedGauss, st);
+    else launchBlendViews<TW, TH, 256>(B, L, T, sortedGauss, st);
+}
+
+// Enqueues the whole batch on st (kernel launches only, so it can be captured). Returns the
+// buffer holding the per-view sorted Gaussian lists.
+const uint32_t* enqueueBatch(ViewBatch& B, const SceneDev& s, const Layout& L, hipStream_t st) {
+    const TableLayout T(L.views);
+    const CamParams* cams = reinterpret_cast<const CamParams*>(B.table.p + T.cams);
+    const int* tilesX = reinterpret_cast<const int*>(B.table.p + T.tilesX);
+    const int* tilesN = reinterpret_cast<const int*>(B.table.p + T.tiles);
+    const uint32_t V = static_cast<uint32_t>(L.views), n = L.n;
+    const uint32_t cap = static_cast<uint32_t>(L.pairCap);
+    FrameCtl* ctl = B.ctl.p;
+
+    k_initViews<<<(std::max(V + 1, L.items) + 255) / 256, 256, 0, st>>>(ctl, V + 1, B.ranges.p, L.items);
+    if (n > 0) {
+        k_preprocessViews<<<(n + 255) / 256, 256, 0, st>>>(static_cast<int>(n), L.views, s.pos, s.scale, s.rot,
+                                                           s.opacity, s.sh, (s.shDegree + 1) * (s.shDegree + 1),
+                                                           cams, B.rec0.p, B.rec1.p, B.rec2.p, B.rect.p,
+                                                           B.tilesTouched.p, B.depthKey.p);
+        // Depth sort of each view's Gaussians: four stable 8-bit passes.
+        const uint32_t histD = kRadix * L.blocksD;
+        const uint32_t* kin = B.depthKey.p;
+        const uint32_t* vin = nullptr;
+        uint32_t* kout[4] = {B.kA.p, B.kB.p, B.kA.p, nullptr};
+        uint32_t* vout[4] = {B.vA.p, B.vB.p, B.vA.p, B.order.p};
+        const dim3 gD(L.blocksD, V);
+        for (int pass = 0; pass < 4; ++pass) {
+            k_radixUpsweep<kDepthItems><<<gD, kSortBlock, 0, st>>>(kin, false, n, n, 8 * pass, 8, B.hist.p, L.blocksD,
+                                                                   ctl, n, histD);
+            k_scanRows<<<dim3(kRadix, V), kSortBlock, 0, st>>>(B.hist.p, L.blocksD, L.blocksD, B.rowTotal.p, ctl,
+                                                               false, n, kSortBlock * kDepthItems, histD);
+            k_radixDownsweep<kDepthItems><<<gD, kSortBlock, 0, st>>>(kin, vin, kout[pass], vout[pass], false, n, n,
+                                                                     8 * pass, 8, B.hist.p, B.rowTotal.p, L.blocksD,
+                                                                     ctl, n, histD);
+            kin = kout[pass];
+            vin = vout[pass];
+        }
+        // Pairs of each view in depth order, into the view's region of the pair arena.
+        const dim3 gDup(L.blocksDup, V);
+        k_pairTotals<<<gDup, kDupBlock, 0, st>>>(B.order.p, B.tilesTouched.p, n, B.blockPairs.p, B.blockVisible.p, n,
+                                                 L.blocksDup);
+        k_scanPairs<<<dim3(1, V), kDupBlock, 0, st>>>(B.blockPairs.p, B.blockVisible.p, L.blocksDup, cap, ctl,
+                                                      L.blocksDup);
+        k_expandPairs<<<gDup, kDupBlock, 0, st>>>(B.order.p, B.tilesTouched.p, B.rect.p, n, 0, B.blockPairs.p,
+                                                  B.tileA.p, B.gaussA.p, ctl, n, L.blocksDup, cap, tilesX);
+    }
+    // Stable sort of each view's pairs by tile id.
+    uint32_t* tk[2] = {B.tileA.p, B.tileB.p};
+    uint32_t* tv[2] = {B.gaussA.p, B.gaussB.p};
+    const uint32_t histP = kRadix * L.blocksP;
+    const dim3 gP(L.blocksP, V);
+    if (n > 0) {
+        for (int pass = 0; pass < L.passes; ++pass) {
+            const int src = pass & 1, dst = src ^ 1;
+            const int shift = pass * L.digitBits;
+            const int bits = std::min(L.digitBits, L.tileBits - shift);
+            k_radixUpsweep<kPairItems><<<gP, kSortBlock, 0, st>>>(tk[src], true, 0, cap, shift, bits, B.hist.p,
+                                                                  L.blocksP, ctl, cap, histP);
+            k_scanRows<<<dim3(1u << bits, V), kSortBlock, 0, st>>>(B.hist.p, L.blocksP, L.blocksP, B.rowTotal.p, ctl,
+                                                                   true, cap, kSortBlock * kPairItems, histP);
+            k_radixDownsweep<kPairItems><<<gP, kSortBlock, 0, st>>>(tk[src], tv[src], tk[dst], tv[dst], true, 0, cap,
+                                                                    shift, bits, B.hist.p, B.rowTotal.p, L.blocksP,
+                                                                    ctl, cap, histP);
+        }
+    }
+    const uint32_t* sortedTile = tk[L.passes & 1];
+    const uint32_t* sortedGauss = tv[L.passes & 1];
+    if (n > 0) {
+        const uint32_t rangeBlocks = std::max<uint32_t>(1, std::min<uint32_t>((cap + 255) / 256, 2048));
+        k_tileRanges<<<dim3(rangeBlocks, V), 256, 0, st>>>(sortedTile, ctl, B.ranges.p, cap, L.maxTiles);
+    }
+    // Work items longest list first: a stable sort of (view, tile) items by a 16-bit key.
+    k_itemKeys<<<(L.items + 255) / 256, 256, 0, st>>>(B.ranges.p, tilesN, L.maxTiles, L.items, B.itemKey.p);
+    {
+        FrameCtl* ictl = ctl + V;  // never overflows
+        const uint32_t* kin = B.itemKey.p;
+        const uint32_t* vin = nullptr;
+        uint32_t* kout[2] = {B.itemK2.p, nullptr};
+        uint32_t* vout[2] = {B.itemV2.p, B.itemOrder.p};
+        for (int pass = 0; pass < 2; ++pass) {
+            k_radixUpsweep<kDepthItems><<<L.blocksI, kSortBlock, 0, st>>>(kin, false, L.items, L.items, 8 * pass, 8,
+                                                                          B.hist.p, L.blocksI, ictl, 0, 0);
+            k_scanRows<<<kRadix, kSortBlock, 0, st>>>(B.hist.p, L.blocksI, L.blocksI, B.rowTotal.p, ictl, false,
+                                                      L.items, kSortBlock * kDepthItems, 0);
+            k_radixDownsweep<kDepthItems><<<L.blocksI, kSortBlock, 0, st>>>(kin, vin, kout[pass], vout[pass], false,
+                                                                            L.items, L.items, 8 * pass, 8, B.hist.p,
+                                                                            B.rowTotal.p, L.blocksI, ictl, 0, 0);
+            kin = kout[pass];
+            vin = vout[pass];
+        }
+    }
+    if (L.opts.tile_w == 8) launchBlendViewsT<8, 8>(B, L, T, sortedGauss, st);
+    else if (L.opts.tile_w == 32) launchBlendViewsT<32, 16>(B, L, T, sortedGauss, st);
+    else launchBlendViewsT<16, 16>(B, L, T, sortedGauss, st);
+    hipCheck(hipGetLastError(), "kernel launch");
+    return sortedGauss;
+}
+
+Layout makeLayout(const ViewBatch& B, const SceneDev& s, int views, const std::vector<CamParams>& cps,
+                  const ans_options& o) {
+    Layout L{};
+    L.views = views;
+    L.n = static_cast<uint32_t>(s.n);
+    L.opts = o;
+    L.cull = !(o.flags & ANS_FLAG_NO_CULL);
+    uint64_t total = 0;
+    uint32_t maxT = 1;
+    for (const CamParams& cp : cps) {
+        const uint32_t t = static_cast<uint32_t>(cp.tilesX) * static_cast<uint32_t>(cp.tilesY);
+        maxT = std::max(maxT, t);
+        total += t;
+    }
+    if (uint64_t(maxT) * views >= (uint64_t(1) << 31)) throw std::invalid_argument("too many tiles in the batch");
+    L.maxTiles = maxT;
+    L.totalTiles = static_cast<uint32_t>(total);
+    L.items = maxT * static_cast<uint32_t>(views);
+    L.tileBits = std::max(1, bitsFor(maxT - 1));
+    L.passes = (L.tileBits + 7) / 8;
+    L.digitBits = (L.tileBits + L.passes - 1) / L.passes;
+    L.blocksD = std::max<uint32_t>(1, (L.n + kSortBlock * kDepthItems - 1) / (kSortBlock * kDepthItems));
+    L.blocksDup = std::max<uint32_t>(1, (L.n + kDupTile - 1) / kDupTile);
+    L.pairCap = B.pairCap ? B.pairCap : std::max<size_t>(size_t(L.n) * 8, size_t(1) << 20);
+    L.blocksP = static_cast<uint32_t>((L.pairCap + kSortBlock * kPairItems - 1) / (kSortBlock * kPairItems));
+    L.blocksI = std::max<uint32_t>(1, (L.items + kSortBlock * kDepthItems - 1) / (kSortBlock * kDepthItems));
+    return L;
+}
+
+// Uploads the per-view table and runs the batch on st (graph replay unless ANS_FLAG_NO_GRAPH;
+// graphs are captured on the batch's own stream and launched on st); waits and returns true if
+// any view's pairs overflowed the arena.
+bool runBatch(ViewBatch& B, const SceneDev& s, const Layout& L, const std::vector<CamParams>& cps,
+              uint8_t* const* outs, hipStream_t st) {
+    const TableLayout T(L.views);
+    for (int v = 0; v < L.views; ++v) {
+        reinterpret_cast<CamParams*>(B.tableHost + T.cams)[v] = cps[v];
+        reinterpret_cast<uint8_t**>(B.tableHost + T.outs)[v] = outs[v];
+        reinterpret_cast<int*>(B.tableHost + T.tilesX)[v] = cps[v].tilesX;
+        reinterpret_cast<int*>(B.tableHost + T.tiles)[v] = cps[v].tilesX * cps[v].tilesY;
+    }
+    hipCheck(hipMemcpyAsync(B.table.p, B.tableHost, T.bytes, hipMemcpyHostToDevice, st), "hipMemcpyAsync table");
+    hipCheck(hipEventRecord(B.ev[0], st), "hipEventRecord");
+    const uint32_t* sorted = nullptr;
+    if (L.opts.flags & ANS_FLAG_NO_GRAPH) {
+        sorted = enqueueBatch(B, s, L, st);
+    } else {
+        const ViewBatch::GraphKey key{L.views, s.n, L.opts.tile_w, L.opts.tile_h, L.opts.batch_depth, L.cull,
+                                      L.maxTiles, L.totalTiles, L.pairCap, B.generation};
+        auto it = B.graphs.find(key);
+        if (it == B.graphs.end()) {
+            hipGraph_t g = nullptr;
+            hipCheck(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            try {
+                enqueueBatch(B, s, L, B.stream);
+            } catch (...) {
+                (void)hipStreamEndCapture(B.stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            hipCheck(hipStreamEndCapture(B.stream, &g), "hipStreamEndCapture");
+            hipGraphExec_t exec = nullptr;
+            const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            hipCheck(e, "hipGraphInstantiate");
+            it = B.graphs.emplace(key, exec).first;
+        }
+        hipCheck(hipGraphLaunch(it->second, st), "hipGraphLaunch");
+        sorted = (L.passes & 1) ? B.gaussB.p : B.gaussA.p;
+    }
+    hipCheck(hipEventRecord(B.ev[1], st), "hipEventRecord");
+    hipCheck(hipMemcpyAsync(B.ctlHost, B.ctl.p, sizeof(FrameCtl) * L.views, hipMemcpyDeviceToHost, st),
+             "hipMemcpyAsync ctl");
+    hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+    B.sortedGauss = sorted;
+    hipCheck(hipEventElapsedTime(&B.lastMs, B.ev[0], B.ev[1]), "hipEventElapsedTime");
+    B.lastCtl.assign(B.ctlHost, B.ctlHost + L.views);
+    uint32_t most = 0;
+    bool over = false;
+    for (const FrameCtl& c : B.lastCtl) {
+        most = std::max(most, c.pairs);
+        over |= c.overflow != 0;
+    }
+    if (over) {
+        if (most >= (1u << 30) - 1) throw std::runtime_error("pair count of a view exceeds 2^30");
+        B.pairCap = static_cast<size_t>(most) + most / 4 + 1024;
+    }
+    return over;
+}
+
+ViewBatch& batchOf(ans_renderer* r) {
+    if (!r->batch) r->batch.reset(new ViewBatch());
+    return *r->batch;
+}
+
+void renderViews(ans_renderer* r, int32_t n_views, const ans_camera* cams, const ans_options* options,
+                 uint8_t* const* dev_rgb, ans_stats* stats, void* stream) {
+    if (!r) throw std::invalid_argument("renderer is null");
+    if (n_views < 1 || n_views > kMaxViews) throw std::invalid_argument("n_views must be in 1..256");
+    if (!cams || !dev_rgb) throw std::invalid_argument("cameras and outputs must not be null");
+    const ans_options o = resolveOptions(options);
+    std::vector<CamParams> cps(n_views);
+    for (int v = 0; v < n_views; ++v) {
+        checkCamera(&cams[v]);
+        if (!dev_rgb[v]) throw std::invalid_argument("output pointer must not be null");
+        cps[v] = makeCamParams(cams[v], o, r->scene.shDegree);
+        if (cps[v].tilesX > 65535 || cps[v].tilesY > 65535) throw std::invalid_argument("too many tiles");
+    }
+    ViewBatch& B = batchOf(r);
+    int retries = 0;
+    while (true) {
+        const Layout L = makeLayout(B, r->scene, n_views, cps, o);
+        reserveArena(B, L);
+        B.pairCap = L.pairCap;
+        B.views = n_views;
+        B.maxTiles = L.maxTiles;
+        B.tilesPerView.resize(n_views);
+        for (int v = 0; v < n_views; ++v) B.tilesPerView[v] = cps[v].tilesX * cps[v].tilesY;
+        if (!runBatch(B, r->scene, L, cps, dev_rgb, static_cast<hipStream_t>(stream))) break;
+        ++retries;
+    }
+    B.lastRetries = retries;
+    if (stats) {
+        for (int v = 0; v < n_views; ++v) {
+            ans_stats& st = stats[v];
+            st.visible = B.lastCtl[v].visible;
+            st.tile_pairs = B.lastCtl[v].pairs;
+            st.project_ms = 0.0f;
+            st.sort_ms = 0.0f;
+            st.blend_ms = B.lastMs;
+            st.pair_capacity = static_cast<int64_t>(B.pairCap);
+            st.retries = retries;
+            st.reserved = 0;
+        }
+    }
+}
+
+}  // namespace
+
 }  // namespace ans
+
+using namespace ans;
+
+extern "C" {
+
+int ans_render_views(ans_renderer* r, int32_t n_views, const ans_camera* cams, const ans_options* options,
+                     uint8_t* const* dev_rgb, void* stream, ans_stats* stats) {
+    return guarded([&] {
+        renderViews(r, n_views, cams, options, dev_rgb, stats, stream);
+        return ANS_OK;
+    });
+}
+
+int ans_views_autotune(ans_renderer* r, int32_t n_views, const ans_camera* cams, const ans_options* base,
+                       uint8_t* const* dev_rgb, void* stream, int32_t repeats, ans_options* best, double* best_ms) {
+    return guarded([&] {
+        if (!best) throw std::invalid_argument("best is null");
+        const ans_options b = resolveOptions(base);
+        const int reps = std::max(1, repeats);
+        static const int kTiles[3][2] = {{8, 8}, {16, 16}, {32, 16}};
+        static const int kDepths[3] = {64, 128, 256};
+        double bestMs = 1e30;
+        ans_options bestOpt = b;
+        for (const auto& t : kTiles) {
+            for (int d : kDepths) {
+                ans_options o = b;
+                o.tile_w = t[0];
+                o.tile_h = t[1];
+                o.batch_depth = d;
+                renderViews(r, n_views, cams, &o, dev_rgb, nullptr, stream);  // builds and warms the graph
+                double ms = 1e30;
+                for (int k = 0; k < reps; ++k) {
+                    renderViews(r, n_views, cams, &o, dev_rgb, nullptr, stream);
+                    ms = std::min(ms, static_cast<double>(r->batch->lastMs));
+                }
+                if (ms < bestMs) {
+                    bestMs = ms;
+                    bestOpt = o;
+                }
+            }
+        }
+        *best = bestOpt;
+        if (best_ms) *best_ms = bestMs;
+        return ANS_OK;
+    });
+}
+
+int ans_views_frame_counts(ans_renderer* r, int32_t view, int64_t* visible, int64_t* pairs, int64_t* tiles) {
+    return guarded([&] {
+        if (!r || !r->batch || r->batch->views == 0) throw std::invalid_argument("no batched render on this handle");
+        ViewBatch& B = *r->batch;
+        if (view < 0 || view >= B.views) throw std::invalid_argument("view out of range");
+        if (visible) *visible = B.lastCtl[view].visible;
+        if (pairs) *pairs = B.lastCtl[view].pairs;
+        if (tiles) *tiles = B.tilesPerView[view];
+        return ANS_OK;
+    });
+}
+
+int ans_views_frame_dump(ans_renderer* r, int32_t view, uint32_t* values, uint32_t* ranges) {
+    return guarded([&] {
+        if (!r || !r->batch || r->batch->views == 0) throw std::invalid_argument("no batched render on this handle");
+        ViewBatch& B = *r->batch;
+        if (view < 0 || view >= B.views) throw std::invalid_argument("view out of range");
+        const size_t P = B.lastCtl[view].pairs, T = static_cast<size_t>(B.tilesPerView[view]);
+        if (values && P)
+            hipCheck(hipMemcpy(values, B.sortedGauss + static_cast<size_t>(view) * B.pairCap, 4 * P,
+                               hipMemcpyDeviceToHost),
+                     "hipMemcpy values");
+        if (ranges && T)
+            hipCheck(hipMemcpy(ranges, B.ranges.p + static_cast<size_t>(view) * B.maxTiles, 8 * T,
+                               hipMemcpyDeviceToHost),
+                     "hipMemcpy ranges");
+        return ANS_OK;
+    });
+}
+
+}  // extern "C"

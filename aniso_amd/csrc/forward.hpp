@@ -233,6 +233,28 @@ __host__ __device__ inline bool projectGaussian(int i, const float* __restrict__
     return true;
 }
 
+// Writes a visible Gaussian's splat record, rect, tile count and depth key at index o.
+__device__ inline void storeSplat(const GaussProj& g, size_t o, float4* __restrict__ rec0, float4* __restrict__ rec1,
+                                  float4* __restrict__ rec2, uint2* __restrict__ rect,
+                                  uint32_t* __restrict__ tilesTouched, uint32_t* __restrict__ depthKey) {
+    // Region where op * exp(-q/2) >= 1/255: q <= 2 ln(255 op); half-extents sqrt(k a), sqrt(k c)
+    // of that ellipse, widened by a 5% + 0.1 margin far above the FP32 rounding of the blend's
+    // exponent.
+    float ex = -1e30f, ey = -1e30f;
+    if (g.opacity >= 1.0f / 255.0f) {
+        const float k = fmaxf(2.0f * logf(255.0f * g.opacity), 0.0f) * 1.05f + 0.1f;
+        ex = sqrtf(k * g.a) + 0.01f;
+        ey = sqrtf(k * g.c) + 0.01f;
+    }
+    rec0[o] = make_float4(g.u, g.v, g.A, g.B);
+    rec1[o] = make_float4(g.C, g.opacity, ex, ey);
+    rec2[o] = make_float4(g.rgb.x, g.rgb.y, g.rgb.z, g.depth);
+    rect[o] = make_uint2(static_cast<uint32_t>(g.x0) | (static_cast<uint32_t>(g.y0) << 16),
+                         static_cast<uint32_t>(g.x1) | (static_cast<uint32_t>(g.y1) << 16));
+    tilesTouched[o] = static_cast<uint32_t>((g.x1 - g.x0 + 1) * (g.y1 - g.y0 + 1));
+    depthKey[o] = __float_as_uint(g.depth);
+}
+
 // Per-Gaussian preprocess, one lane per Gaussian. Writes the splat record
 // {u, v, A, B | C, opacity, ext_x, ext_y | r, g, b, depth}, the packed tile rect, the tile count and
 // the depth sort key (0xffffffff for culled Gaussians). ext_x/y are conservative half-extents of
@@ -256,22 +278,7 @@ __global__ __launch_bounds__(256) void k_preprocess(int n, const float* __restri
         depthKey[i] = 0xffffffffu;
         return;
     }
-    // Region where op * exp(-q/2) >= 1/255: q <= 2 ln(255 op); half-extents sqrt(k a), sqrt(k c)
-    // of that ellipse, widened by a 5% + 0.1 margin far above the FP32 rounding of the blend's
-    // exponent.
-    float ex = -1e30f, ey = -1e30f;
-    if (g.opacity >= 1.0f / 255.0f) {
-        const float k = fmaxf(2.0f * logf(255.0f * g.opacity), 0.0f) * 1.05f + 0.1f;
-        ex = sqrtf(k * g.a) + 0.01f;
-        ey = sqrtf(k * g.c) + 0.01f;
-    }
-    rec0[i] = make_float4(g.u, g.v, g.A, g.B);
-    rec1[i] = make_float4(g.C, g.opacity, ex, ey);
-    rec2[i] = make_float4(g.rgb.x, g.rgb.y, g.rgb.z, g.depth);
-    rect[i] = make_uint2(static_cast<uint32_t>(g.x0) | (static_cast<uint32_t>(g.y0) << 16),
-                         static_cast<uint32_t>(g.x1) | (static_cast<uint32_t>(g.y1) << 16));
-    tilesTouched[i] = static_cast<uint32_t>((g.x1 - g.x0 + 1) * (g.y1 - g.y0 + 1));
-    depthKey[i] = __float_as_uint(g.depth);
+    storeSplat(g, i, rec0, rec1, rec2, rect, tilesTouched, depthKey);
 }
 
 constexpr int kDupBlock = 256;                       // 4 wavefronts x 64 depth ranks
@@ -441,7 +448,7 @@ static __global__ __launch_bounds__(kDupBlock) void k_expandPairs(
 }
 
 // Start and end of each tile's run in the sorted pairs (gpu_render.cu:164-177); ranges were
-// cleared to [0, 0) by the frame init.
+// cleared to [0, 0) by the frame init. Grid-stride over the pairs.
 static __global__ __launch_bounds__(256) void k_tileRanges(const uint32_t* __restrict__ sortedTile,
                                                            const FrameCtl* __restrict__ ctl,
                                                            uint2* __restrict__ ranges, uint32_t pairStride,
@@ -451,11 +458,11 @@ static __global__ __launch_bounds__(256) void k_tileRanges(const uint32_t* __res
     ranges += static_cast<size_t>(blockIdx.y) * tileStride;
     if (ctl->overflow) return;
     const uint32_t total = ctl->pairs;
-    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
-    if (p >= total) return;
-    const uint32_t t = sortedTile[p];
-    if (p == 0 || sortedTile[p - 1] != t) ranges[t].x = p;
-    if (p == total - 1 || sortedTile[p + 1] != t) ranges[t].y = p + 1;
+    for (uint32_t p = blockIdx.x * 256 + threadIdx.x; p < total; p += gridDim.x * 256) {
+        const uint32_t t = sortedTile[p];
+        if (p == 0 || sortedTile[p - 1] != t) ranges[t].x = p;
+        if (p == total - 1 || sortedTile[p + 1] != t) ranges[t].y = p + 1;
+    }
 }
 
 // Smallest value of q(d) = A dx^2 + 2 B dx dy + C dy^2 (A, C > 0, AC > B^2) over the box

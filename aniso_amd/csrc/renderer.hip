@@ -114,9 +114,6 @@ Engine::~Engine() {
     if (ownStream) (void)hipStreamDestroy(ownStream);
 }
 
-constexpr int kDepthItems = 4;   // 1024 depth keys per sort block
-constexpr int kPairItems = 16;   // 4096 pairs per sort block
-
 void Engine::reserveScratch(size_t n, size_t pairCap) {
     const size_t m = std::max<size_t>(n, 1);
     depthKey.reserve(m); sortKA.reserve(m); sortVA.reserve(m); sortKB.reserve(m); sortVB.reserve(m);

@@ -24,6 +24,8 @@ namespace ans {
 constexpr int kSortBlock = 256;
 constexpr int kRadix = 256;
 constexpr int kSortWaves = kSortBlock / kWave;
+constexpr int kDepthItems = 4;  // 1024 keys per sort block (per-Gaussian sorts)
+constexpr int kPairItems = 16;  // 4096 keys per sort block (pair sorts)
 
 // Inclusive scan of one value per lane across a 64-lane wavefront.
 __device__ inline uint32_t waveInclusiveScan(uint32_t v) {

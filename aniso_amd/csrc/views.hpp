@@ -1,0 +1,91 @@
+// Kernels of the batched multi-view forward. Per-Gaussian arrays of view v sit at v * n, pair
+// arrays at v * pairStride, tile ranges at v * maxTiles; a (view, tile) item is v * maxTiles + t.
+// The per-view camera constants, output pointers and tile-grid sizes come from one device table
+// uploaded per call, so a captured graph replays for any poses.
+#pragma once
+
+#include "forward.hpp"
+
+namespace ans {
+
+// Clears the first ctlCount control blocks (all but sticky) and rangeCount tile ranges.
+static __global__ __launch_bounds__(256) void k_initViews(FrameCtl* __restrict__ ctl, uint32_t ctlCount,
+                                                          uint2* __restrict__ ranges, uint32_t rangeCount) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < ctlCount) {
+        ctl[i].epoch += 1;
+        ctl[i].visible = 0;
+        ctl[i].pairs = 0;
+        ctl[i].overflow = 0;
+    }
+    if (i < rangeCount) ranges[i] = make_uint2(0, 0);
+}
+
+// Preprocess of every view, one lane per Gaussian: its parameters and SH (planar on the device,
+// [coefficient * 3 + channel][n]) are loaded once into registers and projected into each view in
+// turn with projectGaussian, the arithmetic of the single-view k_preprocess. Outputs for view v
+// at v * n + i.
+static __global__ __launch_bounds__(256) void k_preprocessViews(
+    int n, int views, const float* __restrict__ pos, const float* __restrict__ scale, const float4* __restrict__ rot,
+    const float* __restrict__ opacity, const float* __restrict__ sh, int shCoeffs, const CamParams* __restrict__ cams,
+    float4* __restrict__ rec0, float4* __restrict__ rec1, float4* __restrict__ rec2, uint2* __restrict__ rect,
+    uint32_t* __restrict__ tilesTouched, uint32_t* __restrict__ depthKey) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float p[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    const float sc[3] = {scale[3 * i], scale[3 * i + 1], scale[3 * i + 2]};
+    const float4 q = rot[i];
+    const float op = opacity[i];
+    const int K3 = shCoeffs * 3;
+    alignas(16) float shr[48];
+#pragma unroll
+    for (int j = 0; j < 48; ++j) shr[j] = j < K3 ? sh[static_cast<size_t>(j) * n + i] : 0.0f;
+    for (int v = 0; v < views; ++v) {
+        const size_t o = static_cast<size_t>(v) * n + i;
+        GaussProj g;
+        if (!projectGaussian<false, false>(0, p, sc, &q, &op, shr, 1, cams[v], g)) {
+            tilesTouched[o] = 0;
+            depthKey[o] = 0xffffffffu;
+            continue;
+        }
+        storeSplat(g, o, rec0, rec1, rec2, rect, tilesTouched, depthKey);
+    }
+}
+
+// Sort key of item v * maxTiles + t for longest-first order: 0x7fff - list length (lengths above
+// 0x7fff count as 0x7fff); items past view v's tile count get 0xffff and come last.
+static __global__ __launch_bounds__(256) void k_itemKeys(const uint2* __restrict__ ranges,
+                                                         const int* __restrict__ tilesView, uint32_t maxTiles,
+                                                         uint32_t items, uint32_t* __restrict__ keys) {
+    const uint32_t it = blockIdx.x * 256 + threadIdx.x;
+    if (it >= items) return;
+    const uint32_t v = it / maxTiles, t = it - v * maxTiles;
+    uint32_t key = 0xffffu;
+    if (t < static_cast<uint32_t>(tilesView[v])) {
+        const uint2 r = ranges[it];
+        key = 0x7fffu - min(r.y - r.x, 0x7fffu);
+    }
+    keys[it] = key;
+}
+
+// Blend of every view's tiles in one launch, workgroups taken longest list first (itemOrder),
+// each tile blended exactly as the single-view k_blend blends it.
+template <int TW, int TH, int BATCH, bool CULL>
+__global__ __launch_bounds__(TW* TH) void k_blendViews(const uint32_t* __restrict__ itemOrder, uint32_t maxTiles,
+                                                       const uint2* __restrict__ ranges,
+                                                       const uint32_t* __restrict__ sortedGauss, uint32_t pairStride,
+                                                       const float4* __restrict__ rec0, const float4* __restrict__ rec1,
+                                                       const float4* __restrict__ rec2, uint32_t gaussStride,
+                                                       const CamParams* __restrict__ cams, uint8_t* const* __restrict__ outs,
+                                                       const FrameCtl* __restrict__ ctl) {
+    const uint32_t item = itemOrder[blockIdx.x];
+    const uint32_t v = item / maxTiles, t = item - v * maxTiles;
+    if (ctl[v].overflow) return;
+    const int tilesX = cams[v].tilesX;
+    const size_t go = static_cast<size_t>(v) * gaussStride;
+    blendTile<TW, TH, BATCH, CULL>(static_cast<int>(t) % tilesX, static_cast<int>(t) / tilesX, ranges[item],
+                                   sortedGauss + static_cast<size_t>(v) * pairStride, rec0 + go, rec1 + go, rec2 + go,
+                                   cams[v].width, cams[v].height, outs[v], nullptr, nullptr);
+}
+
+}  // namespace ans

@@ -28,10 +28,17 @@ class RenderOptions:
     tile: tuple = (16, 16)
     batch_depth: int = 256
     cull: bool = True  # conservative per-8x8-block culling in the blend (output identical either way)
+    graph: bool = True  # batched views: replay the captured HIP graph (else direct launches)
 
     def c(self) -> _lib.Options_:
+        flags = (0 if self.cull else _lib.FLAG_NO_CULL) | (0 if self.graph else _lib.FLAG_NO_GRAPH)
         return _lib.Options_(int(self.sh_degree), float(self.splat_scale), int(self.tile[0]), int(self.tile[1]),
-                             int(self.batch_depth), 0 if self.cull else _lib.FLAG_NO_CULL)
+                             int(self.batch_depth), flags)
+
+    @classmethod
+    def from_c(cls, o: _lib.Options_) -> "RenderOptions":
+        return cls(o.sh_degree, o.splat_scale, (o.tile_w, o.tile_h), o.batch_depth, not (o.flags & _lib.FLAG_NO_CULL),
+                   not (o.flags & _lib.FLAG_NO_GRAPH))
 
 
 @dataclasses.dataclass
@@ -56,6 +63,15 @@ def camera_c(camera: Camera) -> _lib.Camera_:
     c.fx, c.fy, c.cx, c.cy = float(camera.fx), float(camera.fy), float(camera.cx), float(camera.cy)
     c.width, c.height = int(camera.width), int(camera.height)
     return c
+
+
+def _torch_stream():
+    """torch's current HIP stream if torch is loaded with a GPU, else None (the default stream)."""
+    import sys
+    torch = sys.modules.get("torch")
+    if torch is not None and torch.cuda.is_available():
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    return None
 
 
 def _ptr(a: np.ndarray):
@@ -111,6 +127,47 @@ class GpuRenderer:
             for f in dataclasses.fields(RenderStats):
                 setattr(stats, f.name, getattr(st, f.name))
         return img, imgf
+
+    def render_views(self, cameras, outputs, options: RenderOptions | None = None, stats: list | None = None):
+        """Renders every camera in one batched pass (ans_render_views) on torch's current stream.
+        outputs: one device buffer per camera (torch uint8 tensors of H*W*3 elements on the GPU,
+        or raw device addresses). Fills `stats` (a list) with one RenderStats per view if given."""
+        k = len(cameras)
+        if len(outputs) != k:
+            raise ValueError("one output per camera")
+        cams = (_lib.Camera_ * k)(*[camera_c(c) for c in cameras])
+        ptrs = (ctypes.c_void_p * k)(*[o if isinstance(o, int) else o.data_ptr() for o in outputs])
+        st = (_lib.Stats_ * k)()
+        opts = (options or RenderOptions()).c()
+        _lib.check(_lib.load().ans_render_views(self._h, k, cams, ctypes.byref(opts), ptrs, _torch_stream(), st))
+        if stats is not None:
+            stats.clear()
+            for s_ in st:
+                stats.append(RenderStats(**{f.name: getattr(s_, f.name) for f in dataclasses.fields(RenderStats)}))
+
+    def autotune_views(self, cameras, outputs, base: RenderOptions | None = None, repeats: int = 3):
+        """(best RenderOptions, batch ms) over tile shapes x LDS batch depths (ans_views_autotune)."""
+        k = len(cameras)
+        cams = (_lib.Camera_ * k)(*[camera_c(c) for c in cameras])
+        ptrs = (ctypes.c_void_p * k)(*[o if isinstance(o, int) else o.data_ptr() for o in outputs])
+        best = _lib.Options_()
+        ms = ctypes.c_double()
+        _lib.check(_lib.load().ans_views_autotune(self._h, k, cams, ctypes.byref((base or RenderOptions()).c()), ptrs,
+                                                  _torch_stream(), int(repeats), ctypes.byref(best),
+                                                  ctypes.byref(ms)))
+        return RenderOptions.from_c(best), ms.value
+
+    def views_dump(self, view: int):
+        """Lists of one view of the last batched render: dict with visible, pairs, values u32[P],
+        ranges u32[T,2]."""
+        L = _lib.load()
+        vals = [ctypes.c_int64() for _ in range(3)]
+        _lib.check(L.ans_views_frame_counts(self._h, view, *[ctypes.byref(v) for v in vals]))
+        visible, pairs, tiles = (v.value for v in vals)
+        out = dict(visible=visible, pairs=pairs, values=np.empty(pairs, np.uint32),
+                   ranges=np.empty((tiles, 2), np.uint32))
+        _lib.check(L.ans_views_frame_dump(self._h, view, _ptr(out["values"]), _ptr(out["ranges"])))
+        return out
 
     def frame_counts(self):
         vals = [ctypes.c_int64() for _ in range(4)]

@@ -225,9 +225,30 @@ def read_ply(path) -> RawScene:
                     get(["rot_0", "rot_1", "rot_2", "rot_3"]), get(["opacity"])[:, 0].copy(), sh)
 
 
+VIEW_SIZES = ((640, 360), (1280, 720), (1920, 1080))
+
+
+def view_batch_cameras(count: int = 16, seed: int = 1, hfov_deg: float = 60.0):
+    """`count` cameras of mixed sizes for the batched multi-view workload, drawn with numpy's
+    default_rng(seed): size uniform over 640x360, 1280x720, 1920x1080; position uniform in the
+    box [-1, 1]^3 around the origin; looking along +z turned by a yaw uniform in [-30, 30] degrees
+    about the vertical axis, with image +y pointing down as in the synthetic scenes' camera."""
+    rng = np.random.default_rng(seed)
+    cams = []
+    for _ in range(count):
+        w, h = VIEW_SIZES[int(rng.integers(0, len(VIEW_SIZES)))]
+        pos = rng.uniform(-1.0, 1.0, 3)
+        yaw = math.radians(rng.uniform(-30.0, 30.0))
+        fwd = (math.sin(yaw), 0.0, math.cos(yaw))
+        cams.append(pose_camera(pos, fwd, (0.0, -1.0, 0.0), w, h, hfov_deg))
+    return cams
+
+
 # The benchmark workloads, by what they are.
 WORKLOADS = {
     "synthetic-200k-640x360": dict(n=200_000, width=640, height=360, clustered=False),
     "synthetic-741883-1280x720": dict(n=741_883, width=1280, height=720, clustered=False),
     "clustered-2.2M-1280x720": dict(n=2_200_000, width=1280, height=720, clustered=True),
+    # 16 cameras of mixed sizes (view_batch_cameras); the scene fills the 1280x720 frustum.
+    "batched-1M-16views": dict(n=1_000_000, width=1280, height=720, clustered=False, views=16),
 }

@@ -106,6 +106,7 @@ def main():
     if not args.skip_extra:
         extra["fwd_bwd"] = bench_fwd_bwd(raw, cam, max(5, args.steps // 4), args.warmup)
         extra["fwd_clustered_2.2M"] = bench_forward("clustered-2.2M-1280x720", max(10, args.steps // 2), args.warmup)
+        extra["batched_16_views"] = bench_views("batched-1M-16views", max(5, args.steps // 10), args.warmup)
 
     cpu = None
     if not args.skip_cpu:
@@ -165,6 +166,68 @@ def bench_forward(workload, steps, warmup):
             "width": cam.width, "height": cam.height, "steps": steps,
             "stages_ms": {"project": round(st.project_ms, 4), "sort": round(st.sort_ms, 4),
                           "blend": round(st.blend_ms, 4)}}
+
+
+def bench_views(workload, steps, warmup):
+    """Views/s of the batched multi-view forward (one ans_render_views call per batch of 16
+    mixed-size cameras, graph replay, autotuned tile shape and LDS batch depth) against the same
+    16 cameras rendered by single-view launches (16 ans_render_async calls, one wait), both into
+    device images, on one scene."""
+    import torch
+
+    from aniso_amd import _lib, scenes
+    from aniso_amd.renderer import GpuRenderer, RenderOptions, camera_c
+
+    wl = scenes.WORKLOADS[workload]
+    raw = scenes.synthetic_scene(wl["n"], wl["width"], wl["height"], clustered=wl["clustered"])
+    cams = scenes.view_batch_cameras(wl["views"])
+    L = _lib.load()
+    outs = [torch.empty(c.width * c.height * 3, dtype=torch.uint8, device="cuda") for c in cams]
+    with GpuRenderer(raw) as r:
+        # Single-view launches at the default options (16x16 tiles, depth 256).
+        c_opt = RenderOptions().c()
+        c_cams = [camera_c(c) for c in cams]
+
+        def plain():
+            for c, o in zip(c_cams, outs):
+                _lib.check(L.ans_render_async(r._h, c, c_opt, o.data_ptr(), None))
+            _lib.check(L.ans_render_wait(r._h, None))
+
+        for _ in range(max(1, warmup)):
+            plain()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            plain()
+        torch.cuda.synchronize()
+        plain_ms = (time.perf_counter() - t0) * 1e3 / steps
+
+        best, tuned_ms = r.autotune_views(cams, outs, repeats=3)
+        stats = []
+        for _ in range(max(1, warmup)):
+            r.render_views(cams, outs, best, stats)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            r.render_views(cams, outs, best)
+        torch.cuda.synchronize()
+        batch_ms = (time.perf_counter() - t0) * 1e3 / steps
+        r.render_views(cams, outs, RenderOptions(tile=best.tile, batch_depth=best.batch_depth, graph=False))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            r.render_views(cams, outs, RenderOptions(tile=best.tile, batch_depth=best.batch_depth, graph=False))
+        torch.cuda.synchronize()
+        direct_ms = (time.perf_counter() - t0) * 1e3 / steps
+    k = len(cams)
+    return {"views": k, "gaussians": raw.n, "sizes": [f"{c.width}x{c.height}" for c in cams],
+            "pairs_total": int(sum(s.tile_pairs for s in stats)),
+            "batched_views_per_s": round(k / (batch_ms * 1e-3), 1), "batched_ms_per_batch": round(batch_ms, 4),
+            "batched_direct_launch_ms_per_batch": round(direct_ms, 4),
+            "single_view_views_per_s": round(k / (plain_ms * 1e-3), 1),
+            "single_view_ms_per_batch": round(plain_ms, 4), "speedup": round(plain_ms / batch_ms, 3),
+            "autotuned": {"tile": f"{best.tile[0]}x{best.tile[1]}", "batch_depth": best.batch_depth,
+                          "gpu_ms": round(tuned_ms, 4)}, "steps": steps}
 
 
 def bench_fwd_bwd(raw, cam, steps, warmup):

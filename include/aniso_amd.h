@@ -48,6 +48,9 @@ typedef struct ans_options {
 /* Blend every staged splat for every 8x8 pixel block, without the conservative per-block
  * culling. The output is identical; used to validate the culling. */
 #define ANS_FLAG_NO_CULL 1
+/* Batched views only: launch the batch's kernels directly instead of replaying its captured
+ * HIP graph (same results; for comparison). */
+#define ANS_FLAG_NO_GRAPH 2
 
 /* aniso::RenderStats (include/aniso/render.h:9-13) plus the pair-buffer bookkeeping. Times are
  * GPU time per stage from HIP events: project = preprocess + depth sort, sort = pair expansion
@@ -117,6 +120,32 @@ int ans_render_wait(ans_renderer* r, ans_stats* stats);
 int ans_frame_counts(ans_renderer* r, int64_t* n, int64_t* visible, int64_t* pairs, int64_t* tiles);
 int ans_frame_dump(ans_renderer* r, uint32_t* splat_bits, int32_t* rect, uint64_t* keys, uint32_t* values,
                    uint32_t* ranges);
+
+/* ---- Batched multi-view forward (no counterpart in the reference; its nearest caller renders
+ * a camera path one GpuRenderer::render at a time, main.cpp:234-244). Renders n_views
+ * (1..256) cameras of the handle's scene, each of its own size, with one sequence of batched
+ * launches replayed as a captured HIP graph: every view's Gaussians are projected by one
+ * preprocess launch, each sort runs all views' segments at once, and the blend takes every
+ * view's tiles in one launch, longest tile list first. Each view's image equals
+ * ans_render_async of that camera with the same options. dev_rgb[v] is a DEVICE pointer to
+ * cams[v].width * height * 3 bytes. The batch runs on `stream` (a hipStream_t; NULL: the
+ * default stream), after the caller's earlier work there, and the call waits for it; it grows
+ * its pair arena and renders again if a view's pairs exceeded it. stats: NULL or n_views entries
+ * (visible, tile_pairs and pair_capacity per view; blend_ms = the whole batch's GPU time;
+ * project_ms = sort_ms = 0). */
+int ans_render_views(ans_renderer* r, int32_t n_views, const ans_camera* cams, const ans_options* options,
+                     uint8_t* const* dev_rgb, void* stream, ans_stats* stats);
+
+/* Tries every tile shape (8x8, 16x16, 32x16) and LDS batch depth (64, 128, 256) on this batch
+ * (sh_degree, splat_scale and flags from base) and returns the fastest options in *best and
+ * their batch time (the least of `repeats` timed replays after a warm-up) in *best_ms. */
+int ans_views_autotune(ans_renderer* r, int32_t n_views, const ans_camera* cams, const ans_options* base,
+                       uint8_t* const* dev_rgb, void* stream, int32_t repeats, ans_options* best, double* best_ms);
+
+/* Per-view lists of the last ans_render_views call, for parity checks against single-view
+ * renders: counts, then values [pairs] (Gaussian index per sorted pair) and ranges [tiles][2]. */
+int ans_views_frame_counts(ans_renderer* r, int32_t view, int64_t* visible, int64_t* pairs, int64_t* tiles);
+int ans_views_frame_dump(ans_renderer* r, int32_t view, uint32_t* values, uint32_t* ranges);
 
 /* ---- Differentiable path (no counterpart in the reference, whose training calls a
  * third-party rasterizer at pipeline/train_gsplat.py:96-102; this is what a

@@ -1,0 +1,154 @@
+"""Batched multi-view forward on the GPU (ans_render_views), against single-view renders.
+
+Contract: every view of a batch equals a single-view render of that camera with the same
+options, bit for bit: image, visible and pair counts, per-tile lists (values) and tile ranges.
+One small batch is also checked directly against the CPU oracle's key-exact lists. Captured
+graphs are replayed with new poses between eager work, and every replay is checked.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from aniso_amd import scenes
+from aniso_amd.renderer import GpuRenderer, RenderOptions, RenderStats
+
+pytestmark = pytest.mark.gpu
+
+TILES = [(8, 8), (16, 16), (32, 16)]
+
+
+def small_cameras(seed, count=5):
+    """Mixed sizes (odd ones included) and poses inside the synthetic scene's frustum."""
+    rng = np.random.default_rng(seed)
+    sizes = [(320, 180), (333, 217), (640, 360), (97, 61), (480, 270), (256, 256)]
+    cams = []
+    for k in range(count):
+        w, h = sizes[k % len(sizes)]
+        pos = rng.uniform(-0.5, 0.5, 3)
+        yaw = np.radians(rng.uniform(-20, 20))
+        cams.append(scenes.pose_camera(pos, (np.sin(yaw), 0.0, np.cos(yaw)), (0.0, -1.0, 0.0), w, h, 60.0))
+    return cams
+
+
+def device_outputs(cams):
+    return [torch.full((c.height * c.width * 3,), 7, dtype=torch.uint8, device="cuda") for c in cams]
+
+
+def single_view(r, cam, opts):
+    st = RenderStats()
+    img = r.render(cam, st, opts)
+    d = r.dump()
+    return img, st, d
+
+
+def check_batch_against_single(r, cams, opts, label):
+    print(f"case {label}: {len(cams)} views tile={opts.tile} batch={opts.batch_depth} graph={opts.graph}",
+          flush=True)
+    outs = device_outputs(cams)
+    stats = []
+    r.render_views(cams, outs, opts, stats)
+    torch.cuda.synchronize()
+    for v, cam in enumerate(cams):
+        img, st, d = single_view(r, cam, opts)
+        got = outs[v].cpu().numpy().reshape(cam.height, cam.width, 3)
+        vd = r.views_dump(v)
+        assert stats[v].visible == st.visible and stats[v].tile_pairs == st.tile_pairs, f"view {v} counts"
+        assert vd["pairs"] == st.tile_pairs
+        assert np.array_equal(vd["values"], d["values"]), f"view {v} lists differ"
+        assert np.array_equal(vd["ranges"], d["ranges"]), f"view {v} ranges differ"
+        assert np.array_equal(got, img), f"view {v} image differs"
+    return stats
+
+
+@pytest.fixture(scope="module")
+def scene():
+    return scenes.synthetic_scene(120_000, 640, 360, seed=5)
+
+
+@pytest.fixture(scope="module")
+def r(gpu, scene):
+    with GpuRenderer(scene) as rr:
+        yield rr
+
+
+@pytest.mark.parametrize("tile", TILES, ids=["8x8", "16x16", "32x16"])
+def test_views_equal_single_view_renders(r, tile):
+    check_batch_against_single(r, small_cameras(1), RenderOptions(tile=tile, batch_depth=128), f"tile {tile}")
+
+
+def test_views_direct_launches_equal_graph(r):
+    cams = small_cameras(2)
+    opts = RenderOptions(batch_depth=64)
+    a = device_outputs(cams)
+    b = device_outputs(cams)
+    r.render_views(cams, a, opts)
+    r.render_views(cams, b, RenderOptions(batch_depth=64, graph=False))
+    torch.cuda.synchronize()
+    for x, y in zip(a, b):
+        assert torch.equal(x, y)
+
+
+def test_graph_replays_with_new_poses(r):
+    """One graph (same sizes and options), replayed with three pose sets; eager work between."""
+    opts = RenderOptions(tile=(16, 16), batch_depth=256)
+    for seed in (3, 4, 5):
+        check_batch_against_single(r, small_cameras(seed), opts, f"replay seed {seed}")
+        torch.ones(1000, device="cuda").sum().item()
+
+
+def test_views_sh_degree_and_scale_options(r):
+    check_batch_against_single(r, small_cameras(6, count=3), RenderOptions(sh_degree=1, splat_scale=0.7),
+                               "sh1 scale0.7")
+
+
+def test_views_pair_arena_grows(gpu):
+    """A scene whose pairs in one view (about 2.1M) exceed the first arena (max(8 n, 2^20) per
+    view): the batch grows its arena and renders again."""
+    sc = scenes.synthetic_scene(30_000, 640, 360, seed=9)
+    sc.log_scales[:] += 2.0  # large splats: many tiles each
+    cams = small_cameras(7, count=3)
+    with GpuRenderer(sc) as rr:
+        stats = check_batch_against_single(rr, cams, RenderOptions(tile=(8, 8)), "arena growth")
+    assert max(s.tile_pairs for s in stats) > 2 ** 20
+    assert stats[0].retries >= 1
+    assert all(s.pair_capacity >= s.tile_pairs for s in stats)
+
+
+def test_views_lists_match_oracle(gpu):
+    """A small batch's per-view lists against the CPU oracle (key-exact mode) directly."""
+    sc = oracle.mt19937_scene(20000, 7)
+    cams = [scenes.Camera(np.eye(3), np.zeros(3), 0.8 * w, 0.8 * w, w / 2.0, h / 2.0, w, h)
+            for (w, h) in [(320, 240), (333, 217)]]
+    with GpuRenderer(sc, activated=True) as rr:
+        outs = device_outputs(cams)
+        stats = []
+        rr.render_views(cams, outs, RenderOptions(), stats)
+        for v, cam in enumerate(cams):
+            vd = rr.views_dump(v)
+            with oracle.Frame(sc, cam, oracle.KEYEXACT, 3, 1.0, (16, 16)) as f:
+                _, values, ranges = f.pairs_arrays()
+                assert stats[v].visible == f.visible
+                ref = f.image()
+            assert np.array_equal(vd["values"], values)
+            assert np.array_equal(vd["ranges"], ranges)
+            got = outs[v].cpu().numpy().reshape(cam.height, cam.width, 3)
+            assert np.abs(got.astype(int) - ref.astype(int)).max() <= 2
+
+
+def test_1m_gaussians_16_mixed_views(gpu):
+    """The batched workload at full size: 1.0M Gaussians, 16 views of 640x360 to 1920x1080."""
+    sc = scenes.synthetic_scene(1_000_000, 1280, 720)
+    cams = scenes.view_batch_cameras(16)
+    with GpuRenderer(sc) as rr:
+        stats = check_batch_against_single(rr, cams, RenderOptions(), "1M x 16 views")
+    print("pairs per view:", [s.tile_pairs for s in stats], flush=True)
+
+
+def test_autotune_picks_a_supported_config(r):
+    cams = small_cameras(8, count=4)
+    outs = device_outputs(cams)
+    best, ms = r.autotune_views(cams, outs, repeats=1)
+    print(f"autotune: tile {best.tile} batch {best.batch_depth}: {ms:.3f} ms", flush=True)
+    assert best.tile in TILES and best.batch_depth in (64, 128, 256) and ms > 0
+    check_batch_against_single(r, cams, best, "autotuned")
