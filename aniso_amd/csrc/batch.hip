@@ -47,7 +47,8 @@ struct ViewBatch {
     DevBuf<FrameCtl> ctl;  // one per view, plus one for the item-order sort
     DevBuf<float4> rec0, rec1, rec2;
     DevBuf<uint2> rect;
-    DevBuf<uint32_t> tilesTouched, depthKey, order, kA, vA, kB, vB;
+    DevBuf<uint32_t> tilesTouched, depthKey, order, kA, vA, kB, vB, ttByRank;
+    DevBuf<uint2> rectByRank;
     DevBuf<uint32_t> hist, rowTotal, blockPairs, blockVisible;
     DevBuf<uint32_t> tileA, gaussA, tileB, gaussB;
     DevBuf<uint2> ranges;
@@ -127,6 +128,7 @@ void reserveArena(ViewBatch& B, const Layout& L) {
     moved |= grow(B.rec0, vn) | grow(B.rec1, vn) | grow(B.rec2, vn) | grow(B.rect, vn);
     moved |= grow(B.tilesTouched, vn) | grow(B.depthKey, vn) | grow(B.order, vn);
     moved |= grow(B.kA, vn) | grow(B.vA, vn) | grow(B.kB, vn) | grow(B.vB, vn);
+    moved |= grow(B.ttByRank, vn) | grow(B.rectByRank, vn);
     const size_t histD = V * kRadix * size_t(L.blocksD), histP = V * kRadix * size_t(L.blocksP),
                  histI = size_t(kRadix) * L.blocksI;
     moved |= grow(B.hist, std::max({histD, histP, histI}));
@@ -186,18 +188,18 @@ const uint32_t* enqueueBatch(ViewBatch& B, const SceneDev& s, const Layout& L, h
                                                            cams, B.rec0.p, B.rec1.p, B.rec2.p, B.rect.p,
                                                            B.tilesTouched.p, B.depthKey.p);
         // Depth sort of each view's Gaussians: four stable 8-bit passes.
-        const uint32_t histD = kRadix * L.blocksD;
+        const uint32_t histD = kRadix * L.blocksD;  // depth sort: kPairItems keys per thread
         const uint32_t* kin = B.depthKey.p;
         const uint32_t* vin = nullptr;
         uint32_t* kout[4] = {B.kA.p, B.kB.p, B.kA.p, nullptr};
         uint32_t* vout[4] = {B.vA.p, B.vB.p, B.vA.p, B.order.p};
         const dim3 gD(L.blocksD, V);
         for (int pass = 0; pass < 4; ++pass) {
-            k_radixUpsweep<kDepthItems><<<gD, kSortBlock, 0, st>>>(kin, false, n, n, 8 * pass, 8, B.hist.p, L.blocksD,
-                                                                   ctl, n, histD);
+            k_radixUpsweep<kPairItems><<<gD, kSortBlock, 0, st>>>(kin, false, n, n, 8 * pass, 8, B.hist.p, L.blocksD,
+                                                                  ctl, n, histD);
             k_scanRows<<<dim3(kRadix, V), kSortBlock, 0, st>>>(B.hist.p, L.blocksD, L.blocksD, B.rowTotal.p, ctl,
-                                                               false, n, kSortBlock * kDepthItems, histD);
-            k_radixDownsweep<kDepthItems><<<gD, kSortBlock, 0, st>>>(kin, vin, kout[pass], vout[pass], false, n, n,
+                                                               false, n, kSortBlock * kPairItems, histD);
+            k_radixDownsweep<kPairItems><<<gD, kSortBlock, 0, st>>>(kin, vin, kout[pass], vout[pass], false, n, n,
                                                                      8 * pass, 8, B.hist.p, B.rowTotal.p, L.blocksD,
                                                                      ctl, n, histD);
             kin = kout[pass];
@@ -205,11 +207,11 @@ const uint32_t* enqueueBatch(ViewBatch& B, const SceneDev& s, const Layout& L, h
         }
         // Pairs of each view in depth order, into the view's region of the pair arena.
         const dim3 gDup(L.blocksDup, V);
-        k_pairTotals<<<gDup, kDupBlock, 0, st>>>(B.order.p, B.tilesTouched.p, n, B.blockPairs.p, B.blockVisible.p, n,
-                                                 L.blocksDup);
+        k_pairTotals<<<gDup, kDupBlock, 0, st>>>(B.order.p, B.tilesTouched.p, B.rect.p, n, B.ttByRank.p,
+                                                 B.rectByRank.p, B.blockPairs.p, B.blockVisible.p, n, L.blocksDup);
         k_scanPairs<<<dim3(1, V), kDupBlock, 0, st>>>(B.blockPairs.p, B.blockVisible.p, L.blocksDup, cap, ctl,
                                                       L.blocksDup);
-        k_expandPairs<<<gDup, kDupBlock, 0, st>>>(B.order.p, B.tilesTouched.p, B.rect.p, n, 0, B.blockPairs.p,
+        k_expandPairs<<<gDup, kDupBlock, 0, st>>>(B.order.p, B.ttByRank.p, B.rectByRank.p, n, 0, B.blockPairs.p,
                                                   B.tileA.p, B.gaussA.p, ctl, n, L.blocksDup, cap, tilesX);
     }
     // Stable sort of each view's pairs by tile id.
@@ -234,7 +236,7 @@ const uint32_t* enqueueBatch(ViewBatch& B, const SceneDev& s, const Layout& L, h
     const uint32_t* sortedTile = tk[L.passes & 1];
     const uint32_t* sortedGauss = tv[L.passes & 1];
     if (n > 0) {
-        const uint32_t rangeBlocks = std::max<uint32_t>(1, std::min<uint32_t>((cap + 255) / 256, 2048));
+        const uint32_t rangeBlocks = std::max<uint32_t>(1, std::min<uint32_t>((cap + 1023) / 1024, 2048));
         k_tileRanges<<<dim3(rangeBlocks, V), 256, 0, st>>>(sortedTile, ctl, B.ranges.p, cap, L.maxTiles);
     }
     // Work items longest list first: a stable sort of (view, tile) items by a 16-bit key.
@@ -285,7 +287,7 @@ Layout makeLayout(const ViewBatch& B, const SceneDev& s, int views, const std::v
     L.tileBits = std::max(1, bitsFor(maxT - 1));
     L.passes = (L.tileBits + 7) / 8;
     L.digitBits = (L.tileBits + L.passes - 1) / L.passes;
-    L.blocksD = std::max<uint32_t>(1, (L.n + kSortBlock * kDepthItems - 1) / (kSortBlock * kDepthItems));
+    L.blocksD = std::max<uint32_t>(1, (L.n + kSortBlock * kPairItems - 1) / (kSortBlock * kPairItems));
     L.blocksDup = std::max<uint32_t>(1, (L.n + kDupTile - 1) / kDupTile);
     L.pairCap = B.pairCap ? B.pairCap : std::max<size_t>(size_t(L.n) * 8, size_t(1) << 20);
     L.blocksP = static_cast<uint32_t>((L.pairCap + kSortBlock * kPairItems - 1) / (kSortBlock * kPairItems));

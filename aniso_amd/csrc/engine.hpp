@@ -90,6 +90,8 @@ struct Engine {
     FrameCtl* ctlHost = nullptr;  // pinned
 
     DevBuf<uint32_t> depthKey, sortKA, sortVA, sortKB, sortVB;
+    DevBuf<uint32_t> ttByRank;      // tile counts in depth order
+    DevBuf<uint2> rectByRank;       // tile rects in depth order
     DevBuf<uint32_t> hist;          // per-block digit histograms, scanned in place
     DevBuf<uint32_t> rowTotal;      // per-digit totals of one pass
     DevBuf<uint32_t> blockPairs;    // pairs per expansion block, scanned in place

@@ -288,9 +288,14 @@ constexpr int kDupTile = kDupBlock;                  // depth ranks per block
 // totals by view * blockStride, pair arrays by view * pairStride and use ctl[view]; a single frame
 // is gridDim.y = 1.
 
-// Pairs and visible Gaussians per block of 256 depth ranks.
+// Pairs and visible Gaussians per block of 256 depth ranks. Also gathers each rank's tile count
+// and tile rect into depth order (ttByRank, rectByRank; rect left unwritten for culled ranks), so
+// that the expansion reads them with coalesced loads.
 static __global__ __launch_bounds__(kDupBlock) void k_pairTotals(const uint32_t* __restrict__ sortedIdx,
-                                                                 const uint32_t* __restrict__ tilesTouched, uint32_t n,
+                                                                 const uint32_t* __restrict__ tilesTouched,
+                                                                 const uint2* __restrict__ rect, uint32_t n,
+                                                                 uint32_t* __restrict__ ttByRank,
+                                                                 uint2* __restrict__ rectByRank,
                                                                  uint32_t* __restrict__ blockPairs,
                                                                  uint32_t* __restrict__ blockVisible,
                                                                  uint32_t gaussStride, uint32_t blockStride) {
@@ -298,10 +303,19 @@ static __global__ __launch_bounds__(kDupBlock) void k_pairTotals(const uint32_t*
     const size_t go = static_cast<size_t>(blockIdx.y) * gaussStride;
     sortedIdx += go;
     tilesTouched += go;
+    rect += go;
+    ttByRank += go;
+    rectByRank += go;
     blockPairs += static_cast<size_t>(blockIdx.y) * blockStride;
     blockVisible += static_cast<size_t>(blockIdx.y) * blockStride;
     const uint32_t r = blockIdx.x * kDupTile + threadIdx.x;
-    const uint32_t c = r < n ? tilesTouched[sortedIdx[r]] : 0u;
+    uint32_t c = 0;
+    if (r < n) {
+        const uint32_t g = sortedIdx[r];
+        c = tilesTouched[g];
+        ttByRank[r] = c;
+        if (c) rectByRank[r] = rect[g];
+    }
     uint32_t total, visible;
     blockExclusiveScan<kDupBlock>(c, sTmp, &total);
     blockExclusiveScan<kDupBlock>(c ? 1u : 0u, sTmp, &visible);
@@ -375,8 +389,8 @@ __device__ inline int waveInclusiveMax(int v) {
 // a wavefront max-scan.
 // tilesXView (batched views, else null) holds each view's tile-grid width.
 static __global__ __launch_bounds__(kDupBlock) void k_expandPairs(
-    const uint32_t* __restrict__ sortedIdx, const uint32_t* __restrict__ tilesTouched, const uint2* __restrict__ rect,
-    uint32_t n, int tilesX, const uint32_t* __restrict__ blockBase,
+    const uint32_t* __restrict__ sortedIdx, const uint32_t* __restrict__ ttByRank,
+    const uint2* __restrict__ rectByRank, uint32_t n, int tilesX, const uint32_t* __restrict__ blockBase,
     uint32_t* __restrict__ tileOut, uint32_t* __restrict__ gaussOut, const FrameCtl* __restrict__ ctl,
     uint32_t gaussStride, uint32_t blockStride, uint32_t pairStride, const int* __restrict__ tilesXView) {
     __shared__ uint32_t sWave[kDupBlock / kWave];
@@ -386,8 +400,8 @@ static __global__ __launch_bounds__(kDupBlock) void k_expandPairs(
         ctl += view;
         const size_t go = static_cast<size_t>(view) * gaussStride, po = static_cast<size_t>(view) * pairStride;
         sortedIdx += go;
-        tilesTouched += go;
-        rect += go;
+        ttByRank += go;
+        rectByRank += go;
         blockBase += static_cast<size_t>(view) * blockStride;
         tileOut += po;
         gaussOut += po;
@@ -399,9 +413,11 @@ static __global__ __launch_bounds__(kDupBlock) void k_expandPairs(
     uint32_t g = 0, cnt = 0;
     uint2 rp = make_uint2(0, 0);
     if (r < n) {
-        g = sortedIdx[r];
-        cnt = tilesTouched[g];
-        if (cnt) rp = rect[g];
+        cnt = ttByRank[r];
+        if (cnt) {
+            rp = rectByRank[r];
+            g = sortedIdx[r];
+        }
     }
     const uint32_t incl = waveInclusiveScan(cnt);
     const uint32_t excl = incl - cnt;
@@ -448,7 +464,7 @@ static __global__ __launch_bounds__(kDupBlock) void k_expandPairs(
 }
 
 // Start and end of each tile's run in the sorted pairs (gpu_render.cu:164-177); ranges were
-// cleared to [0, 0) by the frame init. Grid-stride over the pairs.
+// cleared to [0, 0) by the frame init. Grid-stride over groups of four pairs.
 static __global__ __launch_bounds__(256) void k_tileRanges(const uint32_t* __restrict__ sortedTile,
                                                            const FrameCtl* __restrict__ ctl,
                                                            uint2* __restrict__ ranges, uint32_t pairStride,
@@ -458,10 +474,22 @@ static __global__ __launch_bounds__(256) void k_tileRanges(const uint32_t* __res
     ranges += static_cast<size_t>(blockIdx.y) * tileStride;
     if (ctl->overflow) return;
     const uint32_t total = ctl->pairs;
-    for (uint32_t p = blockIdx.x * 256 + threadIdx.x; p < total; p += gridDim.x * 256) {
-        const uint32_t t = sortedTile[p];
-        if (p == 0 || sortedTile[p - 1] != t) ranges[t].x = p;
-        if (p == total - 1 || sortedTile[p + 1] != t) ranges[t].y = p + 1;
+    for (uint32_t p0 = (blockIdx.x * 256 + threadIdx.x) * 4; p0 < total; p0 += gridDim.x * 256 * 4) {
+        // Pairs p0 .. p0 + 3 and their neighbours, all loads issued before any comparison.
+        uint32_t t[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const uint32_t p = p0 + k - 1;  // p0 - 1 wraps for p0 = 0 and is not read
+            t[k] = (k > 0 || p0 > 0) && p < total ? sortedTile[p] : 0xffffffffu;
+        }
+#pragma unroll
+        for (int k = 1; k <= 4; ++k) {
+            const uint32_t p = p0 + k - 1;
+            if (p < total) {
+                if (p == 0 || t[k - 1] != t[k]) ranges[t[k]].x = p;
+                if (p == total - 1 || t[k + 1] != t[k]) ranges[t[k]].y = p + 1;
+            }
+        }
     }
 }
 

@@ -117,6 +117,7 @@ Engine::~Engine() {
 void Engine::reserveScratch(size_t n, size_t pairCap) {
     const size_t m = std::max<size_t>(n, 1);
     depthKey.reserve(m); sortKA.reserve(m); sortVA.reserve(m); sortKB.reserve(m); sortVB.reserve(m);
+    ttByRank.reserve(m); rectByRank.reserve(m);
     const size_t blocksDepth = (n + kSortBlock * kDepthItems - 1) / (kSortBlock * kDepthItems);
     const size_t blocksDup = (n + kDupTile - 1) / kDupTile;
     const size_t blocksPair = (pairCap + kSortBlock * kPairItems - 1) / (kSortBlock * kPairItems);
@@ -185,10 +186,10 @@ void Engine::enqueueFrame(FrameState& f, const SceneDev& s, const CamParams& cp,
     if (n > 0) {
         const uint32_t blocksDup = (n + kDupTile - 1) / kDupTile;
         const uint32_t cap = static_cast<uint32_t>(f.pairCap);
-        k_pairTotals<<<blocksDup, kDupBlock, 0, stream>>>(f.order.p, f.tilesTouched.p, n, blockPairs.p,
-                                                          blockVisible.p, 0, 0);
+        k_pairTotals<<<blocksDup, kDupBlock, 0, stream>>>(f.order.p, f.tilesTouched.p, f.rect.p, n, ttByRank.p,
+                                                          rectByRank.p, blockPairs.p, blockVisible.p, 0, 0);
         k_scanPairs<<<1, kDupBlock, 0, stream>>>(blockPairs.p, blockVisible.p, blocksDup, cap, ctl, 0);
-        k_expandPairs<<<blocksDup, kDupBlock, 0, stream>>>(f.order.p, f.tilesTouched.p, f.rect.p, n, cp.tilesX,
+        k_expandPairs<<<blocksDup, kDupBlock, 0, stream>>>(f.order.p, ttByRank.p, rectByRank.p, n, cp.tilesX,
                                                            blockPairs.p, f.tileA.p, f.gaussA.p, ctl, 0, 0, 0,
                                                            nullptr);
         const uint32_t blocksPair =
@@ -209,8 +210,8 @@ void Engine::enqueueFrame(FrameState& f, const SceneDev& s, const CamParams& cp,
         }
         f.sortedTile = tk[passes & 1];
         f.sortedGauss = tv[passes & 1];
-        k_tileRanges<<<static_cast<uint32_t>((f.pairCap + 255) / 256), 256, 0, stream>>>(f.sortedTile, ctl,
-                                                                                          f.ranges.p, 0, 0);
+        k_tileRanges<<<static_cast<uint32_t>((f.pairCap + 1023) / 1024), 256, 0, stream>>>(f.sortedTile, ctl,
+                                                                                            f.ranges.p, 0, 0);
     }
     hipCheck(hipEventRecord(ev[2], stream), "hipEventRecord");
     launchBlend(f, out8, outf);
