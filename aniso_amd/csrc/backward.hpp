@@ -91,8 +91,7 @@ constexpr int kPartialStride = 12;  // floats per pair record: u v A B C opacity
 // partial[slot * 12 + {u, v, A, B, C, opacity, r, g, b}].
 template <int TW, int TH, int BATCH>
 __global__ __launch_bounds__(TW* TH) void k_blendBackward(
-    const uint2* __restrict__ ranges, const uint32_t* __restrict__ sortedGauss, const float4* __restrict__ rec0,
-    const float4* __restrict__ rec1, const float4* __restrict__ rec2, const uint2* __restrict__ rect,
+    const uint2* __restrict__ ranges, const uint32_t* __restrict__ sortedGauss, const SplatRec* __restrict__ rec,
     const uint32_t* __restrict__ slotStart, const uint2* __restrict__ pixState, int width, int height, int tilesX,
     const float* __restrict__ gradImage, float* __restrict__ partial, const FrameCtl* __restrict__ ctl) {
     constexpr int kThreads = TW * TH;
@@ -126,13 +125,14 @@ __global__ __launch_bounds__(TW* TH) void k_blendBackward(
         const int cnt = min(static_cast<uint32_t>(BATCH), range.y - start);
         for (int k = tid; k < cnt; k += kThreads) {
             const uint32_t g = sortedGauss[start + k];
-            const float4 a = rec0[g];
-            const float4 bb = rec1[g];
-            const uint2 rp = rect[g];
+            const float4 a = rec[g].geo;
+            const float4 bb = rec[g].geo2;
+            const uint4 meta = rec[g].meta;
+            const uint2 rp = make_uint2(meta.x, meta.y);
             const uint32_t first = slotStart[g];
             s0[k] = a;
             s1[k] = bb;
-            s2[k] = rec2[g];
+            s2[k] = rec[g].col;
             const uint32_t x0 = rp.x & 0xffffu, y0 = rp.x >> 16, w = (rp.y & 0xffffu) - x0 + 1;
             sSlot[k] = first + (static_cast<uint32_t>(tileY) - y0) * w + (static_cast<uint32_t>(tileX) - x0);
             uint32_t m = 0;
@@ -271,7 +271,8 @@ __global__ __launch_bounds__(TW* TH) void k_blendBackward(
     // Pairs after the point where every pixel had stopped get zero partials.
     for (uint32_t k = static_cast<uint32_t>(nbDone) * BATCH + tid; k < len; k += kThreads) {
         const uint32_t g = sortedGauss[range.x + k];
-        const uint2 rp = rect[g];
+        const uint4 m = rec[g].meta;
+        const uint2 rp = make_uint2(m.x, m.y);
         const uint32_t x0 = rp.x & 0xffffu, y0 = rp.x >> 16, w = (rp.y & 0xffffu) - x0 + 1;
         const uint32_t slot = slotStart[g] + (static_cast<uint32_t>(tileY) - y0) * w + (static_cast<uint32_t>(tileX) - x0);
         float4* out = reinterpret_cast<float4*>(partial + static_cast<size_t>(slot) * kPartialStride);

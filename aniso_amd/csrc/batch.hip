@@ -45,8 +45,7 @@ struct ViewBatch {
     // Grow-only arena. Cached graphs bake its addresses, so growth of any buffer drops them.
     DevBuf<uint8_t> table;
     DevBuf<FrameCtl> ctl;  // one per view, plus one for the item-order sort
-    DevBuf<float4> rec0, rec1, rec2;
-    DevBuf<uint2> rect;
+    DevBuf<SplatRec> rec;
     DevBuf<uint32_t> tilesTouched, depthKey, order, kA, vA, kB, vB, ttByRank;
     DevBuf<uint2> rectByRank;
     DevBuf<uint32_t> hist, rowTotal, blockPairs, blockVisible;
@@ -125,7 +124,7 @@ void reserveArena(ViewBatch& B, const Layout& L) {
     bool moved = false;
     moved |= grow(B.table, TableLayout(L.views).bytes);
     moved |= grow(B.ctl, V + 1);
-    moved |= grow(B.rec0, vn) | grow(B.rec1, vn) | grow(B.rec2, vn) | grow(B.rect, vn);
+    moved |= grow(B.rec, vn);
     moved |= grow(B.tilesTouched, vn) | grow(B.depthKey, vn) | grow(B.order, vn);
     moved |= grow(B.kA, vn) | grow(B.vA, vn) | grow(B.kB, vn) | grow(B.vB, vn);
     moved |= grow(B.ttByRank, vn) | grow(B.rectByRank, vn);
@@ -154,12 +153,12 @@ void launchBlendViews(ViewBatch& B, const Layout& L, const TableLayout& T, const
     uint8_t* const* outs = reinterpret_cast<uint8_t* const*>(B.table.p + T.outs);
     if (L.cull)
         k_blendViews<TW, TH, BATCH, true><<<L.totalTiles, TW * TH, 0, st>>>(
-            B.itemOrder.p, L.maxTiles, B.ranges.p, sortedGauss, static_cast<uint32_t>(L.pairCap), B.rec0.p, B.rec1.p,
-            B.rec2.p, L.n, cams, outs, B.ctl.p);
+            B.itemOrder.p, L.maxTiles, B.ranges.p, sortedGauss, static_cast<uint32_t>(L.pairCap), B.rec.p, L.n, cams,
+            outs, B.ctl.p);
     else
         k_blendViews<TW, TH, BATCH, false><<<L.totalTiles, TW * TH, 0, st>>>(
-            B.itemOrder.p, L.maxTiles, B.ranges.p, sortedGauss, static_cast<uint32_t>(L.pairCap), B.rec0.p, B.rec1.p,
-            B.rec2.p, L.n, cams, outs, B.ctl.p);
+            B.itemOrder.p, L.maxTiles, B.ranges.p, sortedGauss, static_cast<uint32_t>(L.pairCap), B.rec.p, L.n, cams,
+            outs, B.ctl.p);
 }
 
 template <int TW, int TH>
@@ -185,8 +184,7 @@ const uint32_t* enqueueBatch(ViewBatch& B, const SceneDev& s, const Layout& L, h
     if (n > 0) {
         k_preprocessViews<<<(n + 255) / 256, 256, 0, st>>>(static_cast<int>(n), L.views, s.pos, s.scale, s.rot,
                                                            s.opacity, s.sh, (s.shDegree + 1) * (s.shDegree + 1),
-                                                           cams, B.rec0.p, B.rec1.p, B.rec2.p, B.rect.p,
-                                                           B.tilesTouched.p, B.depthKey.p);
+                                                           cams, B.rec.p, B.tilesTouched.p, B.depthKey.p);
         // Depth sort of each view's Gaussians: four stable 8-bit passes.
         const uint32_t histD = kRadix * L.blocksD;  // depth sort: kPairItems keys per thread
         const uint32_t* kin = B.depthKey.p;
@@ -207,8 +205,8 @@ const uint32_t* enqueueBatch(ViewBatch& B, const SceneDev& s, const Layout& L, h
         }
         // Pairs of each view in depth order, into the view's region of the pair arena.
         const dim3 gDup(L.blocksDup, V);
-        k_pairTotals<<<gDup, kDupBlock, 0, st>>>(B.order.p, B.tilesTouched.p, B.rect.p, n, B.ttByRank.p,
-                                                 B.rectByRank.p, B.blockPairs.p, B.blockVisible.p, n, L.blocksDup);
+        k_pairTotals<<<gDup, kDupBlock, 0, st>>>(B.order.p, B.rec.p, n, B.ttByRank.p, B.rectByRank.p, B.blockPairs.p,
+                                                 B.blockVisible.p, n, L.blocksDup);
         k_scanPairs<<<dim3(1, V), kDupBlock, 0, st>>>(B.blockPairs.p, B.blockVisible.p, L.blocksDup, cap, ctl,
                                                       L.blocksDup);
         k_expandPairs<<<gDup, kDupBlock, 0, st>>>(B.order.p, B.ttByRank.p, B.rectByRank.p, n, 0, B.blockPairs.p,

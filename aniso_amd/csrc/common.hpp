@@ -25,6 +25,15 @@ struct CamParams {
     float splatScale;
 };
 
+// One Gaussian's splat record: 64 bytes, one aligned cache line, so that the per-pair gathers of
+// the blend, its adjoint and the pair expansion each touch a single line.
+struct alignas(64) SplatRec {
+    float4 geo;   // u, v (pixels), conic A, B
+    float4 geo2;  // conic C, opacity, alpha-region half-extents x, y
+    float4 col;   // r, g, b, depth
+    uint4 meta;   // tile rect (x0 | y0 << 16, x1 | y1 << 16), tiles touched (0: culled), 0
+};
+
 // Small per-frame control block in device memory. The init kernel clears it; kernels publish
 // counts into it; the host reads it back with the image.
 struct FrameCtl {

@@ -20,8 +20,8 @@ static void launchBlendBackward(Engine& e, FrameState& f, const float* gradImage
     const CamParams& cp = f.cp;
     const uint32_t tiles = static_cast<uint32_t>(cp.tilesX) * cp.tilesY;
     k_blendBackward<TW, TH, kBatch><<<tiles, TW * TH, 0, e.stream>>>(
-        f.ranges.p, f.sortedGauss, f.rec0.p, f.rec1.p, f.rec2.p, f.rect.p, e.slotStart.p, f.pixState.p, cp.width,
-        cp.height, cp.tilesX, gradImage, e.partial.p, f.ctl.p);
+        f.ranges.p, f.sortedGauss, f.rec.p, e.slotStart.p, f.pixState.p, cp.width, cp.height, cp.tilesX, gradImage,
+        e.partial.p, f.ctl.p);
 }
 
 void Engine::enqueueBackward(FrameState& f, const SceneDev& s, const float* gradImage, float* gMeans,

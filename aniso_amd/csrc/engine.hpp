@@ -58,8 +58,7 @@ void checkCamera(const ans_camera* cam);
 // Everything one frame produces that its blend and its backward pass read.
 struct FrameState {
     DevBuf<FrameCtl> ctl;
-    DevBuf<float4> rec0, rec1, rec2;
-    DevBuf<uint2> rect;
+    DevBuf<SplatRec> rec;
     DevBuf<uint32_t> tilesTouched;
     DevBuf<uint32_t> order;  // Gaussians by depth (rank -> index)
     DevBuf<uint32_t> tileA, gaussA, tileB, gaussB;
@@ -86,7 +85,11 @@ struct Engine {
     int device = 0;
     hipStream_t ownStream = nullptr;
     hipStream_t stream = nullptr;
-    hipEvent_t ev[4] = {};
+    // Stage events of the last kEventRing frames (start, after project, after sort, end); stats
+    // average the frames enqueued since the previous readout.
+    static constexpr int kEventRing = 64;
+    hipEvent_t ev[kEventRing][4] = {};
+    int evNext = 0, evPending = 0;
     FrameCtl* ctlHost = nullptr;  // pinned
 
     DevBuf<uint32_t> depthKey, sortKA, sortVA, sortKB, sortVB;

@@ -233,9 +233,17 @@ __host__ __device__ inline bool projectGaussian(int i, const float* __restrict__
     return true;
 }
 
-// Writes a visible Gaussian's splat record, rect, tile count and depth key at index o.
-__device__ inline void storeSplat(const GaussProj& g, size_t o, float4* __restrict__ rec0, float4* __restrict__ rec1,
-                                  float4* __restrict__ rec2, uint2* __restrict__ rect,
+// Writes a culled Gaussian's tile count (0, also in its record) and depth key (the largest).
+__device__ inline void storeCulled(size_t o, SplatRec* __restrict__ rec, uint32_t* __restrict__ tilesTouched,
+                                   uint32_t* __restrict__ depthKey) {
+    rec[o].meta = make_uint4(0u, 0u, 0u, 0u);
+    tilesTouched[o] = 0;
+    depthKey[o] = 0xffffffffu;
+}
+
+// Writes a visible Gaussian's splat record (with its tile rect and count), tile count and depth
+// key at index o.
+__device__ inline void storeSplat(const GaussProj& g, size_t o, SplatRec* __restrict__ rec,
                                   uint32_t* __restrict__ tilesTouched, uint32_t* __restrict__ depthKey) {
     // Region where op * exp(-q/2) >= 1/255: q <= 2 ln(255 op); half-extents sqrt(k a), sqrt(k c)
     // of that ellipse, widened by a 5% + 0.1 margin far above the FP32 rounding of the blend's
@@ -246,12 +254,14 @@ __device__ inline void storeSplat(const GaussProj& g, size_t o, float4* __restri
         ex = sqrtf(k * g.a) + 0.01f;
         ey = sqrtf(k * g.c) + 0.01f;
     }
-    rec0[o] = make_float4(g.u, g.v, g.A, g.B);
-    rec1[o] = make_float4(g.C, g.opacity, ex, ey);
-    rec2[o] = make_float4(g.rgb.x, g.rgb.y, g.rgb.z, g.depth);
-    rect[o] = make_uint2(static_cast<uint32_t>(g.x0) | (static_cast<uint32_t>(g.y0) << 16),
-                         static_cast<uint32_t>(g.x1) | (static_cast<uint32_t>(g.y1) << 16));
-    tilesTouched[o] = static_cast<uint32_t>((g.x1 - g.x0 + 1) * (g.y1 - g.y0 + 1));
+    const uint32_t tt = static_cast<uint32_t>((g.x1 - g.x0 + 1) * (g.y1 - g.y0 + 1));
+    SplatRec& r = rec[o];
+    r.geo = make_float4(g.u, g.v, g.A, g.B);
+    r.geo2 = make_float4(g.C, g.opacity, ex, ey);
+    r.col = make_float4(g.rgb.x, g.rgb.y, g.rgb.z, g.depth);
+    r.meta = make_uint4(static_cast<uint32_t>(g.x0) | (static_cast<uint32_t>(g.y0) << 16),
+                        static_cast<uint32_t>(g.x1) | (static_cast<uint32_t>(g.y1) << 16), tt, 0u);
+    tilesTouched[o] = tt;
     depthKey[o] = __float_as_uint(g.depth);
 }
 
@@ -264,9 +274,8 @@ template <bool PLANAR, bool RAW>
 __global__ __launch_bounds__(256) void k_preprocess(int n, const float* __restrict__ pos,
                                                     const float* __restrict__ scale, const float4* __restrict__ rot,
                                                     const float* __restrict__ opacity, const float* __restrict__ sh,
-                                                    CamParams cam, float4* __restrict__ rec0,
-                                                    float4* __restrict__ rec1, float4* __restrict__ rec2,
-                                                    uint2* __restrict__ rect, uint32_t* __restrict__ tilesTouched,
+                                                    CamParams cam, SplatRec* __restrict__ rec,
+                                                    uint32_t* __restrict__ tilesTouched,
                                                     uint32_t* __restrict__ depthKey) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
@@ -274,11 +283,10 @@ __global__ __launch_bounds__(256) void k_preprocess(int n, const float* __restri
     const float* shBase = PLANAR ? sh + i : sh + static_cast<size_t>(i) * cam.shCoeffs * 3;
     if (!projectGaussian<PLANAR, RAW>(i, pos, scale, rot, opacity, shBase, PLANAR ? static_cast<size_t>(n) : 1, cam,
                                       g)) {
-        tilesTouched[i] = 0;
-        depthKey[i] = 0xffffffffu;
+        storeCulled(i, rec, tilesTouched, depthKey);
         return;
     }
-    storeSplat(g, i, rec0, rec1, rec2, rect, tilesTouched, depthKey);
+    storeSplat(g, i, rec, tilesTouched, depthKey);
 }
 
 constexpr int kDupBlock = 256;                       // 4 wavefronts x 64 depth ranks
@@ -289,11 +297,10 @@ constexpr int kDupTile = kDupBlock;                  // depth ranks per block
 // is gridDim.y = 1.
 
 // Pairs and visible Gaussians per block of 256 depth ranks. Also gathers each rank's tile count
-// and tile rect into depth order (ttByRank, rectByRank; rect left unwritten for culled ranks), so
-// that the expansion reads them with coalesced loads.
+// and tile rect (one line of its splat record) into depth order (ttByRank, rectByRank; rect left
+// unwritten for culled ranks), so that the expansion reads them with coalesced loads.
 static __global__ __launch_bounds__(kDupBlock) void k_pairTotals(const uint32_t* __restrict__ sortedIdx,
-                                                                 const uint32_t* __restrict__ tilesTouched,
-                                                                 const uint2* __restrict__ rect, uint32_t n,
+                                                                 const SplatRec* __restrict__ rec, uint32_t n,
                                                                  uint32_t* __restrict__ ttByRank,
                                                                  uint2* __restrict__ rectByRank,
                                                                  uint32_t* __restrict__ blockPairs,
@@ -302,8 +309,7 @@ static __global__ __launch_bounds__(kDupBlock) void k_pairTotals(const uint32_t*
     __shared__ uint32_t sTmp[kDupBlock / kWave];
     const size_t go = static_cast<size_t>(blockIdx.y) * gaussStride;
     sortedIdx += go;
-    tilesTouched += go;
-    rect += go;
+    rec += go;
     ttByRank += go;
     rectByRank += go;
     blockPairs += static_cast<size_t>(blockIdx.y) * blockStride;
@@ -311,10 +317,10 @@ static __global__ __launch_bounds__(kDupBlock) void k_pairTotals(const uint32_t*
     const uint32_t r = blockIdx.x * kDupTile + threadIdx.x;
     uint32_t c = 0;
     if (r < n) {
-        const uint32_t g = sortedIdx[r];
-        c = tilesTouched[g];
+        const uint4 m = rec[sortedIdx[r]].meta;
+        c = m.z;
         ttByRank[r] = c;
-        if (c) rectByRank[r] = rect[g];
+        if (c) rectByRank[r] = make_uint2(m.x, m.y);
     }
     uint32_t total, visible;
     blockExclusiveScan<kDupBlock>(c, sTmp, &total);
@@ -526,8 +532,7 @@ __device__ inline float quadMinOverBox(float A, float B, float C, float rB_C, fl
 // position + 1 in the tile's list of the last splat blended into it (0: none).
 template <int TW, int TH, int BATCH, bool CULL>
 __device__ inline void blendTile(int tileX, int tileY, uint2 range, const uint32_t* __restrict__ sortedGauss,
-                                 const float4* __restrict__ rec0, const float4* __restrict__ rec1,
-                                 const float4* __restrict__ rec2, int width, int height, uint8_t* __restrict__ out8,
+                                 const SplatRec* __restrict__ rec, int width, int height, uint8_t* __restrict__ out8,
                                  float* __restrict__ outf, uint2* __restrict__ pixState) {
     constexpr int kThreads = TW * TH;
     constexpr int kWaves = kThreads / kWave;
@@ -554,11 +559,11 @@ __device__ inline void blendTile(int tileX, int tileY, uint2 range, const uint32
         const int cnt = min(static_cast<uint32_t>(BATCH), range.y - start);
         for (int k = tid; k < cnt; k += kThreads) {
             const uint32_t g = sortedGauss[start + k];
-            const float4 a = rec0[g];
-            const float4 b = rec1[g];
+            const float4 a = rec[g].geo;
+            const float4 b = rec[g].geo2;
             s0[k] = a;
             s1[k] = b;
-            s2[k] = rec2[g];
+            s2[k] = rec[g].col;
             if constexpr (CULL) {
                 uint32_t m = 0;
                 const float op = b.y;
@@ -638,14 +643,13 @@ __device__ inline void blendTile(int tileX, int tileY, uint2 range, const uint32
 template <int TW, int TH, int BATCH, bool CULL>
 __global__ __launch_bounds__(TW* TH) void k_blend(const uint2* __restrict__ ranges,
                                                   const uint32_t* __restrict__ sortedGauss,
-                                                  const float4* __restrict__ rec0, const float4* __restrict__ rec1,
-                                                  const float4* __restrict__ rec2, int width, int height, int tilesX,
+                                                  const SplatRec* __restrict__ rec, int width, int height, int tilesX,
                                                   uint8_t* __restrict__ out8, float* __restrict__ outf,
                                                   uint2* __restrict__ pixState, const FrameCtl* __restrict__ ctl) {
     if (ctl->overflow) return;
     const int tile = blockIdx.x;
-    blendTile<TW, TH, BATCH, CULL>(tile % tilesX, tile / tilesX, ranges[tile], sortedGauss, rec0, rec1, rec2, width,
-                                   height, out8, outf, pixState);
+    blendTile<TW, TH, BATCH, CULL>(tile % tilesX, tile / tilesX, ranges[tile], sortedGauss, rec, width, height, out8,
+                                   outf, pixState);
 }
 
 }  // namespace ans

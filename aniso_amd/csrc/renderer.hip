@@ -87,8 +87,7 @@ FrameState::FrameState() {
 
 void FrameState::reserve(size_t gaussians, size_t tileCount) {
     const size_t m = std::max<size_t>(gaussians, 1);
-    rec0.reserve(m); rec1.reserve(m); rec2.reserve(m);
-    rect.reserve(m); tilesTouched.reserve(m); order.reserve(m);
+    rec.reserve(m); tilesTouched.reserve(m); order.reserve(m);
     ranges.reserve(std::max<size_t>(tileCount, 1));
 }
 
@@ -102,15 +101,17 @@ Engine::Engine() {
     hipCheck(hipGetDevice(&device), "hipGetDevice");
     hipCheck(hipStreamCreateWithFlags(&ownStream, hipStreamNonBlocking), "hipStreamCreate");
     stream = ownStream;
-    for (auto& e : ev) hipCheck(hipEventCreate(&e), "hipEventCreate");
+    for (auto& f : ev)
+        for (auto& e : f) hipCheck(hipEventCreate(&e), "hipEventCreate");
     hipCheck(hipHostMalloc(reinterpret_cast<void**>(&ctlHost), sizeof(FrameCtl), hipHostMallocDefault),
              "hipHostMalloc");
 }
 
 Engine::~Engine() {
     if (ctlHost) (void)hipHostFree(ctlHost);
-    for (auto& e : ev)
-        if (e) (void)hipEventDestroy(e);
+    for (auto& f : ev)
+        for (auto& e : f)
+            if (e) (void)hipEventDestroy(e);
     if (ownStream) (void)hipStreamDestroy(ownStream);
 }
 
@@ -141,23 +142,24 @@ void Engine::enqueueFrame(FrameState& f, const SceneDev& s, const CamParams& cp,
     f.cp = cp;
     f.opts = o;
     FrameCtl* ctl = f.ctl.p;
+    hipEvent_t* E = ev[evNext];
+    evNext = (evNext + 1) % kEventRing;
+    evPending = std::min(evPending + 1, kEventRing);
 
-    hipCheck(hipEventRecord(ev[0], stream), "hipEventRecord");
+    hipCheck(hipEventRecord(E[0], stream), "hipEventRecord");
     k_initFrame<<<(std::max<uint32_t>(tiles, 1) + 255) / 256, 256, 0, stream>>>(ctl, f.ranges.p, tiles);
     if (n > 0) {
         const uint32_t grid = (n + 255) / 256;
         if (s.raw)
             k_preprocess<false, true><<<grid, 256, 0, stream>>>(static_cast<int>(n), s.pos, s.scale, s.rot, s.opacity,
-                                                                s.sh, cp, f.rec0.p, f.rec1.p, f.rec2.p, f.rect.p,
-                                                                f.tilesTouched.p, depthKey.p);
+                                                                s.sh, cp, f.rec.p, f.tilesTouched.p, depthKey.p);
         else if (s.shPlanar)
             k_preprocess<true, false><<<grid, 256, 0, stream>>>(static_cast<int>(n), s.pos, s.scale, s.rot, s.opacity,
-                                                                s.sh, cp, f.rec0.p, f.rec1.p, f.rec2.p, f.rect.p,
-                                                                f.tilesTouched.p, depthKey.p);
+                                                                s.sh, cp, f.rec.p, f.tilesTouched.p, depthKey.p);
         else
             k_preprocess<false, false><<<grid, 256, 0, stream>>>(static_cast<int>(n), s.pos, s.scale, s.rot,
-                                                                 s.opacity, s.sh, cp, f.rec0.p, f.rec1.p, f.rec2.p,
-                                                                 f.rect.p, f.tilesTouched.p, depthKey.p);
+                                                                 s.opacity, s.sh, cp, f.rec.p, f.tilesTouched.p,
+                                                                 depthKey.p);
         // Depth sort: four stable 8-bit passes over all Gaussians (culled ones carry the largest
         // key); ties keep ascending Gaussian index.
         const uint32_t blocks = (n + kSortBlock * kDepthItems - 1) / (kSortBlock * kDepthItems);
@@ -176,7 +178,7 @@ void Engine::enqueueFrame(FrameState& f, const SceneDev& s, const CamParams& cp,
             vin = vout[pass];
         }
     }
-    hipCheck(hipEventRecord(ev[1], stream), "hipEventRecord");
+    hipCheck(hipEventRecord(E[1], stream), "hipEventRecord");
     // Pairs in depth order, then a stable sort by tile id.
     const int tileBits = std::max(1, bitsFor(tiles - 1));
     const int passes = (tileBits + 7) / 8;
@@ -186,8 +188,8 @@ void Engine::enqueueFrame(FrameState& f, const SceneDev& s, const CamParams& cp,
     if (n > 0) {
         const uint32_t blocksDup = (n + kDupTile - 1) / kDupTile;
         const uint32_t cap = static_cast<uint32_t>(f.pairCap);
-        k_pairTotals<<<blocksDup, kDupBlock, 0, stream>>>(f.order.p, f.tilesTouched.p, f.rect.p, n, ttByRank.p,
-                                                          rectByRank.p, blockPairs.p, blockVisible.p, 0, 0);
+        k_pairTotals<<<blocksDup, kDupBlock, 0, stream>>>(f.order.p, f.rec.p, n, ttByRank.p, rectByRank.p,
+                                                          blockPairs.p, blockVisible.p, 0, 0);
         k_scanPairs<<<1, kDupBlock, 0, stream>>>(blockPairs.p, blockVisible.p, blocksDup, cap, ctl, 0);
         k_expandPairs<<<blocksDup, kDupBlock, 0, stream>>>(f.order.p, ttByRank.p, rectByRank.p, n, cp.tilesX,
                                                            blockPairs.p, f.tileA.p, f.gaussA.p, ctl, 0, 0, 0,
@@ -213,10 +215,10 @@ void Engine::enqueueFrame(FrameState& f, const SceneDev& s, const CamParams& cp,
         k_tileRanges<<<static_cast<uint32_t>((f.pairCap + 1023) / 1024), 256, 0, stream>>>(f.sortedTile, ctl,
                                                                                             f.ranges.p, 0, 0);
     }
-    hipCheck(hipEventRecord(ev[2], stream), "hipEventRecord");
+    hipCheck(hipEventRecord(E[2], stream), "hipEventRecord");
     launchBlend(f, out8, outf);
     hipCheck(hipGetLastError(), "kernel launch");
-    hipCheck(hipEventRecord(ev[3], stream), "hipEventRecord");
+    hipCheck(hipEventRecord(E[3], stream), "hipEventRecord");
 }
 
 template <int TW, int TH, int BATCH>
@@ -229,13 +231,11 @@ void Engine::launchBlendB(FrameState& f, bool cull, uint8_t* out8, float* outf) 
         pix = f.pixState.p;
     }
     if (cull)
-        k_blend<TW, TH, BATCH, true><<<tiles, TW * TH, 0, stream>>>(f.ranges.p, f.sortedGauss, f.rec0.p, f.rec1.p,
-                                                                     f.rec2.p, cp.width, cp.height, cp.tilesX, out8,
-                                                                     outf, pix, f.ctl.p);
+        k_blend<TW, TH, BATCH, true><<<tiles, TW * TH, 0, stream>>>(f.ranges.p, f.sortedGauss, f.rec.p, cp.width,
+                                                                     cp.height, cp.tilesX, out8, outf, pix, f.ctl.p);
     else
-        k_blend<TW, TH, BATCH, false><<<tiles, TW * TH, 0, stream>>>(f.ranges.p, f.sortedGauss, f.rec0.p, f.rec1.p,
-                                                                      f.rec2.p, cp.width, cp.height, cp.tilesX, out8,
-                                                                      outf, pix, f.ctl.p);
+        k_blend<TW, TH, BATCH, false><<<tiles, TW * TH, 0, stream>>>(f.ranges.p, f.sortedGauss, f.rec.p, cp.width,
+                                                                      cp.height, cp.tilesX, out8, outf, pix, f.ctl.p);
 }
 
 template <int TW, int TH>
@@ -272,6 +272,7 @@ int Engine::renderSync(FrameState& f, const SceneDev& s, const CamParams& cp, co
                        float* outf) {
     int retries = 0;
     while (true) {
+        evPending = 0;  // stage times of the frame that completes
         enqueueFrame(f, s, cp, o, out8, outf);
         if (!readBack(f)) break;
         ++retries;
@@ -281,16 +282,30 @@ int Engine::renderSync(FrameState& f, const SceneDev& s, const CamParams& cp, co
 }
 
 void Engine::fillStats(const FrameState& f, ans_stats* st) {
-    if (!st) return;
-    float a = 0, b = 0, c = 0;
-    hipCheck(hipEventElapsedTime(&a, ev[0], ev[1]), "hipEventElapsedTime");
-    hipCheck(hipEventElapsedTime(&b, ev[1], ev[2]), "hipEventElapsedTime");
-    hipCheck(hipEventElapsedTime(&c, ev[2], ev[3]), "hipEventElapsedTime");
+    if (!st) {
+        evPending = 0;
+        return;
+    }
+    // Stage times averaged over the frames enqueued since the last readout (the newest
+    // kEventRing of them); every one of them has completed by now.
+    double a = 0, b = 0, c = 0;
+    const int frames = std::max(evPending, 1);
+    for (int k = 0; k < frames; ++k) {
+        hipEvent_t* E = ev[(evNext - 1 - k + kEventRing) % kEventRing];
+        float x = 0, y = 0, z = 0;
+        hipCheck(hipEventElapsedTime(&x, E[0], E[1]), "hipEventElapsedTime");
+        hipCheck(hipEventElapsedTime(&y, E[1], E[2]), "hipEventElapsedTime");
+        hipCheck(hipEventElapsedTime(&z, E[2], E[3]), "hipEventElapsedTime");
+        a += x;
+        b += y;
+        c += z;
+    }
+    evPending = 0;
     st->visible = f.visible;
     st->tile_pairs = f.pairs;
-    st->project_ms = a;
-    st->sort_ms = b;
-    st->blend_ms = c;
+    st->project_ms = a / frames;
+    st->sort_ms = b / frames;
+    st->blend_ms = c / frames;
     st->pair_capacity = static_cast<int64_t>(f.pairCap);
     st->retries = f.retries;
     st->reserved = 0;
@@ -301,6 +316,7 @@ void dumpFrame(hipStream_t stream, const FrameState& f, uint32_t* splat_bits, in
                uint32_t* values, uint32_t* ranges) {
     const size_t n = f.n, P = f.pairs, T = f.tiles;
     hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    std::vector<SplatRec> rec(n);
     std::vector<float4> r0(n), r1(n), r2(n);
     std::vector<uint2> rc(n);
     std::vector<uint32_t> tt(n);
@@ -308,12 +324,15 @@ void dumpFrame(hipStream_t stream, const FrameState& f, uint32_t* splat_bits, in
         if (bytes) hipCheck(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
     };
     down(tt.data(), f.tilesTouched.p, 4 * n);
-    if (splat_bits || keys) {
-        down(r0.data(), f.rec0.p, 16 * n);
-        down(r1.data(), f.rec1.p, 16 * n);
-        down(r2.data(), f.rec2.p, 16 * n);
+    if (splat_bits || keys || rect) {
+        down(rec.data(), f.rec.p, sizeof(SplatRec) * n);
+        for (size_t i = 0; i < n; ++i) {
+            r0[i] = rec[i].geo;
+            r1[i] = rec[i].geo2;
+            r2[i] = rec[i].col;
+            rc[i] = make_uint2(rec[i].meta.x, rec[i].meta.y);
+        }
     }
-    if (rect) down(rc.data(), f.rect.p, 8 * n);
     for (size_t i = 0; i < n; ++i) {
         const bool vis = tt[i] > 0;
         if (splat_bits) {

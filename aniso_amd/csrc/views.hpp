@@ -28,8 +28,7 @@ static __global__ __launch_bounds__(256) void k_initViews(FrameCtl* __restrict__
 static __global__ __launch_bounds__(256) void k_preprocessViews(
     int n, int views, const float* __restrict__ pos, const float* __restrict__ scale, const float4* __restrict__ rot,
     const float* __restrict__ opacity, const float* __restrict__ sh, int shCoeffs, const CamParams* __restrict__ cams,
-    float4* __restrict__ rec0, float4* __restrict__ rec1, float4* __restrict__ rec2, uint2* __restrict__ rect,
-    uint32_t* __restrict__ tilesTouched, uint32_t* __restrict__ depthKey) {
+    SplatRec* __restrict__ rec, uint32_t* __restrict__ tilesTouched, uint32_t* __restrict__ depthKey) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const float p[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
@@ -44,11 +43,10 @@ static __global__ __launch_bounds__(256) void k_preprocessViews(
         const size_t o = static_cast<size_t>(v) * n + i;
         GaussProj g;
         if (!projectGaussian<false, false>(0, p, sc, &q, &op, shr, 1, cams[v], g)) {
-            tilesTouched[o] = 0;
-            depthKey[o] = 0xffffffffu;
+            storeCulled(o, rec, tilesTouched, depthKey);
             continue;
         }
-        storeSplat(g, o, rec0, rec1, rec2, rect, tilesTouched, depthKey);
+        storeSplat(g, o, rec, tilesTouched, depthKey);
     }
 }
 
@@ -74,8 +72,7 @@ template <int TW, int TH, int BATCH, bool CULL>
 __global__ __launch_bounds__(TW* TH) void k_blendViews(const uint32_t* __restrict__ itemOrder, uint32_t maxTiles,
                                                        const uint2* __restrict__ ranges,
                                                        const uint32_t* __restrict__ sortedGauss, uint32_t pairStride,
-                                                       const float4* __restrict__ rec0, const float4* __restrict__ rec1,
-                                                       const float4* __restrict__ rec2, uint32_t gaussStride,
+                                                       const SplatRec* __restrict__ rec, uint32_t gaussStride,
                                                        const CamParams* __restrict__ cams, uint8_t* const* __restrict__ outs,
                                                        const FrameCtl* __restrict__ ctl) {
     const uint32_t item = itemOrder[blockIdx.x];
@@ -84,8 +81,8 @@ __global__ __launch_bounds__(TW* TH) void k_blendViews(const uint32_t* __restric
     const int tilesX = cams[v].tilesX;
     const size_t go = static_cast<size_t>(v) * gaussStride;
     blendTile<TW, TH, BATCH, CULL>(static_cast<int>(t) % tilesX, static_cast<int>(t) / tilesX, ranges[item],
-                                   sortedGauss + static_cast<size_t>(v) * pairStride, rec0 + go, rec1 + go, rec2 + go,
-                                   cams[v].width, cams[v].height, outs[v], nullptr, nullptr);
+                                   sortedGauss + static_cast<size_t>(v) * pairStride, rec + go, cams[v].width,
+                                   cams[v].height, outs[v], nullptr, nullptr);
 }
 
 }  // namespace ans

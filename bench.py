@@ -11,6 +11,7 @@ Prints one JSON line (rank 0). `--gpus N` with N > 1 exits: this build measures 
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import math
 import os
@@ -36,6 +37,8 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--skip-extra", action="store_true", help="only the headline forward workload")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run a plain 1 GiB device copy (the reference kernel for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -60,29 +63,31 @@ def main():
     c_cam = camera_c(cam)
     c_opt = opts.c()
 
-    # Warmup (sizes every buffer), then per-stage event times over a few synchronous frames.
+    if args.calibrate_copy:
+        copy_calibration()
+
+    # Warmup: sizes every buffer.
     st = RenderStats()
     for _ in range(max(1, args.warmup)):
         r.render(cam, st, opts)
-    stages = []
-    for _ in range(10):
-        r.render(cam, st, opts)
-        stages.append((st.project_ms, st.sort_ms, st.blend_ms))
-    proj, sort_, blend = (statistics.median(x[i] for x in stages) for i in range(3))
-    P, V = st.tile_pairs, st.visible
-    T = ((W + 15) // 16) * ((H + 15) // 16)
 
-    # Timed region: K frames enqueued back to back, device-resident input and output.
+    # Timed region: K frames enqueued back to back, device-resident input and output. The
+    # library's per-frame HIP events (recorded on its stream around each stage) give the stage
+    # times averaged over the timed frames (the newest 64), read by the wait.
     torch.cuda.synchronize()
     _lib.check(L.ans_render_wait(r._h, None))
     t0 = time.perf_counter()
     for _ in range(args.steps):
         _lib.check(L.ans_render_async(r._h, c_cam, c_opt, dev_img.data_ptr(), None))
-    rc = L.ans_render_wait(r._h, None)
+    cst = _lib.Stats_()
+    rc = L.ans_render_wait(r._h, ctypes.byref(cst))
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     _lib.check(rc)
     ms = (t1 - t0) * 1e3 / args.steps
+    proj, sort_, blend = cst.project_ms, cst.sort_ms, cst.blend_ms
+    P, V = cst.tile_pairs, cst.visible
+    T = ((W + 15) // 16) * ((H + 15) // 16)
 
     # Algorithmic HBM bytes per frame (compulsory unique traffic).
     n = raw.n
@@ -97,9 +102,11 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(b_blend / (blend * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": measured_traffic("k_blend", args.workload),
         "algorithmic_bytes": b_blend,
         "kernel_ms": round(blend, 4),
+        "timing": f"HIP events around the blend on the library's stream, mean of the last "
+                  f"{min(args.steps, 64)} timed frames",
     }
 
     extra = {}
@@ -139,6 +146,35 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+COPY_BYTES = 1 << 30
+
+
+def copy_calibration():
+    """Three launches of a plain elementwise copy (torch.mul by 1.0) reading COPY_BYTES and writing
+    COPY_BYTES: the known-size kernel whose FETCH_SIZE / WRITE_SIZE counters scale the counters
+    of the rasterizer's kernels to bytes."""
+    import torch
+
+    src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    del src, dst
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed counter summary
+    profiles/<kernel>_<workload>.md (its `traffic_bytes_per_launch:` line), or None."""
+    import re
+
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"^traffic_bytes_per_launch:\s*(\d+)", open(path).read(), re.M)
+    return int(m.group(1)) if m else None
+
+
 def bench_forward(workload, steps, warmup):
     """Forward ms/frame of another synthetic workload, same protocol as the headline line."""
     import torch
@@ -160,12 +196,13 @@ def bench_forward(workload, steps, warmup):
         t0 = time.perf_counter()
         for _ in range(steps):
             _lib.check(L.ans_render_async(r._h, c_cam, c_opt, dev.data_ptr(), None))
-        _lib.check(L.ans_render_wait(r._h, None))
+        cst = _lib.Stats_()
+        _lib.check(L.ans_render_wait(r._h, ctypes.byref(cst)))
         ms = (time.perf_counter() - t0) * 1e3 / steps
-    return {"ms_per_frame": round(ms, 4), "gaussians": raw.n, "visible": st.visible, "pairs": st.tile_pairs,
+    return {"ms_per_frame": round(ms, 4), "gaussians": raw.n, "visible": cst.visible, "pairs": cst.tile_pairs,
             "width": cam.width, "height": cam.height, "steps": steps,
-            "stages_ms": {"project": round(st.project_ms, 4), "sort": round(st.sort_ms, 4),
-                          "blend": round(st.blend_ms, 4)}}
+            "stages_ms": {"project": round(cst.project_ms, 4), "sort": round(cst.sort_ms, 4),
+                          "blend": round(cst.blend_ms, 4)}}
 
 
 def bench_views(workload, steps, warmup):

@@ -106,9 +106,10 @@ int ans_render_float(ans_renderer* r, const ans_camera* cam, const ans_options* 
 int ans_render_async(ans_renderer* r, const ans_camera* cam, const ans_options* options, uint8_t* dev_rgb,
                      float* dev_rgbf);
 
-/* Waits for every frame enqueued on the handle and fills stats for the last one. Returns
- * ANS_ERR_CAPACITY if any of them had more pairs than the pair buffers held; the buffers have
- * then been grown and the caller renders those frames again. */
+/* Waits for every frame enqueued on the handle and fills stats: counts of the last frame, stage
+ * times averaged over the frames enqueued since the previous wait (the newest 64 of them).
+ * Returns ANS_ERR_CAPACITY if any of them had more pairs than the pair buffers held; the buffers
+ * have then been grown and the caller renders those frames again. */
 int ans_render_wait(ans_renderer* r, ans_stats* stats);
 
 /* Intermediates of the last frame rendered on this handle, copied to host buffers (any may be
