@@ -109,7 +109,18 @@ def main():
                   f"{min(args.steps, 64)} timed frames",
     }
 
-    extra = {}
+    # The same frame through the host-output entry (ans_render: synchronous, image copied to host
+    # memory over PCIe), as the reference's GpuRenderer::render returns it.
+    host_img = np.empty((H, W, 3), np.uint8)
+    host_steps = max(5, args.steps // 5)
+    r.render(cam, None, opts)
+    t0 = time.perf_counter()
+    for _ in range(host_steps):
+        _lib.check(L.ans_render(r._h, c_cam, c_opt, host_img.ctypes.data_as(ctypes.c_void_p), host_img.size, None))
+    host_ms = (time.perf_counter() - t0) * 1e3 / host_steps
+
+    extra = {"host_output": {"ms_per_frame": round(host_ms, 4), "steps": host_steps,
+                             "includes": "synchronous ans_render: frame, pair-count readback, RGB8 copy to host"}}
     if not args.skip_extra:
         extra["fwd_bwd"] = bench_fwd_bwd(raw, cam, max(5, args.steps // 4), args.warmup)
         extra["fwd_clustered_2.2M"] = bench_forward("clustered-2.2M-1280x720", max(10, args.steps // 2), args.warmup)
