@@ -596,32 +596,54 @@ __device__ inline void blendTile(int tileX, int tileY, uint2 range, const uint32
             }
             waveSync();
         }
-        for (int i = 0; i < count; ++i) {
-            const int j = CULL ? sList[wave][i] : i;
-            const float4 a = s0[j];
-            const float4 b = s1[j];
-            if (!done) {
-                const float dx = px - a.x, dy = py - a.y;
-                const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
-                if (!(power > 0.0f)) {
-                    const float alpha = fminf(0.99f, b.y * __expf(power));
-                    if (alpha >= 1.0f / 255.0f) {
-                        const float nextT = T * (1.0f - alpha);
-                        if (nextT < 1e-4f) {
-                            done = true;
-                        } else {
-                            const float4 c = s2[j];
-                            const float w = alpha * T;
-                            cr = __builtin_fmaf(c.x, w, cr);
-                            cg = __builtin_fmaf(c.y, w, cg);
-                            cb = __builtin_fmaf(c.z, w, cb);
-                            T = nextT;
-                            last = start - range.x + static_cast<uint32_t>(j) + 1;
-                        }
-                    }
+        // Software-pipelined walk over the list, two splats per step with two register sets:
+        // while one splat is composited, the other set receives the next splat's records and the
+        // list index after it is read. The per-pixel rule is applied with selects instead of
+        // divergent branches; a skipped or stopping splat changes nothing, so every value equals
+        // the branching form's.
+        auto composite = [&](const float4& a, const float4& b, const float4& c, int j) {
+            const float dx = px - a.x, dy = py - a.y;
+            const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
+            const float alpha = fminf(0.99f, b.y * __expf(power));
+            const float nextT = T * (1.0f - alpha);
+            const bool hit = !done && !(power > 0.0f) && alpha >= 1.0f / 255.0f;
+            const bool stop = hit && nextT < 1e-4f;
+            const bool use = hit && !stop;
+            done = done || stop;
+            const float w = alpha * T;
+            cr = use ? __builtin_fmaf(c.x, w, cr) : cr;
+            cg = use ? __builtin_fmaf(c.y, w, cg) : cg;
+            cb = use ? __builtin_fmaf(c.z, w, cb) : cb;
+            T = use ? nextT : T;
+            last = use ? start - range.x + static_cast<uint32_t>(j) + 1 : last;
+        };
+        auto listAt = [&](int i) { return CULL ? static_cast<int>(sList[wave][i]) : i; };
+        if (count > 0) {
+            int jA = listAt(0);
+            float4 aA = s0[jA], bA = s1[jA], cA = s2[jA];
+            int jB = count > 1 ? listAt(1) : 0;
+            float4 aB, bB, cB;
+            for (int i = 0; i < count; i += 2) {
+                const bool more1 = i + 1 < count;
+                if (more1) {
+                    aB = s0[jB];
+                    bB = s1[jB];
+                    cB = s2[jB];
                 }
+                const int jNextA = i + 2 < count ? listAt(i + 2) : 0;
+                composite(aA, bA, cA, jA);
+                if (!more1) break;
+                jA = jNextA;
+                if (i + 2 < count) {
+                    aA = s0[jA];
+                    bA = s1[jA];
+                    cA = s2[jA];
+                }
+                const int jNextB = i + 3 < count ? listAt(i + 3) : 0;
+                composite(aB, bB, cB, jB);
+                jB = jNextB;
+                if ((i & 7) == 6 && __ballot(!done) == 0) break;
             }
-            if ((i & 7) == 7 && __ballot(!done) == 0) break;
         }
     }
     if (inside) {
