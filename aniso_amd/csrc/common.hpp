@@ -20,6 +20,7 @@ struct CamParams {
     float limX, limY;  // 1.3 * tan(half field of view), the Jacobian clamp
     int width, height;
     int tileW, tileH, tilesX, tilesY;
+    int tileShiftX, tileShiftY;  // log2 of tileW, tileH (powers of two)
     int shDegree;  // degree evaluated (option clamped to the scene)
     int shCoeffs;  // coefficients stored per channel in the scene
     float splatScale;

@@ -121,29 +121,26 @@ struct GaussProj {
     int shCount;
 };
 
-// Projects Gaussian i (the maths of src/render.cpp:39-104 with the FP32 camera transform of
-// src/gpu_render.cu:94-96): world -> camera; near cull; Sigma = M M^T, M = R(q) diag(s); EWA 2D
+// Projection of one Gaussian (the maths of src/render.cpp:39-104 with the FP32 camera transform
+// of src/gpu_render.cu:94-96), in three parts below: world -> camera; near cull; Sigma = M M^T, M = R(q) diag(s); EWA 2D
 // covariance with the clamped Jacobian and the 0.3 low-pass; conic; radius = ceil(3 sqrt(lambda));
 // centre in pixels; screen cull; tile rect; SH colour. RAW: the parameters are the optimizer's
 // (scale = exp(log-scale), opacity = sigmoid(logit), rotation = q / |q|, src/ply_loader.cpp:124-130),
 // activated here in FP32. Deliberate deviations shared with the CPU oracle: the near test is
 // !(z >= 0.2) so a NaN depth is culled, and a non-finite centre or a radius of 2^24 pixels or more
-// is culled. Returns false if culled.
-// shBase points at this Gaussian's coefficient 0 (red); coefficient * 3 + channel j sits at
-// shBase[j * shStride] (PLANAR: shStride = n; otherwise 1 and shBase 16-byte aligned).
-template <bool PLANAR, bool RAW>
-__host__ __device__ inline bool projectGaussian(int i, const float* __restrict__ pos, const float* __restrict__ scale,
-                                                const float4* __restrict__ rot, const float* __restrict__ opacity,
-                                                const float* __restrict__ shBase, size_t shStride, const CamParams& cam,
-                                                GaussProj& g) {
-    const F3 m = reinterpret_cast<const F3*>(pos)[i];
+// is culled.
+
+// World -> camera (FP32, left to right as gpu_render.cu:94-96) and the near cull; false if culled.
+__host__ __device__ inline bool cameraTransform(const F3 m, const CamParams& cam, GaussProj& g) {
     g.px = cam.R[0] * m.x + cam.R[1] * m.y + cam.R[2] * m.z + cam.t[0];
     g.py = cam.R[3] * m.x + cam.R[4] * m.y + cam.R[5] * m.z + cam.t[1];
     g.pz = cam.R[6] * m.x + cam.R[7] * m.y + cam.R[8] * m.z + cam.t[2];
-    const float px = g.px, py = g.py, pz = g.pz;
-    if (!(pz >= 0.2f)) return false;
+    return g.pz >= 0.2f;
+}
 
-    const float4 q4 = rot[i];  // w, x, y, z
+// The view-independent part: rotation, scale (times splatScale), M = R(q) diag(s), Sigma = M M^T.
+template <bool RAW>
+__host__ __device__ inline void prepareGaussian(const float4 q4, const F3 sc, float splatScale, GaussProj& g) {
     if constexpr (RAW) {
         g.qlen = sqrtf(q4.x * q4.x + q4.y * q4.y + q4.z * q4.z + q4.w * q4.w);
         if (g.qlen > 0.0f) {
@@ -159,11 +156,10 @@ __host__ __device__ inline bool projectGaussian(int i, const float* __restrict__
     g.Rq[0][0] = 1 - 2 * (y * y + z * z); g.Rq[0][1] = 2 * (x * y - w * z);     g.Rq[0][2] = 2 * (x * z + w * y);
     g.Rq[1][0] = 2 * (x * y + w * z);     g.Rq[1][1] = 1 - 2 * (x * x + z * z); g.Rq[1][2] = 2 * (y * z - w * x);
     g.Rq[2][0] = 2 * (x * z - w * y);     g.Rq[2][1] = 2 * (y * z + w * x);     g.Rq[2][2] = 1 - 2 * (x * x + y * y);
-    const F3 sc = reinterpret_cast<const F3*>(scale)[i];
     if constexpr (RAW) {
-        g.s[0] = expf(sc.x) * cam.splatScale; g.s[1] = expf(sc.y) * cam.splatScale; g.s[2] = expf(sc.z) * cam.splatScale;
+        g.s[0] = expf(sc.x) * splatScale; g.s[1] = expf(sc.y) * splatScale; g.s[2] = expf(sc.z) * splatScale;
     } else {
-        g.s[0] = sc.x * cam.splatScale; g.s[1] = sc.y * cam.splatScale; g.s[2] = sc.z * cam.splatScale;
+        g.s[0] = sc.x * splatScale; g.s[1] = sc.y * splatScale; g.s[2] = sc.z * splatScale;
     }
 #pragma unroll
     for (int r = 0; r < 3; ++r)
@@ -173,7 +169,33 @@ __host__ __device__ inline bool projectGaussian(int i, const float* __restrict__
     for (int r = 0; r < 3; ++r)
 #pragma unroll
         for (int c = 0; c < 3; ++c) g.Sigma[r][c] = g.M[r][0] * g.M[c][0] + g.M[r][1] * g.M[c][1] + g.M[r][2] * g.M[c][2];
+}
 
+template <bool PLANAR, bool RAW>
+__host__ __device__ inline bool finishProjection(const F3 m, float opacityIn, const float* __restrict__ shBase,
+                                                 size_t shStride, const CamParams& cam, GaussProj& g);
+
+// Projects Gaussian i: cameraTransform, prepareGaussian, finishProjection. shBase points at this
+// Gaussian's coefficient 0 (red); coefficient * 3 + channel j sits at shBase[j * shStride]
+// (PLANAR: shStride = n; otherwise 1 and shBase 16-byte aligned).
+template <bool PLANAR, bool RAW>
+__host__ __device__ inline bool projectGaussian(int i, const float* __restrict__ pos, const float* __restrict__ scale,
+                                                const float4* __restrict__ rot, const float* __restrict__ opacity,
+                                                const float* __restrict__ shBase, size_t shStride, const CamParams& cam,
+                                                GaussProj& g) {
+    const F3 m = reinterpret_cast<const F3*>(pos)[i];
+    if (!cameraTransform(m, cam, g)) return false;
+    prepareGaussian<RAW>(rot[i], reinterpret_cast<const F3*>(scale)[i], cam.splatScale, g);
+    return finishProjection<PLANAR, RAW>(m, opacity[i], shBase, shStride, cam, g);
+}
+
+// The view-dependent rest: EWA 2D covariance with the clamped Jacobian and the 0.3 low-pass,
+// conic, radius, centre, screen cull, tile rect, SH colour, opacity (sigmoid of opacityIn when
+// RAW). Needs cameraTransform and prepareGaussian done on g.
+template <bool PLANAR, bool RAW>
+__host__ __device__ inline bool finishProjection(const F3 m, float opacityIn, const float* __restrict__ shBase,
+                                                 size_t shStride, const CamParams& cam, GaussProj& g) {
+    const float px = g.px, py = g.py, pz = g.pz;
     const float rx = px / pz, ry = py / pz;
     g.clampX = rx < -cam.limX || cam.limX < rx;
     g.clampY = ry < -cam.limY || cam.limY < ry;
@@ -206,10 +228,14 @@ __host__ __device__ inline bool projectGaussian(int i, const float* __restrict__
     if (!__builtin_isfinite(g.u) || !__builtin_isfinite(g.v)) return false;
     if (g.u + radius < 0 || g.v + radius < 0 || g.u - radius >= cam.width || g.v - radius >= cam.height) return false;
 
-    g.x0 = max(0, static_cast<int>(g.u - radius) / cam.tileW);
-    g.y0 = max(0, static_cast<int>(g.v - radius) / cam.tileH);
-    g.x1 = min(cam.tilesX - 1, static_cast<int>(g.u + radius) / cam.tileW);
-    g.y1 = min(cam.tilesY - 1, static_cast<int>(g.v + radius) / cam.tileH);
+    // The reference divides the truncated bounds by the tile size (truncation toward zero); the
+    // tile sizes are powers of two, so an arithmetic shift gives the same tile for every bound
+    // that is not negative, and a negative lower bound clamps to 0 either way. The upper bounds
+    // are not negative (u + radius >= 0 after the screen cull).
+    g.x0 = max(0, static_cast<int>(g.u - radius) >> cam.tileShiftX);
+    g.y0 = max(0, static_cast<int>(g.v - radius) >> cam.tileShiftY);
+    g.x1 = min(cam.tilesX - 1, static_cast<int>(g.u + radius) >> cam.tileShiftX);
+    g.y1 = min(cam.tilesY - 1, static_cast<int>(g.v + radius) >> cam.tileShiftY);
 
     g.dir[0] = m.x - cam.eye[0];
     g.dir[1] = m.y - cam.eye[1];
@@ -224,8 +250,8 @@ __host__ __device__ inline bool projectGaussian(int i, const float* __restrict__
     g.rgb.x = (0.0f < g.rgbRaw.x) ? g.rgbRaw.x : 0.0f;
     g.rgb.y = (0.0f < g.rgbRaw.y) ? g.rgbRaw.y : 0.0f;
     g.rgb.z = (0.0f < g.rgbRaw.z) ? g.rgbRaw.z : 0.0f;
-    if constexpr (RAW) g.opacity = 1.0f / (1.0f + expf(-opacity[i]));
-    else g.opacity = opacity[i];
+    if constexpr (RAW) g.opacity = 1.0f / (1.0f + expf(-opacityIn));
+    else g.opacity = opacityIn;
     g.A = g.c / g.det;
     g.B = -g.b / g.det;
     g.C = g.a / g.det;

@@ -43,6 +43,8 @@ CamParams makeCamParams(const ans_camera& c, const ans_options& o, int sceneShDe
     p.height = c.height;
     p.tileW = o.tile_w;
     p.tileH = o.tile_h;
+    p.tileShiftX = bitsFor(static_cast<uint32_t>(o.tile_w)) - 1;
+    p.tileShiftY = bitsFor(static_cast<uint32_t>(o.tile_h)) - 1;
     p.tilesX = (c.width + o.tile_w - 1) / o.tile_w;
     p.tilesY = (c.height + o.tile_h - 1) / o.tile_h;
     p.shDegree = std::max(0, std::min(o.sh_degree, sceneShDegree));

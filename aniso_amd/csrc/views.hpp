@@ -22,27 +22,27 @@ static __global__ __launch_bounds__(256) void k_initViews(FrameCtl* __restrict__
 }
 
 // Preprocess of every view, one lane per Gaussian: its parameters and SH (planar on the device,
-// [coefficient * 3 + channel][n]) are loaded once into registers and projected into each view in
-// turn with projectGaussian, the arithmetic of the single-view k_preprocess. Outputs for view v
-// at v * n + i.
+// [coefficient * 3 + channel][n]) are loaded once into registers, its view-independent covariance
+// is formed once (prepareGaussian), and it is projected into each view in turn (cameraTransform,
+// finishProjection): the arithmetic of the single-view k_preprocess. Outputs for view v at
+// v * n + i.
 static __global__ __launch_bounds__(256) void k_preprocessViews(
     int n, int views, const float* __restrict__ pos, const float* __restrict__ scale, const float4* __restrict__ rot,
     const float* __restrict__ opacity, const float* __restrict__ sh, int shCoeffs, const CamParams* __restrict__ cams,
     SplatRec* __restrict__ rec, uint32_t* __restrict__ tilesTouched, uint32_t* __restrict__ depthKey) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    const float p[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
-    const float sc[3] = {scale[3 * i], scale[3 * i + 1], scale[3 * i + 2]};
-    const float4 q = rot[i];
+    const F3 m = reinterpret_cast<const F3*>(pos)[i];
     const float op = opacity[i];
     const int K3 = shCoeffs * 3;
     alignas(16) float shr[48];
 #pragma unroll
     for (int j = 0; j < 48; ++j) shr[j] = j < K3 ? sh[static_cast<size_t>(j) * n + i] : 0.0f;
+    GaussProj g;
+    prepareGaussian<false>(rot[i], reinterpret_cast<const F3*>(scale)[i], cams[0].splatScale, g);
     for (int v = 0; v < views; ++v) {
         const size_t o = static_cast<size_t>(v) * n + i;
-        GaussProj g;
-        if (!projectGaussian<false, false>(0, p, sc, &q, &op, shr, 1, cams[v], g)) {
+        if (!cameraTransform(m, cams[v], g) || !finishProjection<false, false>(m, op, shr, 1, cams[v], g)) {
             storeCulled(o, rec, tilesTouched, depthKey);
             continue;
         }
