@@ -554,9 +554,14 @@ __device__ inline float quadMinOverBox(float A, float B, float C, float rB_C, fl
 // wavefront compacts the indices it must visit with ballots, walks them in order, and stops as
 // soon as all of its pixels are done; the workgroup stops when all wavefronts are.
 //
-// pixState (optional, for the backward pass): per pixel, the final transmittance's bits and the
-// position + 1 in the tile's list of the last splat blended into it (0: none).
-template <int TW, int TH, int BATCH, bool CULL>
+// SAVE (frames the backward pass will read): pixState receives, per pixel, the final
+// transmittance's bits and the position + 1 in the tile's list of the last splat blended into it
+// (0: none).
+//
+// The staged conic is (-A/2, B, -C/2): power = (-A/2 dx) dx + (-C/2 dy) dy - (B dx) dy equals the
+// reference's -0.5 (A dx dx + C dy dy) - B dx dy bit for bit, since scaling by a power of two
+// commutes with rounding.
+template <int TW, int TH, int BATCH, bool CULL, bool SAVE>
 __device__ inline void blendTile(int tileX, int tileY, uint2 range, const uint32_t* __restrict__ sortedGauss,
                                  const SplatRec* __restrict__ rec, int width, int height, uint8_t* __restrict__ out8,
                                  float* __restrict__ outf, uint2* __restrict__ pixState) {
@@ -587,8 +592,8 @@ __device__ inline void blendTile(int tileX, int tileY, uint2 range, const uint32
             const uint32_t g = sortedGauss[start + k];
             const float4 a = rec[g].geo;
             const float4 b = rec[g].geo2;
-            s0[k] = a;
-            s1[k] = b;
+            s0[k] = make_float4(a.x, a.y, -0.5f * a.z, a.w);
+            s1[k] = make_float4(-0.5f * b.x, b.y, b.z, b.w);
             s2[k] = rec[g].col;
             if constexpr (CULL) {
                 uint32_t m = 0;
@@ -629,19 +634,20 @@ __device__ inline void blendTile(int tileX, int tileY, uint2 range, const uint32
         // the branching form's.
         auto composite = [&](const float4& a, const float4& b, const float4& c, int j) {
             const float dx = px - a.x, dy = py - a.y;
-            const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
+            const float power = a.z * dx * dx + b.x * dy * dy - a.w * dx * dy;
             const float alpha = fminf(0.99f, b.y * __expf(power));
             const float nextT = T * (1.0f - alpha);
             const bool hit = !done && !(power > 0.0f) && alpha >= 1.0f / 255.0f;
             const bool stop = hit && nextT < 1e-4f;
             const bool use = hit && !stop;
             done = done || stop;
-            const float w = alpha * T;
-            cr = use ? __builtin_fmaf(c.x, w, cr) : cr;
-            cg = use ? __builtin_fmaf(c.y, w, cg) : cg;
-            cb = use ? __builtin_fmaf(c.z, w, cb) : cb;
+            // A weight of +0 leaves the (non-negative, finite) colour sums unchanged exactly.
+            const float w = use ? alpha * T : 0.0f;
+            cr = __builtin_fmaf(c.x, w, cr);
+            cg = __builtin_fmaf(c.y, w, cg);
+            cb = __builtin_fmaf(c.z, w, cb);
             T = use ? nextT : T;
-            last = use ? start - range.x + static_cast<uint32_t>(j) + 1 : last;
+            if constexpr (SAVE) last = use ? start - range.x + static_cast<uint32_t>(j) + 1 : last;
         };
         auto listAt = [&](int i) { return CULL ? static_cast<int>(sList[wave][i]) : i; };
         if (count > 0) {
@@ -684,11 +690,11 @@ __device__ inline void blendTile(int tileX, int tileY, uint2 range, const uint32
             outf[3 * p + 1] = cg;
             outf[3 * p + 2] = cb;
         }
-        if (pixState) pixState[p] = make_uint2(__float_as_uint(T), last);
+        if constexpr (SAVE) pixState[p] = make_uint2(__float_as_uint(T), last);
     }
 }
 
-template <int TW, int TH, int BATCH, bool CULL>
+template <int TW, int TH, int BATCH, bool CULL, bool SAVE>
 __global__ __launch_bounds__(TW* TH) void k_blend(const uint2* __restrict__ ranges,
                                                   const uint32_t* __restrict__ sortedGauss,
                                                   const SplatRec* __restrict__ rec, int width, int height, int tilesX,
@@ -696,8 +702,8 @@ __global__ __launch_bounds__(TW* TH) void k_blend(const uint2* __restrict__ rang
                                                   uint2* __restrict__ pixState, const FrameCtl* __restrict__ ctl) {
     if (ctl->overflow) return;
     const int tile = blockIdx.x;
-    blendTile<TW, TH, BATCH, CULL>(tile % tilesX, tile / tilesX, ranges[tile], sortedGauss, rec, width, height, out8,
-                                   outf, pixState);
+    blendTile<TW, TH, BATCH, CULL, SAVE>(tile % tilesX, tile / tilesX, ranges[tile], sortedGauss, rec, width, height,
+                                         out8, outf, pixState);
 }
 
 }  // namespace ans

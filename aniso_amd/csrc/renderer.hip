@@ -227,17 +227,18 @@ template <int TW, int TH, int BATCH>
 void Engine::launchBlendB(FrameState& f, bool cull, uint8_t* out8, float* outf) {
     const CamParams& cp = f.cp;
     const uint32_t tiles = static_cast<uint32_t>(cp.tilesX) * cp.tilesY;
-    uint2* pix = nullptr;
     if (f.savePixels) {
+        // Frames for the backward pass always cull (culling changes no pixel).
         f.pixState.reserve(static_cast<size_t>(cp.width) * cp.height);
-        pix = f.pixState.p;
+        k_blend<TW, TH, BATCH, true, true><<<tiles, TW * TH, 0, stream>>>(
+            f.ranges.p, f.sortedGauss, f.rec.p, cp.width, cp.height, cp.tilesX, out8, outf, f.pixState.p, f.ctl.p);
+    } else if (cull) {
+        k_blend<TW, TH, BATCH, true, false><<<tiles, TW * TH, 0, stream>>>(
+            f.ranges.p, f.sortedGauss, f.rec.p, cp.width, cp.height, cp.tilesX, out8, outf, nullptr, f.ctl.p);
+    } else {
+        k_blend<TW, TH, BATCH, false, false><<<tiles, TW * TH, 0, stream>>>(
+            f.ranges.p, f.sortedGauss, f.rec.p, cp.width, cp.height, cp.tilesX, out8, outf, nullptr, f.ctl.p);
     }
-    if (cull)
-        k_blend<TW, TH, BATCH, true><<<tiles, TW * TH, 0, stream>>>(f.ranges.p, f.sortedGauss, f.rec.p, cp.width,
-                                                                     cp.height, cp.tilesX, out8, outf, pix, f.ctl.p);
-    else
-        k_blend<TW, TH, BATCH, false><<<tiles, TW * TH, 0, stream>>>(f.ranges.p, f.sortedGauss, f.rec.p, cp.width,
-                                                                      cp.height, cp.tilesX, out8, outf, pix, f.ctl.p);
 }
 
 template <int TW, int TH>

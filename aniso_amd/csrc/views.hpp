@@ -80,7 +80,7 @@ __global__ __launch_bounds__(TW* TH) void k_blendViews(const uint32_t* __restric
     if (ctl[v].overflow) return;
     const int tilesX = cams[v].tilesX;
     const size_t go = static_cast<size_t>(v) * gaussStride;
-    blendTile<TW, TH, BATCH, CULL>(static_cast<int>(t) % tilesX, static_cast<int>(t) / tilesX, ranges[item],
+    blendTile<TW, TH, BATCH, CULL, false>(static_cast<int>(t) % tilesX, static_cast<int>(t) / tilesX, ranges[item],
                                    sortedGauss + static_cast<size_t>(v) * pairStride, rec + go, cams[v].width,
                                    cams[v].height, outs[v], nullptr, nullptr);
 }
