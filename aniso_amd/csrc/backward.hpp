@@ -130,8 +130,8 @@ __global__ __launch_bounds__(TW* TH) void k_blendBackward(
             const uint4 meta = rec[g].meta;
             const uint2 rp = make_uint2(meta.x, meta.y);
             const uint32_t first = slotStart[g];
-            s0[k] = a;
-            s1[k] = bb;
+            s0[k] = make_float4(a.x, a.y, -0.5f * a.z, a.w);  // the blend's staged (-A/2, B, -C/2)
+            s1[k] = make_float4(-0.5f * bb.x, bb.y, bb.z, bb.w);
             s2[k] = rec[g].col;
             const uint32_t x0 = rp.x & 0xffffu, y0 = rp.x >> 16, w = (rp.y & 0xffffu) - x0 + 1;
             sSlot[k] = first + (static_cast<uint32_t>(tileY) - y0) * w + (static_cast<uint32_t>(tileX) - x0);
@@ -190,43 +190,39 @@ __global__ __launch_bounds__(TW* TH) void k_blendBackward(
     }
     float ar = 0.0f, ag = 0.0f, ab = 0.0f;  // accum
     float Tcur = T;
-    // This pixel's gradient contributions from splat j of batch b (zero if it does not blend
-    // there); steps T and accum back past it. True if it contributed.
-    auto visit = [&](int b, int j, float (&d)[9]) -> bool {
-#pragma unroll
-        for (int c = 0; c < 9; ++c) d[c] = 0.0f;
-        if (static_cast<uint32_t>(b) * BATCH + j >= lastContrib) return false;
-        const float4 a = s0[j];
-        const float4 bb = s1[j];
+    // This pixel's gradient contributions from the splat with staged records (a, bb, c) at list
+    // position pos of the tile (zero if it does not blend there); steps T and accum back past it.
+    // True if it contributed. power and alpha are the blend's, bit for bit (same staged conic, same
+    // operations), so every blend/skip decision matches the forward's; selects replace branches.
+    auto visit = [&](const float4& a, const float4& bb, const float4& c, uint32_t pos, float (&d)[9]) -> bool {
         const float dx = px - a.x, dy = py - a.y;
-        const float power = -0.5f * (a.z * dx * dx + bb.x * dy * dy) - a.w * dx * dy;
-        if (power > 0.0f) return false;
+        const float power = a.z * dx * dx + bb.x * dy * dy - a.w * dx * dy;
         const float G = __expf(power);
         const float raw = bb.y * G;
         const float alpha = fminf(0.99f, raw);
-        if (!(alpha >= 1.0f / 255.0f)) return false;
-        const float4 c = s2[j];
+        const bool use = pos < lastContrib && !(power > 0.0f) && alpha >= 1.0f / 255.0f;
         const float oneMinus = 1.0f - alpha;
         const float Ti = Tcur * __builtin_amdgcn_rcpf(oneMinus);
-        const float w = alpha * Ti;
+        const float w = use ? alpha * Ti : 0.0f;
         d[6] = gr * w;
         d[7] = gg * w;
         d[8] = gb * w;
         const float gAlpha = Ti * (gr * (c.x - ar) + gg * (c.y - ag) + gb * (c.z - ab));
-        ar = alpha * c.x + oneMinus * ar;
-        ag = alpha * c.y + oneMinus * ag;
-        ab = alpha * c.z + oneMinus * ab;
-        if (raw <= 0.99f) {
-            d[5] = gAlpha * G;
-            const float gPow = gAlpha * raw;
-            d[0] = gPow * (a.z * dx + a.w * dy);
-            d[1] = gPow * (bb.x * dy + a.w * dx);
-            d[2] = gPow * (-0.5f * dx * dx);
-            d[3] = gPow * (-dx * dy);
-            d[4] = gPow * (-0.5f * dy * dy);
-        }
-        Tcur = Ti;
-        return true;
+        // No gradient where the 0.99 clamp is active; gPow = 0 zeroes the conic and centre terms.
+        const bool open = use && raw <= 0.99f;
+        const float gPow = open ? gAlpha * raw : 0.0f;
+        d[5] = open ? gAlpha * G : 0.0f;
+        // Staged a.z = -A/2, bb.x = -C/2, a.w = B.
+        d[0] = gPow * (-2.0f * a.z * dx + a.w * dy);
+        d[1] = gPow * (-2.0f * bb.x * dy + a.w * dx);
+        d[2] = gPow * (-0.5f * dx * dx);
+        d[3] = gPow * (-dx * dy);
+        d[4] = gPow * (-0.5f * dy * dy);
+        ar = use ? alpha * c.x + oneMinus * ar : ar;
+        ag = use ? alpha * c.y + oneMinus * ag : ag;
+        ab = use ? alpha * c.z + oneMinus * ab : ab;
+        Tcur = use ? Ti : Tcur;
+        return use;
     };
     const int rowHalf = (lane >> 4) & 1;  // rows 1 and 3 write components 5..9
     const bool secondSplat = lane >= 32;  // rows 2 and 3 write the second splat
@@ -235,20 +231,44 @@ __global__ __launch_bounds__(TW* TH) void k_blendBackward(
         for (int k = tid; k < kWaves * (BATCH + 1) * 10; k += kThreads) (&sGrad[0][0][0])[k] = 0.0f;
         int count;
         const int cnt = stage(b, count);
-        if (__ballot(lastContrib > static_cast<uint32_t>(b) * BATCH) != 0) {
-            for (int i = count - 1; i >= 0; i -= 2) {
-                const int j1 = sList[wave][i];
-                const int j2 = i >= 1 ? static_cast<int>(sList[wave][i - 1]) : BATCH;
+        if (__ballot(lastContrib > static_cast<uint32_t>(b) * BATCH) != 0 && count > 0) {
+            // Pairs (i, i - 1) back to front; the next pair's records are read from LDS while
+            // this pair's contributions are reduced.
+            const uint32_t base = static_cast<uint32_t>(b) * BATCH;
+            int i = count - 1;
+            int j1 = sList[wave][i];
+            int j2 = i >= 1 ? static_cast<int>(sList[wave][i - 1]) : BATCH;
+            float4 a1 = s0[j1], b1 = s1[j1], c1 = s2[j1];
+            float4 a2 = a1, b2 = b1, c2 = c1;
+            if (i >= 1) {
+                a2 = s0[j2];
+                b2 = s1[j2];
+                c2 = s2[j2];
+            }
+            for (; i >= 0; i -= 2) {
+                const int n1 = i >= 2 ? static_cast<int>(sList[wave][i - 2]) : BATCH;
+                const int n2 = i >= 3 ? static_cast<int>(sList[wave][i - 3]) : BATCH;
                 float d1[9], d2[9];
-                bool any = visit(b, j1, d1);
+                bool any = visit(a1, b1, c1, base + j1, d1);
                 if (i >= 1) {
-                    any |= visit(b, j2, d2);
+                    any |= visit(a2, b2, c2, base + j2, d2);
                 } else {
 #pragma unroll
                     for (int c = 0; c < 9; ++c) d2[c] = 0.0f;
                 }
-                if (__ballot(any) != 0)
-                    pairSumStore(d1, d2, &sGrad[wave][secondSplat ? j2 : j1][rowHalf * 5]);
+                if (i >= 2) {
+                    a1 = s0[n1];
+                    b1 = s1[n1];
+                    c1 = s2[n1];
+                }
+                if (i >= 3) {
+                    a2 = s0[n2];
+                    b2 = s1[n2];
+                    c2 = s2[n2];
+                }
+                if (__ballot(any) != 0) pairSumStore(d1, d2, &sGrad[wave][secondSplat ? j2 : j1][rowHalf * 5]);
+                j1 = n1;
+                j2 = n2;
             }
         }
         __syncthreads();

@@ -694,6 +694,8 @@ __device__ inline void blendTile(int tileX, int tileY, uint2 range, const uint32
     }
 }
 
+// Workgroups take tiles in raster order: neighbouring tiles share splat records in L2, and a
+// longest-list-first order measured slower on this path.
 template <int TW, int TH, int BATCH, bool CULL, bool SAVE>
 __global__ __launch_bounds__(TW* TH) void k_blend(const uint2* __restrict__ ranges,
                                                   const uint32_t* __restrict__ sortedGauss,
