@@ -75,6 +75,8 @@ SIGNATURES = {
     "ans_diff_backward": (ctypes.c_int, [P, ctypes.c_int64, ctypes.POINTER(Params_), P, ctypes.POINTER(Grads_), P]),
     "ans_diff_release": (ctypes.c_int, [P, ctypes.c_int64]),
     "ans_diff_splat_grads": (ctypes.c_int, [P, ctypes.c_int64, P]),
+    "ans_diff_radii": (ctypes.c_int, [P, ctypes.c_int64, P, P]),
+    "ans_diff_screen_grads": (ctypes.c_int, [P, ctypes.c_int64, P, P]),
     "ans_diff_frame_counts": (ctypes.c_int, [P, ctypes.c_int64, P, P, P]),
     "ans_diff_frame_dump": (ctypes.c_int, [P, ctypes.c_int64, P, P]),
 }

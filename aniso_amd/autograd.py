@@ -76,58 +76,83 @@ def _stream():
 
 
 class RasterizeFunction(torch.autograd.Function):
+    """Inputs: the five raw parameter tensors, an optional `means2d` leaf ([N, 2], only its
+    gradient is used: it receives dL/d(u, v), the screen-space mean gradient), the camera, the
+    options, optional stats. Outputs: the image and the per-Gaussian radii (not differentiable)."""
+
     @staticmethod
-    def forward(ctx, means, log_scales, quats, opacity_logits, sh, camera: Camera, options: RenderOptions,
+    def forward(ctx, means, log_scales, quats, opacity_logits, sh, means2d, camera: Camera, options: RenderOptions,
                 stats: RenderStats | None):
         n = means.shape[0]
         k = sh.shape[1]
         for t, name, shape in ((means, "means", (n, 3)), (log_scales, "log_scales", (n, 3)), (quats, "quats", (n, 4)),
                                (opacity_logits, "opacity_logits", (n,)), (sh, "sh", (n, k, 3))):
             _check(t, name, shape)
+        if means2d is not None:
+            _check(means2d, "means2d", (n, 2))
         means, log_scales, quats, opacity_logits, sh = (t.contiguous() for t in
                                                         (means, log_scales, quats, opacity_logits, sh))
         dev = means.device.index if means.device.index is not None else torch.cuda.current_device()
         handle = _handle(dev)
+        L = _lib.load()
         image = torch.empty((camera.height, camera.width, 3), dtype=torch.float32, device=means.device)
         fid = ctypes.c_int64()
         st = _lib.Stats_()
-        _lib.check(_lib.load().ans_diff_forward(handle, ctypes.byref(_params(means, log_scales, quats, opacity_logits,
-                                                                               sh)),
-                                                ctypes.byref(camera_c(camera)), ctypes.byref(options.c()),
-                                                image.data_ptr(), _stream(), ctypes.byref(fid), ctypes.byref(st)))
+        _lib.check(L.ans_diff_forward(handle, ctypes.byref(_params(means, log_scales, quats, opacity_logits, sh)),
+                                      ctypes.byref(camera_c(camera)), ctypes.byref(options.c()), image.data_ptr(),
+                                      _stream(), ctypes.byref(fid), ctypes.byref(st)))
         frame = _Frame(handle, fid.value)
+        radii = torch.empty(n, dtype=torch.int32, device=means.device)
+        _lib.check(L.ans_diff_radii(handle, fid.value, radii.data_ptr(), _stream()))
         if stats is not None:
             for f in ("visible", "tile_pairs", "project_ms", "sort_ms", "blend_ms", "pair_capacity", "retries"):
                 setattr(stats, f, getattr(st, f))
         # Inside forward, grad mode is off; needs_input_grad says whether a backward will come.
-        if any(ctx.needs_input_grad[:5]):
+        if any(ctx.needs_input_grad[:6]):
             ctx.frame = frame
             ctx.save_for_backward(means, log_scales, quats, opacity_logits, sh)
         else:
             frame.release()
-        return image
+        ctx.mark_non_differentiable(radii)
+        return image, radii
 
     @staticmethod
-    def backward(ctx, grad_image):
+    def backward(ctx, grad_image, grad_radii):
         means, log_scales, quats, opacity_logits, sh = ctx.saved_tensors
         grads = [torch.empty_like(t) for t in (means, log_scales, quats, opacity_logits, sh)]
         g = grad_image.contiguous().float()
         frame = ctx.frame
-        _lib.check(_lib.load().ans_diff_backward(frame.handle, frame.fid,
-                                                 ctypes.byref(_params(means, log_scales, quats, opacity_logits, sh)),
-                                                 g.data_ptr(), ctypes.byref(_lib.Grads_(*[t.data_ptr() for t in grads])),
-                                                 _stream()))
+        L = _lib.load()
+        _lib.check(L.ans_diff_backward(frame.handle, frame.fid,
+                                       ctypes.byref(_params(means, log_scales, quats, opacity_logits, sh)),
+                                       g.data_ptr(), ctypes.byref(_lib.Grads_(*[t.data_ptr() for t in grads])),
+                                       _stream()))
+        grad_uv = None
+        if ctx.needs_input_grad[5]:
+            grad_uv = torch.empty((means.shape[0], 2), dtype=torch.float32, device=means.device)
+            _lib.check(L.ans_diff_screen_grads(frame.handle, frame.fid, grad_uv.data_ptr(), _stream()))
         # Releasing on the host is safe: the device buffers go back to a pool that is only reused
         # by later work on the same stream.
         frame.release()
-        return (*grads, None, None, None)
+        return (*grads, grad_uv, None, None, None)
 
 
 def rasterize(means, log_scales, quats, opacity_logits, sh, camera: Camera, options: RenderOptions | None = None,
-              stats: RenderStats | None = None):
-    """Differentiable render: unclamped colour image [H, W, 3] (float32)."""
-    return RasterizeFunction.apply(means, log_scales, quats, opacity_logits, sh, camera, options or RenderOptions(),
-                                   stats)
+              stats: RenderStats | None = None, info: bool = False):
+    """Differentiable render: unclamped colour image [H, W, 3] (float32).
+
+    With info=True returns (image, info) where info holds what a densification strategy reads
+    (gsplat's `info`, pipeline/train_gsplat.py:177, :185): "radii" (int32 [N], pixels, 0 if
+    culled), "means2d" (a zero [N, 2] leaf whose .grad receives dL/d(u, v) in pixels after
+    backward), "width", "height", "n_cameras" (1)."""
+    opts = options or RenderOptions()
+    if not info:
+        image, _ = RasterizeFunction.apply(means, log_scales, quats, opacity_logits, sh, None, camera, opts, stats)
+        return image
+    means2d = torch.zeros((means.shape[0], 2), dtype=torch.float32, device=means.device, requires_grad=True)
+    image, radii = RasterizeFunction.apply(means, log_scales, quats, opacity_logits, sh, means2d, camera, opts, stats)
+    return image, {"radii": radii, "means2d": means2d, "width": camera.width, "height": camera.height,
+                   "n_cameras": 1}
 
 
 def frame_lists(means, log_scales, quats, opacity_logits, sh, camera: Camera, options: RenderOptions | None = None):

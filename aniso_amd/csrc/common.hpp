@@ -32,7 +32,7 @@ struct alignas(64) SplatRec {
     float4 geo;   // u, v (pixels), conic A, B
     float4 geo2;  // conic C, opacity, alpha-region half-extents x, y
     float4 col;   // r, g, b, depth
-    uint4 meta;   // tile rect (x0 | y0 << 16, x1 | y1 << 16), tiles touched (0: culled), 0
+    uint4 meta;   // tile rect (x0 | y0 << 16, x1 | y1 << 16), tiles touched (0: culled), radius (px)
 };
 
 // Small per-frame control block in device memory. The init kernel clears it; kernels publish

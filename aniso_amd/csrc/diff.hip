@@ -24,6 +24,21 @@ static void launchBlendBackward(Engine& e, FrameState& f, const float* gradImage
         e.partial.p, f.ctl.p);
 }
 
+// Per Gaussian: its splat radius in pixels (0 if culled).
+static __global__ __launch_bounds__(256) void k_radii(uint32_t n, const SplatRec* __restrict__ rec,
+                                                      const uint32_t* __restrict__ tilesTouched,
+                                                      int32_t* __restrict__ radii) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) radii[i] = tilesTouched[i] ? static_cast<int32_t>(rec[i].meta.w) : 0;
+}
+
+// Per Gaussian: dL/du, dL/dv (columns 0 and 1 of the splat-record gradient).
+static __global__ __launch_bounds__(256) void k_screenGrads(uint32_t n, const float* __restrict__ splatGrad,
+                                                            float2* __restrict__ out) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = make_float2(splatGrad[static_cast<size_t>(i) * 9], splatGrad[static_cast<size_t>(i) * 9 + 1]);
+}
+
 void Engine::enqueueBackward(FrameState& f, const SceneDev& s, const float* gradImage, float* gMeans,
                              float* gScales, float* gQuats, float* gOpacity, float* gSh) {
     if (f.n == 0) return;
@@ -192,6 +207,36 @@ int ans_diff_splat_grads(ans_diff* d, int64_t frame, float* out) {
         hipCheck(hipStreamSynchronize(e.stream), "hipStreamSynchronize");
         if (f.n && e.splatGrad.cap >= f.n * 9)
             hipCheck(hipMemcpy(out, e.splatGrad.p, f.n * 9 * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy");
+        return ANS_OK;
+    });
+}
+
+int ans_diff_radii(ans_diff* d, int64_t frame, int32_t* radii, void* stream) {
+    return guardedDiff([&] {
+        const FrameState& f = frameOf(d, frame);
+        if (!radii && f.n) throw std::invalid_argument("radii is null");
+        const uint32_t n = static_cast<uint32_t>(f.n);
+        if (n) {
+            k_radii<<<(n + 255) / 256, 256, 0, static_cast<hipStream_t>(stream)>>>(n, f.rec.p, f.tilesTouched.p, radii);
+            hipCheck(hipGetLastError(), "kernel launch");
+        }
+        return ANS_OK;
+    });
+}
+
+int ans_diff_screen_grads(ans_diff* d, int64_t frame, float* grad_uv, void* stream) {
+    return guardedDiff([&] {
+        const FrameState& f = frameOf(d, frame);
+        Engine& e = *d->engine;
+        if (!grad_uv && f.n) throw std::invalid_argument("grad_uv is null");
+        if (reinterpret_cast<uintptr_t>(grad_uv) % 8 != 0) throw std::invalid_argument("grad_uv must be 8-byte aligned");
+        if (f.n && e.splatGrad.cap < f.n * 9) throw std::invalid_argument("no backward has run on this frame");
+        const uint32_t n = static_cast<uint32_t>(f.n);
+        if (n) {
+            k_screenGrads<<<(n + 255) / 256, 256, 0, static_cast<hipStream_t>(stream)>>>(
+                n, e.splatGrad.p, reinterpret_cast<float2*>(grad_uv));
+            hipCheck(hipGetLastError(), "kernel launch");
+        }
         return ANS_OK;
     });
 }

@@ -119,6 +119,7 @@ struct GaussProj {
     float dir[3], dlen;
     float basis[16];
     int shCount;
+    int radius;  // ceil(3 sqrt(lambda_max)), pixels
 };
 
 // Projection of one Gaussian (the maths of src/render.cpp:39-104 with the FP32 camera transform
@@ -223,6 +224,7 @@ __host__ __device__ inline bool finishProjection(const F3 m, float opacityIn, co
     const float rf = ceilf(3.0f * sqrtf(lambda));
     if (!(rf < 16777216.0f)) return false;
     const int radius = static_cast<int>(rf);
+    g.radius = radius;
     g.u = cam.fx * px / pz + cam.cx;
     g.v = cam.fy * py / pz + cam.cy;
     if (!__builtin_isfinite(g.u) || !__builtin_isfinite(g.v)) return false;
@@ -286,7 +288,8 @@ __device__ inline void storeSplat(const GaussProj& g, size_t o, SplatRec* __rest
     r.geo2 = make_float4(g.C, g.opacity, ex, ey);
     r.col = make_float4(g.rgb.x, g.rgb.y, g.rgb.z, g.depth);
     r.meta = make_uint4(static_cast<uint32_t>(g.x0) | (static_cast<uint32_t>(g.y0) << 16),
-                        static_cast<uint32_t>(g.x1) | (static_cast<uint32_t>(g.y1) << 16), tt, 0u);
+                        static_cast<uint32_t>(g.x1) | (static_cast<uint32_t>(g.y1) << 16), tt,
+                        static_cast<uint32_t>(g.radius));
     tilesTouched[o] = tt;
     depthKey[o] = __float_as_uint(g.depth);
 }

@@ -200,6 +200,14 @@ int ans_diff_release(ans_diff* d, int64_t frame);
  * gradient that densification heuristics read). Valid until the next backward on the handle. */
 int ans_diff_splat_grads(ans_diff* d, int64_t frame, float* out);
 
+/* What a densification strategy reads next to the image (gsplat's `info`, consumed at
+ * pipeline/train_gsplat.py:177, :185), as DEVICE arrays on `stream`: per-Gaussian splat radius in
+ * pixels, 0 if culled (int32 [n]; any time after ans_diff_forward), and after ans_diff_backward on
+ * the frame, the screen-space mean gradient dL/d(u, v) (float32 [n][2], 8-byte aligned; valid
+ * until the next backward on the handle). */
+int ans_diff_radii(ans_diff* d, int64_t frame, int32_t* radii, void* stream);
+int ans_diff_screen_grads(ans_diff* d, int64_t frame, float* grad_uv, void* stream);
+
 /* Per-tile lists of a frame (sizes from ans_diff_frame_counts), for gradient checks against a
  * CPU reference evaluated on the same lists: values [pairs], ranges [tiles][2]. */
 int ans_diff_frame_counts(ans_diff* d, int64_t frame, int64_t* visible, int64_t* pairs, int64_t* tiles);

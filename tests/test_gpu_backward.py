@@ -162,3 +162,38 @@ def test_741883_l1_gradients_vs_fp64_reference(c4_case):
         print(f"  vs FP64 central differences {name:15s} rel {err:.2e}", flush=True)
         worst = max(worst, err)
     assert worst < TOL
+
+
+def test_info_radii_and_screen_gradients(gpu):
+    """rasterize(info=True): radii equal the oracle's (key-exact mode on the host-activated scene;
+    device and host activation may differ in a last bit, so a handful of radii may differ by one),
+    culled Gaussians have radius 0, and means2d.grad equals dL/d(u, v) of the splat-record gradient
+    from the same kernels, and the FP64 reference's within the gradient tolerance."""
+    raw, cam = l1_case(20000, 320, 180, seed=11, scale_boost=1.0)
+    rng = np.random.default_rng(5)
+    dLdC = rng.normal(size=(cam.height, cam.width, 3)).astype(np.float32)
+    params = tensors(raw)
+    img, info = rasterize(*params, cam, info=True)
+    (img * torch.tensor(dLdC, device="cuda")).sum().backward()
+    radii = info["radii"].cpu().numpy()
+    g2d = info["means2d"].grad.cpu().numpy()
+    with oracle.Frame(oracle.activate(raw), cam, oracle.KEYEXACT, tile=(16, 16)) as f:
+        _, _, radius, alive = f.splats()
+    alive = alive.astype(bool)
+    assert np.all(radii[~alive] == 0)
+    diff = radii[alive] != radius[alive]
+    print(f"radii: {alive.sum()} visible, {diff.sum()} differ", flush=True)
+    assert diff.sum() <= max(2, alive.sum() // 10000)
+    assert np.all(np.abs(radii[alive].astype(int) - radius[alive].astype(int)) <= 1)
+    # Same kernels, same frame: the splat-record gradient's first two columns.
+    from aniso_amd.autograd import forward_backward
+    with torch.no_grad():
+        _, _, splat = forward_backward(*[p.detach() for p in params], cam,
+                                       torch.tensor(dLdC, device="cuda"))
+    assert np.array_equal(g2d, splat[:, :2])
+    # FP64 reference on the GPU's lists.
+    start, lst, _ = frame_lists(*[p.detach() for p in params], cam)
+    _, _, _, sg = oracle.grad_run(raw, cam, dLdC, lists=(start, lst), splat_grad=True)
+    err = np.linalg.norm(g2d - sg[:, :2]) / np.linalg.norm(sg[:, :2])
+    print(f"means2d grad rel err vs FP64: {err:.2e}", flush=True)
+    assert err < TOL
