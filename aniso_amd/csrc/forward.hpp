@@ -18,6 +18,7 @@ static __global__ __launch_bounds__(256) void k_initFrame(FrameCtl* __restrict__
         ctl->visible = 0;
         ctl->pairs = 0;
         ctl->overflow = 0;
+        ctl->big = 0;
     }
     if (i < tiles) ranges[i] = make_uint2(0, 0);
 }
@@ -325,7 +326,11 @@ constexpr int kDupTile = kDupBlock;                  // depth ranks per block
 // totals by view * blockStride, pair arrays by view * pairStride and use ctl[view]; a single frame
 // is gridDim.y = 1.
 
-// Pairs and visible Gaussians per block of 256 depth ranks. Also gathers each rank's tile count
+// Gaussians with more tiles than this are expanded one wavefront each (k_expandBig).
+constexpr uint32_t kBigPairs = 32;
+
+// Pairs, visible Gaussians and Gaussians with more than kBigPairs tiles per block of 256 depth
+// ranks (blockVisible = visible | wide << 16). Also gathers each rank's tile count
 // and tile rect (one line of its splat record) into depth order (ttByRank, rectByRank; rect left
 // unwritten for culled ranks), so that the expansion reads them with coalesced loads.
 static __global__ __launch_bounds__(kDupBlock) void k_pairTotals(const uint32_t* __restrict__ sortedIdx,
@@ -354,17 +359,19 @@ static __global__ __launch_bounds__(kDupBlock) void k_pairTotals(const uint32_t*
     uint32_t total, visible;
     blockExclusiveScan<kDupBlock>(c, sTmp, &total);
     blockExclusiveScan<kDupBlock>(c ? 1u : 0u, sTmp, &visible);
+    const uint32_t wide = static_cast<uint32_t>(__syncthreads_count(c > kBigPairs));
     if (threadIdx.x == 0) {
         blockPairs[blockIdx.x] = total;
-        blockVisible[blockIdx.x] = visible;
+        blockVisible[blockIdx.x] = visible | (wide << 16);
     }
 }
 
-// One workgroup: exclusive scan of the per-block pair counts; publishes the pair total (the sum
-// gpu_render.cu:324-329 reads back to the host) and the visible count, and flags a pair total
-// above the pair buffers' capacity.
+// One workgroup: exclusive scans of the per-block pair counts and wide-Gaussian counts (the
+// latter replacing blockVisible); publishes the pair total (the sum gpu_render.cu:324-329 reads
+// back to the host), the visible and wide counts, and flags a pair total above the pair buffers'
+// capacity.
 static __global__ __launch_bounds__(kDupBlock) void k_scanPairs(uint32_t* __restrict__ blockPairs,
-                                                                const uint32_t* __restrict__ blockVisible,
+                                                                uint32_t* __restrict__ blockVisible,
                                                                 uint32_t blocks, uint32_t cap,
                                                                 FrameCtl* __restrict__ ctl, uint32_t blockStride) {
     __shared__ uint32_t sTmp[kDupBlock / kWave];
@@ -373,31 +380,41 @@ static __global__ __launch_bounds__(kDupBlock) void k_scanPairs(uint32_t* __rest
     ctl += blockIdx.y;
     constexpr int kPer = 8;
     uint64_t carry = 0;
-    uint32_t visible = 0;
+    uint32_t visible = 0, wideCarry = 0;
     for (uint32_t base = 0; base < blocks; base += kDupBlock * kPer) {
         const uint32_t first = base + threadIdx.x * kPer;
-        uint32_t v[kPer];
-        uint32_t sum = 0, vis = 0;
+        uint32_t v[kPer], wd[kPer];
+        uint32_t sum = 0, vis = 0, wsum = 0;
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const bool in = first + k < blocks;
             v[k] = in ? blockPairs[first + k] : 0u;
-            vis += in ? blockVisible[first + k] : 0u;
+            const uint32_t packed = in ? blockVisible[first + k] : 0u;
+            vis += packed & 0xffffu;
+            wd[k] = packed >> 16;
+            wsum += wd[k];
             sum += v[k];
         }
-        uint32_t total, visTotal;
+        uint32_t total, visTotal, wideTotal;
         uint32_t run = static_cast<uint32_t>(carry) + blockExclusiveScan<kDupBlock>(sum, sTmp, &total);
         blockExclusiveScan<kDupBlock>(vis, sTmp, &visTotal);
+        uint32_t wrun = wideCarry + blockExclusiveScan<kDupBlock>(wsum, sTmp, &wideTotal);
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
-            if (first + k < blocks) blockPairs[first + k] = run;
+            if (first + k < blocks) {
+                blockPairs[first + k] = run;
+                blockVisible[first + k] = wrun;
+            }
             run += v[k];
+            wrun += wd[k];
         }
         carry += total;
         visible += visTotal;
+        wideCarry += wideTotal;
     }
     if (threadIdx.x == 0) {
         ctl->visible = visible;
+        ctl->big = wideCarry;
         ctl->pairs = carry > 0xffffffffull ? 0xffffffffu : static_cast<uint32_t>(carry);
         if (carry > cap) {
             ctl->overflow = 1;
@@ -421,14 +438,22 @@ __device__ inline int waveInclusiveMax(int v) {
 // pairs (tile id, Gaussian index) contiguously, each Gaussian's tiles row-major (the order
 // gpu_render.cu:155-156 writes them), 64 pairs per step with one pair per lane. A step finds
 // the Gaussian owning each slot by marking the slots where Gaussians start (LDS head flags) and
-// a wavefront max-scan.
-// tilesXView (batched views, else null) holds each view's tile-grid width.
+// a wavefront max-scan. Gaussians with more than kBigPairs tiles would serialize their wavefront
+// (depth order puts the nearest, largest splats next to each other), so they are only given
+// their slots here and listed in `big`, and k_expandBig writes their pairs, one wavefront per
+// Gaussian; the head-flag walk runs over the remaining Gaussians' slots.
+
+// tilesXView (batched views, else null) holds each view's tile-grid width; big entries are
+// {first slot, Gaussian, rect.x, rect.y} at view * gaussStride, in rank order from blockBig (the
+// scanned wide counts).
 static __global__ __launch_bounds__(kDupBlock) void k_expandPairs(
     const uint32_t* __restrict__ sortedIdx, const uint32_t* __restrict__ ttByRank,
     const uint2* __restrict__ rectByRank, uint32_t n, int tilesX, const uint32_t* __restrict__ blockBase,
-    uint32_t* __restrict__ tileOut, uint32_t* __restrict__ gaussOut, const FrameCtl* __restrict__ ctl,
-    uint32_t gaussStride, uint32_t blockStride, uint32_t pairStride, const int* __restrict__ tilesXView) {
+    const uint32_t* __restrict__ blockBig, uint32_t* __restrict__ tileOut, uint32_t* __restrict__ gaussOut,
+    const FrameCtl* __restrict__ ctl, uint4* __restrict__ big, uint32_t gaussStride, uint32_t blockStride,
+    uint32_t pairStride, const int* __restrict__ tilesXView) {
     __shared__ uint32_t sWave[kDupBlock / kWave];
+    __shared__ uint32_t sWide[kDupBlock / kWave];
     __shared__ int sHead[kDupBlock / kWave][kWave];
     {
         const uint32_t view = blockIdx.y;
@@ -437,7 +462,9 @@ static __global__ __launch_bounds__(kDupBlock) void k_expandPairs(
         sortedIdx += go;
         ttByRank += go;
         rectByRank += go;
+        big += go;
         blockBase += static_cast<size_t>(view) * blockStride;
+        blockBig += static_cast<size_t>(view) * blockStride;
         tileOut += po;
         gaussOut += po;
         if (tilesXView) tilesX = tilesXView[view];
@@ -454,35 +481,49 @@ static __global__ __launch_bounds__(kDupBlock) void k_expandPairs(
             g = sortedIdx[r];
         }
     }
+    const bool isBig = cnt > kBigPairs;
+    const uint32_t cntS = isBig ? 0u : cnt;  // slots the head-flag walk writes
     const uint32_t incl = waveInclusiveScan(cnt);
     const uint32_t excl = incl - cnt;
+    const uint32_t inclS = waveInclusiveScan(cntS);
+    const uint32_t exclS = inclS - cntS;
     const uint32_t waveTotal = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), kWave - 1));
-    if (lane == 0) sWave[wave] = waveTotal;
+    const uint32_t waveSmall = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(inclS), kWave - 1));
+    const uint64_t bigLanes = __ballot(isBig);
+    if (lane == 0) {
+        sWave[wave] = waveTotal;
+        sWide[wave] = static_cast<uint32_t>(__popcll(bigLanes));
+    }
     __syncthreads();
     uint32_t out = blockBase[blockIdx.x];
-    for (int w = 0; w < wave; ++w) out += sWave[w];
+    uint32_t wideBase = blockBig[blockIdx.x];
+    for (int w = 0; w < wave; ++w) {
+        out += sWave[w];
+        wideBase += sWide[w];
+    }
+    if (isBig) big[wideBase + __popcll(bigLanes & lanesBelow())] = make_uint4(out + excl, g, rp.x, rp.y);
 
     const uint32_t x0 = rp.x & 0xffffu, y0 = rp.x >> 16;
     const uint32_t w = (rp.y & 0xffffu) - x0 + 1;
     int carry = -1;
-    for (uint32_t chunk = 0; chunk < waveTotal; chunk += kWave) {
+    for (uint32_t chunk = 0; chunk < waveSmall; chunk += kWave) {
         waveSync();
         sHead[wave][lane] = -1;
         waveSync();
-        if (cnt && excl >= chunk && excl < chunk + kWave) sHead[wave][excl - chunk] = lane;
+        if (cntS && exclS >= chunk && exclS < chunk + kWave) sHead[wave][exclS - chunk] = lane;
         waveSync();
         const int owner = max(carry, waveInclusiveMax(sHead[wave][lane]));
         carry = __builtin_amdgcn_readlane(owner, kWave - 1);
+        const uint32_t oStartS = static_cast<uint32_t>(__shfl(static_cast<int>(exclS), owner, kWave));
         const uint32_t oStart = static_cast<uint32_t>(__shfl(static_cast<int>(excl), owner, kWave));
         const uint32_t oX0 = static_cast<uint32_t>(__shfl(static_cast<int>(x0), owner, kWave));
         const uint32_t oY0 = static_cast<uint32_t>(__shfl(static_cast<int>(y0), owner, kWave));
         const uint32_t oW = static_cast<uint32_t>(__shfl(static_cast<int>(w), owner, kWave));
         const uint32_t oG = static_cast<uint32_t>(__shfl(static_cast<int>(g), owner, kWave));
-        const uint32_t slot = chunk + lane;
-        if (slot < waveTotal) {
-            // row = k / oW with a float reciprocal and one correction each way (rows are far
-            // below 2^20, so the estimate is within one of the quotient).
-            const uint32_t k = slot - oStart;
+        const uint32_t slotS = chunk + lane;
+        if (slotS < waveSmall) {
+            // row = k / oW with a float reciprocal and one correction each way (k <= kBigPairs).
+            const uint32_t k = slotS - oStartS;
             uint32_t row = static_cast<uint32_t>(static_cast<float>(k) * __builtin_amdgcn_rcpf(static_cast<float>(oW)));
             int32_t col = static_cast<int32_t>(k) - static_cast<int32_t>(row * oW);
             if (col < 0) {
@@ -492,8 +533,49 @@ static __global__ __launch_bounds__(kDupBlock) void k_expandPairs(
                 row += 1;
                 col -= static_cast<int32_t>(oW);
             }
-            tileOut[out + slot] = (oY0 + row) * static_cast<uint32_t>(tilesX) + oX0 + static_cast<uint32_t>(col);
-            gaussOut[out + slot] = oG;
+            const uint32_t slot = out + oStart + k;
+            tileOut[slot] = (oY0 + row) * static_cast<uint32_t>(tilesX) + oX0 + static_cast<uint32_t>(col);
+            gaussOut[slot] = oG;
+        }
+    }
+}
+
+// Pairs of the Gaussians k_expandPairs listed in `big`: one wavefront per Gaussian (grid-strided
+// over the list), lane l writing tiles l, l + 64, ... of its rect, row-major, from its first slot.
+constexpr int kBigBlock = 256;
+static __global__ __launch_bounds__(kBigBlock) void k_expandBig(const uint4* __restrict__ big, int tilesX,
+                                                                uint32_t* __restrict__ tileOut,
+                                                                uint32_t* __restrict__ gaussOut,
+                                                                const FrameCtl* __restrict__ ctl, uint32_t gaussStride,
+                                                                uint32_t pairStride,
+                                                                const int* __restrict__ tilesXView) {
+    const uint32_t view = blockIdx.y;
+    ctl += view;
+    if (ctl->overflow) return;
+    big += static_cast<size_t>(view) * gaussStride;
+    tileOut += static_cast<size_t>(view) * pairStride;
+    gaussOut += static_cast<size_t>(view) * pairStride;
+    if (tilesXView) tilesX = tilesXView[view];
+    const uint32_t count = ctl->big;
+    const uint32_t lane = __lane_id();
+    const uint32_t wavesPerBlock = kBigBlock / kWave;
+    for (uint32_t e = blockIdx.x * wavesPerBlock + threadIdx.x / kWave; e < count; e += gridDim.x * wavesPerBlock) {
+        const uint4 b = big[e];
+        const uint32_t x0 = b.z & 0xffffu, y0 = b.z >> 16;
+        const uint32_t w = (b.w & 0xffffu) - x0 + 1, h = (b.w >> 16) - y0 + 1;
+        uint32_t slot = b.x + lane;
+        // Walk the rect row-major 64 tiles at a time, carrying (row, col) instead of dividing.
+        uint32_t row = lane / w, col = lane - (lane / w) * w;
+        const uint32_t stepRows = kWave / w, stepCols = kWave - stepRows * w;
+        for (uint32_t k = lane; k < w * h; k += kWave, slot += kWave) {
+            tileOut[slot] = (y0 + row) * static_cast<uint32_t>(tilesX) + x0 + col;
+            gaussOut[slot] = b.y;
+            row += stepRows;
+            col += stepCols;
+            if (col >= w) {
+                col -= w;
+                row += 1;
+            }
         }
     }
 }

@@ -249,6 +249,7 @@ WORKLOADS = {
     "synthetic-200k-640x360": dict(n=200_000, width=640, height=360, clustered=False),
     "synthetic-741883-1280x720": dict(n=741_883, width=1280, height=720, clustered=False),
     "clustered-2.2M-1280x720": dict(n=2_200_000, width=1280, height=720, clustered=True),
+    "clustered-2.2M-1920x1080": dict(n=2_200_000, width=1920, height=1080, clustered=True),
     # 16 cameras of mixed sizes (view_batch_cameras); the scene fills the 1280x720 frustum.
     "batched-1M-16views": dict(n=1_000_000, width=1280, height=720, clustered=False, views=16),
 }

@@ -124,6 +124,8 @@ def main():
     if not args.skip_extra:
         extra["fwd_bwd"] = bench_fwd_bwd(raw, cam, max(5, args.steps // 4), args.warmup)
         extra["fwd_clustered_2.2M"] = bench_forward("clustered-2.2M-1280x720", max(10, args.steps // 2), args.warmup)
+        extra["fwd_clustered_2.2M_1080p"] = bench_forward("clustered-2.2M-1920x1080", max(10, args.steps // 2),
+                                                         args.warmup)
         extra["batched_16_views"] = bench_views("batched-1M-16views", max(5, args.steps // 10), args.warmup)
 
     cpu = None

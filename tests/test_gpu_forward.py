@@ -139,6 +139,18 @@ def test_741883_gaussians_1280x720_full_size(gpu):
               f"{np.abs(img.astype(int) - ref.astype(int)).max()}/255", flush=True)
 
 
+def test_clustered_2_2m_1920x1080_full_size(gpu):
+    """2.2M clustered Gaussians (half in 50 blobs) at 1920x1080: bit-exact records, keys, lists and
+    ranges, images within 2/255 of the key-exact oracle, PSNR vs the compiled reference's image."""
+    act = oracle.activate(scenes.synthetic_scene(2_200_000, 1920, 1080, clustered=True))
+    cam = scenes.pinhole_camera(1920, 1080)
+    with GpuRenderer(act, activated=True) as r:
+        img, st = check_frame(r, act, cam, label="clustered-2.2M-1080p")
+        ref = oracle.render(act, cam, oracle.REFERENCE, tile=(8, 8))
+        print(f"  pairs {st.tile_pairs}; vs compiled-reference image: PSNR {psnr(img, ref):.1f} dB, max diff "
+              f"{np.abs(img.astype(int) - ref.astype(int)).max()}/255", flush=True)
+
+
 @pytest.mark.parametrize("tile", [(8, 8), (16, 16), (32, 16)])
 def test_blend_culling_changes_nothing(gpu, tile):
     """The blend's per-8x8-block culling is conservative: float images with and without it are
