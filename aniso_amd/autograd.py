@@ -97,10 +97,10 @@ class RasterizeFunction(torch.autograd.Function):
         L = _lib.load()
         image = torch.empty((camera.height, camera.width, 3), dtype=torch.float32, device=means.device)
         fid = ctypes.c_int64()
-        st = _lib.Stats_()
+        st = _lib.Stats_() if stats is not None else None  # stats wait for the frame to finish
         _lib.check(L.ans_diff_forward(handle, ctypes.byref(_params(means, log_scales, quats, opacity_logits, sh)),
                                       ctypes.byref(camera_c(camera)), ctypes.byref(options.c()), image.data_ptr(),
-                                      _stream(), ctypes.byref(fid), ctypes.byref(st)))
+                                      _stream(), ctypes.byref(fid), ctypes.byref(st) if st is not None else None))
         frame = _Frame(handle, fid.value)
         radii = torch.empty(n, dtype=torch.int32, device=means.device)
         _lib.check(L.ans_diff_radii(handle, fid.value, radii.data_ptr(), _stream()))

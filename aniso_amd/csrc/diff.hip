@@ -160,7 +160,7 @@ int ans_diff_forward(ans_diff* d, const ans_params* params, const ans_camera* ca
             f = std::make_unique<FrameState>();
         }
         f->savePixels = true;
-        e.renderSync(*f, s, cp, o, nullptr, image);
+        e.renderCounted(*f, s, cp, o, nullptr, image);
         e.fillStats(*f, stats);
         const int64_t id = d->next++;
         d->frames[id] = std::move(f);

@@ -90,6 +90,7 @@ struct Engine {
     static constexpr int kEventRing = 64;
     hipEvent_t ev[kEventRing][4] = {};
     int evNext = 0, evPending = 0;
+    hipEvent_t* curEv = nullptr;  // events of the frame being enqueued
     FrameCtl* ctlHost = nullptr;  // pinned
 
     DevBuf<uint32_t> depthKey, sortKA, sortVA, sortKB, sortVB;
@@ -113,6 +114,8 @@ struct Engine {
     void reserveScratch(size_t n, size_t pairCap);
     void enqueueFrame(FrameState& f, const SceneDev& s, const CamParams& cp, const ans_options& o, uint8_t* out8,
                       float* outf);
+    void enqueueCount(FrameState& f, const SceneDev& s, const CamParams& cp, const ans_options& o);
+    void enqueueRest(FrameState& f, uint8_t* out8, float* outf);
     void launchBlend(FrameState& f, uint8_t* out8, float* outf);
     template <int TW, int TH>
     void launchBlendT(FrameState& f, bool cull, uint8_t* out8, float* outf);
@@ -120,8 +123,8 @@ struct Engine {
     void launchBlendB(FrameState& f, bool cull, uint8_t* out8, float* outf);
     // Copies the frame's control block to the host and waits; true if the pair buffers overflowed.
     bool readBack(FrameState& f);
-    int renderSync(FrameState& f, const SceneDev& s, const CamParams& cp, const ans_options& o, uint8_t* out8,
-                   float* outf);
+    int renderCounted(FrameState& f, const SceneDev& s, const CamParams& cp, const ans_options& o, uint8_t* out8,
+                      float* outf);
     void fillStats(const FrameState& f, ans_stats* st);
     void enqueueBackward(FrameState& f, const SceneDev& s, const float* gradImage, float* gMeans, float* gScales,
                          float* gQuats, float* gOpacity, float* gSh);

@@ -134,6 +134,13 @@ void Engine::reserveScratch(size_t n, size_t pairCap) {
 // pointers.
 void Engine::enqueueFrame(FrameState& f, const SceneDev& s, const CamParams& cp, const ans_options& o,
                           uint8_t* out8, float* outf) {
+    enqueueCount(f, s, cp, o);
+    enqueueRest(f, out8, outf);
+}
+
+// First part of a frame: preprocess, depth sort, pair counts and their scan (the pair total is
+// in the control block when it completes).
+void Engine::enqueueCount(FrameState& f, const SceneDev& s, const CamParams& cp, const ans_options& o) {
     const uint32_t n = static_cast<uint32_t>(s.n);
     const uint32_t tiles = static_cast<uint32_t>(cp.tilesX) * static_cast<uint32_t>(cp.tilesY);
     if (f.pairCap == 0) f.reservePairs(std::max<size_t>(static_cast<size_t>(n) * 8, size_t(1) << 20));
@@ -145,6 +152,7 @@ void Engine::enqueueFrame(FrameState& f, const SceneDev& s, const CamParams& cp,
     f.opts = o;
     FrameCtl* ctl = f.ctl.p;
     hipEvent_t* E = ev[evNext];
+    curEv = E;
     evNext = (evNext + 1) % kEventRing;
     evPending = std::min(evPending + 1, kEventRing);
 
@@ -181,7 +189,24 @@ void Engine::enqueueFrame(FrameState& f, const SceneDev& s, const CamParams& cp,
         }
     }
     hipCheck(hipEventRecord(E[1], stream), "hipEventRecord");
-    // Pairs in depth order, then a stable sort by tile id.
+    if (n > 0) {
+        const uint32_t blocksDup = (n + kDupTile - 1) / kDupTile;
+        k_pairTotals<<<blocksDup, kDupBlock, 0, stream>>>(f.order.p, f.rec.p, n, ttByRank.p, rectByRank.p,
+                                                          blockPairs.p, blockVisible.p, 0, 0);
+        k_scanPairs<<<1, kDupBlock, 0, stream>>>(blockPairs.p, blockVisible.p, blocksDup,
+                                                 static_cast<uint32_t>(f.pairCap), ctl, 0);
+    }
+}
+
+// Second part: pairs in depth order, a stable sort by tile id, tile ranges, blend. The pair
+// buffers may have grown since enqueueCount (their contents are first written here).
+void Engine::enqueueRest(FrameState& f, uint8_t* out8, float* outf) {
+    const uint32_t n = static_cast<uint32_t>(f.n);
+    const uint32_t tiles = static_cast<uint32_t>(f.tiles);
+    const CamParams& cp = f.cp;
+    FrameCtl* ctl = f.ctl.p;
+    hipEvent_t* E = curEv;
+    reserveScratch(n, f.pairCap);
     const int tileBits = std::max(1, bitsFor(tiles - 1));
     const int passes = (tileBits + 7) / 8;
     const int digitBits = (tileBits + passes - 1) / passes;
@@ -190,9 +215,6 @@ void Engine::enqueueFrame(FrameState& f, const SceneDev& s, const CamParams& cp,
     if (n > 0) {
         const uint32_t blocksDup = (n + kDupTile - 1) / kDupTile;
         const uint32_t cap = static_cast<uint32_t>(f.pairCap);
-        k_pairTotals<<<blocksDup, kDupBlock, 0, stream>>>(f.order.p, f.rec.p, n, ttByRank.p, rectByRank.p,
-                                                          blockPairs.p, blockVisible.p, 0, 0);
-        k_scanPairs<<<1, kDupBlock, 0, stream>>>(blockPairs.p, blockVisible.p, blocksDup, cap, ctl, 0);
         k_expandPairs<<<blocksDup, kDupBlock, 0, stream>>>(f.order.p, ttByRank.p, rectByRank.p, n, cp.tilesX,
                                                            blockPairs.p, blockVisible.p, f.tileA.p, f.gaussA.p, ctl,
                                                            bigPairs.p, 0, 0, 0, nullptr);
@@ -270,19 +292,27 @@ bool Engine::readBack(FrameState& f) {
     return false;
 }
 
-// Renders a frame and waits for it; grows the pair buffers and renders again if the pair count
-// exceeded them.
-int Engine::renderSync(FrameState& f, const SceneDev& s, const CamParams& cp, const ans_options& o, uint8_t* out8,
-                       float* outf) {
-    int retries = 0;
-    while (true) {
-        evPending = 0;  // stage times of the frame that completes
-        enqueueFrame(f, s, cp, o, out8, outf);
-        if (!readBack(f)) break;
-        ++retries;
+// Renders a frame, waiting only for its pair count (the reference reads the same total back,
+// gpu_render.cu:324-329): if the pairs exceed the pair buffers they grow before any pair is
+// written, and the rest of the frame is enqueued without waiting for it.
+int Engine::renderCounted(FrameState& f, const SceneDev& s, const CamParams& cp, const ans_options& o,
+                          uint8_t* out8, float* outf) {
+    evPending = 0;  // stage times of this frame
+    enqueueCount(f, s, cp, o);
+    hipCheck(hipMemcpyAsync(ctlHost, f.ctl.p, sizeof(FrameCtl), hipMemcpyDeviceToHost, stream), "hipMemcpyAsync ctl");
+    hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    f.visible = ctlHost->visible;
+    f.pairs = ctlHost->pairs;
+    f.retries = 0;
+    if (ctlHost->overflow) {
+        if (ctlHost->pairs >= (1u << 30) - 1) throw std::runtime_error("pair count exceeds 2^30");
+        f.reservePairs(static_cast<size_t>(ctlHost->pairs) + ctlHost->pairs / 4 + 1024);
+        // The flags only stop the kernels of this frame; with room now, clear them.
+        hipCheck(hipMemsetAsync(&f.ctl.p->overflow, 0, 8, stream), "hipMemsetAsync");  // overflow, sticky
+        f.retries = 1;
     }
-    f.retries = retries;
-    return retries;
+    enqueueRest(f, out8, outf);
+    return f.retries;
 }
 
 void Engine::fillStats(const FrameState& f, ans_stats* st) {
@@ -290,6 +320,7 @@ void Engine::fillStats(const FrameState& f, ans_stats* st) {
         evPending = 0;
         return;
     }
+    hipCheck(hipEventSynchronize(ev[(evNext - 1 + kEventRing) % kEventRing][3]), "hipEventSynchronize");
     // Stage times averaged over the frames enqueued since the last readout (the newest
     // kEventRing of them); every one of them has completed by now.
     double a = 0, b = 0, c = 0;
@@ -473,12 +504,15 @@ int ans_render_float(ans_renderer* r, const ans_camera* cam, const ans_options* 
         r->image8.reserve(3 * pixels);
         if (rgbf_out) r->imagef.reserve(3 * pixels);
         Engine& e = *r->engine;
-        e.renderSync(e.frame, r->scene, cp, o, r->image8.p, rgbf_out ? r->imagef.p : nullptr);
+        e.renderCounted(e.frame, r->scene, cp, o, r->image8.p, rgbf_out ? r->imagef.p : nullptr);
         if (rgb_out)
-            hipCheck(hipMemcpy(rgb_out, r->image8.p, 3 * pixels, hipMemcpyDeviceToHost), "hipMemcpy image");
+            hipCheck(hipMemcpyAsync(rgb_out, r->image8.p, 3 * pixels, hipMemcpyDeviceToHost, e.stream),
+                     "hipMemcpyAsync image");
         if (rgbf_out)
-            hipCheck(hipMemcpy(rgbf_out, r->imagef.p, 3 * pixels * sizeof(float), hipMemcpyDeviceToHost),
-                     "hipMemcpy image");
+            hipCheck(hipMemcpyAsync(rgbf_out, r->imagef.p, 3 * pixels * sizeof(float), hipMemcpyDeviceToHost,
+                                    e.stream),
+                     "hipMemcpyAsync image");
+        hipCheck(hipStreamSynchronize(e.stream), "hipStreamSynchronize");
         e.fillStats(e.frame, stats);
         return ANS_OK;
     });

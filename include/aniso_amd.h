@@ -183,8 +183,11 @@ int ans_diff_create(ans_diff** out);
 void ans_diff_destroy(ans_diff* d);
 
 /* Forward: renders the unclamped colour image [height][width][3] into `image` and keeps the
- * frame's state (splat records, tile lists) for the backward pass under *frame. Waits for the
- * pair count (as gpu_render.cu:329 does) and renders again if the pair buffers overflowed. */
+ * frame's state (splat records, tile lists) for the backward pass under *frame. The call waits
+ * only for the pair count (gpu_render.cu:329 reads the same total back); if it exceeds the pair
+ * buffers they grow before any pair is written. The rest of the frame is enqueued on `stream`
+ * and not waited for: work the caller enqueues after the call sees the finished image. A non-NULL
+ * stats waits for the frame. */
 int ans_diff_forward(ans_diff* d, const ans_params* params, const ans_camera* cam, const ans_options* options,
                      float* image, void* stream, int64_t* frame, ans_stats* stats);
 
