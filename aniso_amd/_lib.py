@@ -46,6 +46,16 @@ class Grads_(ctypes.Structure):
                 ("opacity_logits", ctypes.c_void_p), ("sh", ctypes.c_void_p)]
 
 
+class PathStats_(ctypes.Structure):
+    _fields_ = [("frames", ctypes.c_int64), ("chunks", ctypes.c_int64), ("gpu_ms", ctypes.c_double),
+                ("total_ms", ctypes.c_double)]
+
+
+# ans_frame_sink: int (*)(void* user, int64_t frame, const uint8_t* rgb, size_t bytes)
+FrameSink = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_uint8),
+                             ctypes.c_size_t)
+
+
 # Every symbol include/aniso_amd.h declares, with (restype, argtypes).
 SIGNATURES = {
     "ans_available": (ctypes.c_int, []),
@@ -66,6 +76,8 @@ SIGNATURES = {
                                         ctypes.POINTER(Stats_)]),
     "ans_views_autotune": (ctypes.c_int, [P, ctypes.c_int32, ctypes.POINTER(Camera_), ctypes.POINTER(Options_), P, P,
                                           ctypes.c_int32, ctypes.POINTER(Options_), ctypes.POINTER(ctypes.c_double)]),
+    "ans_render_path": (ctypes.c_int, [P, ctypes.c_int64, ctypes.POINTER(Camera_), P, ctypes.POINTER(Options_),
+                                       ctypes.c_int32, FrameSink, P, ctypes.POINTER(PathStats_)]),
     "ans_views_frame_counts": (ctypes.c_int, [P, ctypes.c_int32, P, P, P]),
     "ans_views_frame_dump": (ctypes.c_int, [P, ctypes.c_int32, P, P]),
     "ans_diff_create": (ctypes.c_int, [ctypes.POINTER(P)]),

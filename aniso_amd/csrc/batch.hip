@@ -12,9 +12,12 @@
 #include "views.hpp"
 
 #include <algorithm>
+#include <chrono>
+#include <cmath>
 #include <map>
 #include <memory>
 #include <stdexcept>
+#include <string>
 #include <tuple>
 #include <vector>
 
@@ -54,17 +57,21 @@ struct ViewBatch {
     DevBuf<uint2> ranges;
     DevBuf<uint32_t> itemKey, itemK2, itemV2, itemOrder;
 
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[2] = {};
-    uint8_t* tableHost = nullptr;  // pinned staging of the per-view table
-    FrameCtl* ctlHost = nullptr;   // pinned copy of the control blocks
-    size_t pairCap = 0;            // pairs per view
-    uint64_t generation = 0;       // bumped when the arena moves
+    hipStream_t stream = nullptr;  // graphs are captured here and launched on the caller's stream
+    // Per submission slot (two batches may be in flight along a camera path): start and end of
+    // the launches, control-block copy done; pinned staging of the per-view table; pinned copy of
+    // the control blocks; the buffer the batch's sorted lists ended in.
+    hipEvent_t ev[2][3] = {};
+    uint8_t* tableHost[2] = {};
+    FrameCtl* ctlHost[2] = {};
+    const uint32_t* sortedSlot[2] = {};
+    size_t pairCap = 0;       // pairs per view
+    uint64_t generation = 0;  // bumped when the arena moves
 
     using GraphKey = std::tuple<int, int64_t, int, int, int, bool, uint32_t, uint32_t, size_t, uint64_t>;
     std::map<GraphKey, hipGraphExec_t> graphs;
 
-    // The last call, for stats and dumps.
+    // The last finished batch, for stats and dumps.
     int views = 0;
     uint32_t maxTiles = 0;
     std::vector<int> tilesPerView;
@@ -73,22 +80,42 @@ struct ViewBatch {
     float lastMs = 0.0f;
     int lastRetries = 0;
 
+    // Camera-path streaming: its render and copy streams, per-slot device frames and pinned host
+    // frames, copy-done events. Created on first use.
+    hipStream_t pathRender = nullptr, pathCopy = nullptr;
+    hipEvent_t copied[2] = {};
+    DevBuf<uint8_t> pathDev[2];
+    uint8_t* pathHost[2] = {};
+    size_t pathHostCap[2] = {};
+
     ViewBatch() {
         hipCheck(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
-        for (auto& e : ev) hipCheck(hipEventCreate(&e), "hipEventCreate");
-        hipCheck(hipHostMalloc(reinterpret_cast<void**>(&tableHost), TableLayout(kMaxViews).bytes,
-                               hipHostMallocDefault),
-                 "hipHostMalloc");
-        hipCheck(hipHostMalloc(reinterpret_cast<void**>(&ctlHost), sizeof(FrameCtl) * (kMaxViews + 1),
-                               hipHostMallocDefault),
-                 "hipHostMalloc");
+        for (auto& slot : ev)
+            for (auto& e : slot) hipCheck(hipEventCreate(&e), "hipEventCreate");
+        for (int k = 0; k < 2; ++k) {
+            hipCheck(hipHostMalloc(reinterpret_cast<void**>(&tableHost[k]), TableLayout(kMaxViews).bytes,
+                                   hipHostMallocDefault),
+                     "hipHostMalloc");
+            hipCheck(hipHostMalloc(reinterpret_cast<void**>(&ctlHost[k]), sizeof(FrameCtl) * (kMaxViews + 1),
+                                   hipHostMallocDefault),
+                     "hipHostMalloc");
+        }
     }
     ~ViewBatch() {
+        if (pathRender) (void)hipStreamSynchronize(pathRender);
+        if (pathCopy) (void)hipStreamSynchronize(pathCopy);
         dropGraphs();
-        if (tableHost) (void)hipHostFree(tableHost);
-        if (ctlHost) (void)hipHostFree(ctlHost);
-        for (auto& e : ev)
-            if (e) (void)hipEventDestroy(e);
+        for (int k = 0; k < 2; ++k) {
+            if (tableHost[k]) (void)hipHostFree(tableHost[k]);
+            if (ctlHost[k]) (void)hipHostFree(ctlHost[k]);
+            if (pathHost[k]) (void)hipHostFree(pathHost[k]);
+            if (copied[k]) (void)hipEventDestroy(copied[k]);
+        }
+        for (auto& slot : ev)
+            for (auto& e : slot)
+                if (e) (void)hipEventDestroy(e);
+        if (pathRender) (void)hipStreamDestroy(pathRender);
+        if (pathCopy) (void)hipStreamDestroy(pathCopy);
         if (stream) (void)hipStreamDestroy(stream);
     }
     void dropGraphs() {
@@ -296,23 +323,24 @@ Layout makeLayout(const ViewBatch& B, const SceneDev& s, int views, const std::v
     return L;
 }
 
-// Uploads the per-view table and runs the batch on st (graph replay unless ANS_FLAG_NO_GRAPH;
-// graphs are captured on the batch's own stream and launched on st); waits and returns true if
-// any view's pairs overflowed the arena.
-bool runBatch(ViewBatch& B, const SceneDev& s, const Layout& L, const std::vector<CamParams>& cps,
-              uint8_t* const* outs, hipStream_t st) {
+// Uploads the per-view table (through submission slot `slot`'s pinned staging) and enqueues the
+// batch on st: graph replay unless ANS_FLAG_NO_GRAPH (graphs are captured on the batch's own
+// stream and launched on st), then the control blocks' copy to the slot's pinned buffer. Returns
+// without waiting.
+void submitBatch(ViewBatch& B, const SceneDev& s, const Layout& L, const std::vector<CamParams>& cps,
+                 uint8_t* const* outs, hipStream_t st, int slot) {
     const TableLayout T(L.views);
+    uint8_t* th = B.tableHost[slot];
     for (int v = 0; v < L.views; ++v) {
-        reinterpret_cast<CamParams*>(B.tableHost + T.cams)[v] = cps[v];
-        reinterpret_cast<uint8_t**>(B.tableHost + T.outs)[v] = outs[v];
-        reinterpret_cast<int*>(B.tableHost + T.tilesX)[v] = cps[v].tilesX;
-        reinterpret_cast<int*>(B.tableHost + T.tiles)[v] = cps[v].tilesX * cps[v].tilesY;
+        reinterpret_cast<CamParams*>(th + T.cams)[v] = cps[v];
+        reinterpret_cast<uint8_t**>(th + T.outs)[v] = outs[v];
+        reinterpret_cast<int*>(th + T.tilesX)[v] = cps[v].tilesX;
+        reinterpret_cast<int*>(th + T.tiles)[v] = cps[v].tilesX * cps[v].tilesY;
     }
-    hipCheck(hipMemcpyAsync(B.table.p, B.tableHost, T.bytes, hipMemcpyHostToDevice, st), "hipMemcpyAsync table");
-    hipCheck(hipEventRecord(B.ev[0], st), "hipEventRecord");
-    const uint32_t* sorted = nullptr;
+    hipCheck(hipMemcpyAsync(B.table.p, th, T.bytes, hipMemcpyHostToDevice, st), "hipMemcpyAsync table");
+    hipCheck(hipEventRecord(B.ev[slot][0], st), "hipEventRecord");
     if (L.opts.flags & ANS_FLAG_NO_GRAPH) {
-        sorted = enqueueBatch(B, s, L, st);
+        B.sortedSlot[slot] = enqueueBatch(B, s, L, st);
     } else {
         const ViewBatch::GraphKey key{L.views, s.n, L.opts.tile_w, L.opts.tile_h, L.opts.batch_depth, L.cull,
                                       L.maxTiles, L.totalTiles, L.pairCap, B.generation};
@@ -335,15 +363,26 @@ bool runBatch(ViewBatch& B, const SceneDev& s, const Layout& L, const std::vecto
             it = B.graphs.emplace(key, exec).first;
         }
         hipCheck(hipGraphLaunch(it->second, st), "hipGraphLaunch");
-        sorted = (L.passes & 1) ? B.gaussB.p : B.gaussA.p;
+        B.sortedSlot[slot] = (L.passes & 1) ? B.gaussB.p : B.gaussA.p;
     }
-    hipCheck(hipEventRecord(B.ev[1], st), "hipEventRecord");
-    hipCheck(hipMemcpyAsync(B.ctlHost, B.ctl.p, sizeof(FrameCtl) * L.views, hipMemcpyDeviceToHost, st),
+    hipCheck(hipEventRecord(B.ev[slot][1], st), "hipEventRecord");
+    hipCheck(hipMemcpyAsync(B.ctlHost[slot], B.ctl.p, sizeof(FrameCtl) * L.views, hipMemcpyDeviceToHost, st),
              "hipMemcpyAsync ctl");
-    hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
-    B.sortedGauss = sorted;
-    hipCheck(hipEventElapsedTime(&B.lastMs, B.ev[0], B.ev[1]), "hipEventElapsedTime");
-    B.lastCtl.assign(B.ctlHost, B.ctlHost + L.views);
+    hipCheck(hipEventRecord(B.ev[slot][2], st), "hipEventRecord");
+}
+
+// Waits for the batch submitted through `slot`, makes it the last finished batch (stats, dumps)
+// and returns true if any view's pairs overflowed the arena; the per-view capacity has then been
+// raised for the next layout.
+bool finishBatch(ViewBatch& B, const Layout& L, const std::vector<CamParams>& cps, int slot) {
+    hipCheck(hipEventSynchronize(B.ev[slot][2]), "hipEventSynchronize");
+    hipCheck(hipEventElapsedTime(&B.lastMs, B.ev[slot][0], B.ev[slot][1]), "hipEventElapsedTime");
+    B.views = L.views;
+    B.maxTiles = L.maxTiles;
+    B.tilesPerView.resize(L.views);
+    for (int v = 0; v < L.views; ++v) B.tilesPerView[v] = cps[v].tilesX * cps[v].tilesY;
+    B.sortedGauss = B.sortedSlot[slot];
+    B.lastCtl.assign(B.ctlHost[slot], B.ctlHost[slot] + L.views);
     uint32_t most = 0;
     bool over = false;
     for (const FrameCtl& c : B.lastCtl) {
@@ -352,7 +391,7 @@ bool runBatch(ViewBatch& B, const SceneDev& s, const Layout& L, const std::vecto
     }
     if (over) {
         if (most >= (1u << 30) - 1) throw std::runtime_error("pair count of a view exceeds 2^30");
-        B.pairCap = static_cast<size_t>(most) + most / 4 + 1024;
+        B.pairCap = std::max(B.pairCap, static_cast<size_t>(most) + most / 4 + 1024);
     }
     return over;
 }
@@ -381,11 +420,8 @@ void renderViews(ans_renderer* r, int32_t n_views, const ans_camera* cams, const
         const Layout L = makeLayout(B, r->scene, n_views, cps, o);
         reserveArena(B, L);
         B.pairCap = L.pairCap;
-        B.views = n_views;
-        B.maxTiles = L.maxTiles;
-        B.tilesPerView.resize(n_views);
-        for (int v = 0; v < n_views; ++v) B.tilesPerView[v] = cps[v].tilesX * cps[v].tilesY;
-        if (!runBatch(B, r->scene, L, cps, dev_rgb, static_cast<hipStream_t>(stream))) break;
+        submitBatch(B, r->scene, L, cps, dev_rgb, static_cast<hipStream_t>(stream), 0);
+        if (!finishBatch(B, L, cps, 0)) break;
         ++retries;
     }
     B.lastRetries = retries;
@@ -402,6 +438,147 @@ void renderViews(ans_renderer* r, int32_t n_views, const ans_camera* cams, const
             st.reserved = 0;
         }
     }
+}
+
+// One chunk of a camera path: frames [first, first + count), their constants, and each frame's
+// byte offset in the chunk's output slot.
+struct PathChunk {
+    int64_t first = 0;
+    int count = 0;
+    std::vector<CamParams> cps;
+    std::vector<size_t> offset;
+    size_t bytes = 0;
+};
+
+void ensurePinned(uint8_t*& p, size_t& cap, size_t bytes) {
+    if (bytes <= cap) return;
+    if (p) hipCheck(hipHostFree(p), "hipHostFree");
+    p = nullptr;
+    cap = 0;
+    hipCheck(hipHostMalloc(reinterpret_cast<void**>(&p), bytes, hipHostMallocDefault), "hipHostMalloc");
+    cap = bytes;
+}
+
+// The frame loop of `aniso path` (src/main.cpp:234-244) as a two-deep pipeline of batched
+// launches: chunk c + 1 renders on the path's render stream while chunk c is copied to pinned
+// host memory on the copy stream and handed to the sink, frame by frame, in order.
+void renderPath(ans_renderer* r, int64_t nFrames, const ans_camera* cams, const float* scales,
+                const ans_options* options, int chunk, ans_frame_sink sink, void* user, ans_path_stats* stats) {
+    const auto t0 = std::chrono::steady_clock::now();
+    if (!r) throw std::invalid_argument("renderer is null");
+    if (nFrames < 0) throw std::invalid_argument("n_frames must not be negative");
+    if (chunk < 1 || chunk > kMaxViews) throw std::invalid_argument("chunk must be in 1..256");
+    if (nFrames > 0 && (!cams || !sink)) throw std::invalid_argument("cameras and sink must not be null");
+    const ans_options o = resolveOptions(options);
+
+    // Every frame's constants, validated before any GPU work.
+    std::vector<PathChunk> chunks;
+    uint32_t maxTiles = 1;
+    size_t maxBytes = 0;
+    for (int64_t f = 0; f < nFrames; ++f) {
+        checkCamera(&cams[f]);
+        CamParams cp = makeCamParams(cams[f], o, r->scene.shDegree);
+        if (scales) {
+            if (!(scales[f] > 0.0f) || !std::isfinite(scales[f]))
+                throw std::invalid_argument("splat scales must be positive and finite");
+            cp.splatScale = scales[f];
+        }
+        if (cp.tilesX > 65535 || cp.tilesY > 65535) throw std::invalid_argument("too many tiles");
+        if (chunks.empty() || chunks.back().count == chunk) {
+            chunks.emplace_back();
+            chunks.back().first = f;
+        }
+        PathChunk& k = chunks.back();
+        k.cps.push_back(cp);
+        k.offset.push_back(k.bytes);
+        k.bytes += alignUp(static_cast<size_t>(cp.width) * cp.height * 3, 256);
+        ++k.count;
+        maxTiles = std::max(maxTiles, static_cast<uint32_t>(cp.tilesX) * static_cast<uint32_t>(cp.tilesY));
+        maxBytes = std::max(maxBytes, k.bytes);
+    }
+    ans_path_stats st{};
+    if (chunks.empty()) {
+        if (stats) *stats = st;
+        return;
+    }
+
+    ViewBatch& B = batchOf(r);
+    if (!B.pathRender) {
+        hipCheck(hipStreamCreateWithFlags(&B.pathRender, hipStreamNonBlocking), "hipStreamCreate");
+        hipCheck(hipStreamCreateWithFlags(&B.pathCopy, hipStreamNonBlocking), "hipStreamCreate");
+        for (auto& e : B.copied) hipCheck(hipEventCreate(&e), "hipEventCreate");
+    }
+    // The arena for the largest chunk shape up front: growth frees buffers, so it only happens
+    // below after the render stream has drained.
+    {
+        Layout env = makeLayout(B, r->scene, chunks[0].count, chunks[0].cps, o);
+        env.views = std::min<int>(chunk, static_cast<int>(nFrames));
+        env.maxTiles = maxTiles;
+        env.items = maxTiles * static_cast<uint32_t>(env.views);
+        env.blocksI = std::max<uint32_t>(1, (env.items + kSortBlock * kDepthItems - 1) / (kSortBlock * kDepthItems));
+        reserveArena(B, env);
+        B.pairCap = env.pairCap;
+    }
+    for (int k = 0; k < 2; ++k) {
+        B.pathDev[k].reserve(maxBytes);
+        ensurePinned(B.pathHost[k], B.pathHostCap[k], maxBytes);
+    }
+
+    auto submit = [&](size_t c) {
+        const PathChunk& k = chunks[c];
+        const int slot = static_cast<int>(c & 1);
+        const Layout L = makeLayout(B, r->scene, k.count, k.cps, o);
+        reserveArena(B, L);
+        B.pairCap = L.pairCap;
+        std::vector<uint8_t*> outs(k.count);
+        for (int v = 0; v < k.count; ++v) outs[v] = B.pathDev[slot].p + k.offset[v];
+        submitBatch(B, r->scene, L, k.cps, outs.data(), B.pathRender, slot);
+        ++st.chunks;
+        return L;
+    };
+    auto drain = [&] {
+        (void)hipStreamSynchronize(B.pathRender);
+        (void)hipStreamSynchronize(B.pathCopy);
+    };
+    try {
+        Layout cur = submit(0);
+        for (size_t c = 0; c < chunks.size(); ++c) {
+            const PathChunk& k = chunks[c];
+            const int slot = static_cast<int>(c & 1);
+            // The next chunk's slot was last used by chunk c - 1, handed to the sink already.
+            Layout next{};
+            if (c + 1 < chunks.size()) next = submit(c + 1);
+            bool over = finishBatch(B, cur, k.cps, slot);
+            st.gpu_ms += B.lastMs;
+            while (over) {
+                // Pairs overflowed this chunk's arena: let the chunk in flight finish (growth frees
+                // buffers), grow, and render this chunk again. The chunk in flight is checked on
+                // its own turn and rendered again there if it overflowed too.
+                hipCheck(hipStreamSynchronize(B.pathRender), "hipStreamSynchronize");
+                const Layout L = submit(c);
+                over = finishBatch(B, L, k.cps, slot);
+                st.gpu_ms += B.lastMs;
+            }
+            hipCheck(hipMemcpyAsync(B.pathHost[slot], B.pathDev[slot].p, k.bytes, hipMemcpyDeviceToHost, B.pathCopy),
+                     "hipMemcpyAsync frames");
+            hipCheck(hipEventRecord(B.copied[slot], B.pathCopy), "hipEventRecord");
+            hipCheck(hipEventSynchronize(B.copied[slot]), "hipEventSynchronize");
+            for (int v = 0; v < k.count; ++v) {
+                const CamParams& cp = k.cps[v];
+                const int64_t f = k.first + v;
+                const int rc = sink(user, f, B.pathHost[slot] + k.offset[v], static_cast<size_t>(cp.width) * cp.height * 3);
+                if (rc != 0)
+                    throw std::invalid_argument("sink returned " + std::to_string(rc) + " at frame " + std::to_string(f));
+                ++st.frames;
+            }
+            cur = next;
+        }
+    } catch (...) {
+        drain();
+        throw;
+    }
+    st.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (stats) *stats = st;
 }
 
 }  // namespace
@@ -450,6 +627,15 @@ int ans_views_autotune(ans_renderer* r, int32_t n_views, const ans_camera* cams,
         }
         *best = bestOpt;
         if (best_ms) *best_ms = bestMs;
+        return ANS_OK;
+    });
+}
+
+int ans_render_path(ans_renderer* r, int64_t n_frames, const ans_camera* cams, const float* splat_scales,
+                    const ans_options* options, int32_t chunk, ans_frame_sink sink, void* user,
+                    ans_path_stats* stats) {
+    return guarded([&] {
+        renderPath(r, n_frames, cams, splat_scales, options, chunk, sink, user, stats);
         return ANS_OK;
     });
 }

@@ -24,7 +24,8 @@ static __global__ __launch_bounds__(256) void k_initViews(FrameCtl* __restrict__
 
 // Preprocess of every view, one lane per Gaussian: its parameters and SH (planar on the device,
 // [coefficient * 3 + channel][n]) are loaded once into registers, its view-independent covariance
-// is formed once (prepareGaussian), and it is projected into each view in turn (cameraTransform,
+// is formed once per run of views with one splat scale (prepareGaussian), and it is projected
+// into each view in turn (cameraTransform,
 // finishProjection): the arithmetic of the single-view k_preprocess. Outputs for view v at
 // v * n + i.
 static __global__ __launch_bounds__(256) void k_preprocessViews(
@@ -40,8 +41,15 @@ static __global__ __launch_bounds__(256) void k_preprocessViews(
 #pragma unroll
     for (int j = 0; j < 48; ++j) shr[j] = j < K3 ? sh[static_cast<size_t>(j) * n + i] : 0.0f;
     GaussProj g;
-    prepareGaussian<false>(rot[i], reinterpret_cast<const F3*>(scale)[i], cams[0].splatScale, g);
+    float splatScale = cams[0].splatScale;
+    prepareGaussian<false>(rot[i], reinterpret_cast<const F3*>(scale)[i], splatScale, g);
     for (int v = 0; v < views; ++v) {
+        // A view with another splat scale (a reveal along a camera path) forms the covariance
+        // again; the branch is uniform across the wavefront.
+        if (cams[v].splatScale != splatScale) {
+            splatScale = cams[v].splatScale;
+            prepareGaussian<false>(rot[i], reinterpret_cast<const F3*>(scale)[i], splatScale, g);
+        }
         const size_t o = static_cast<size_t>(v) * n + i;
         if (!cameraTransform(m, cams[v], g) || !finishProjection<false, false>(m, op, shr, 1, cams[v], g)) {
             storeCulled(o, rec, tilesTouched, depthKey);

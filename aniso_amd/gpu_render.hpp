@@ -19,6 +19,7 @@
 #include "aniso_amd.h"
 
 #include <cstddef>
+#include <exception>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -53,16 +54,7 @@ public:
 
     // Timings in stats are GPU time per stage, measured with HIP events.
     Image render(const Camera& camera, RenderStats* stats = nullptr, const RenderOptions& options = {}) {
-        ans_camera c{};
-        for (int r = 0; r < 3; ++r)
-            for (int k = 0; k < 3; ++k) c.R[3 * r + k] = camera.R[r][k];
-        for (int k = 0; k < 3; ++k) c.t[k] = camera.t[k];
-        c.fx = camera.fx;
-        c.fy = camera.fy;
-        c.cx = camera.cx;
-        c.cy = camera.cy;
-        c.width = camera.width;
-        c.height = camera.height;
+        const ans_camera c = toC(camera);
         ans_options o{};
         o.sh_degree = options.shDegree;
         o.splat_scale = options.splatScale;  // tile shape and batch depth: the library defaults
@@ -82,7 +74,57 @@ public:
     // True if a HIP device is present and usable.
     static bool available() { return ans_available() != 0; }
 
+    // Camera-path streaming, an addition to the reference class for the frame loop of
+    // `aniso path` (src/main.cpp:234-244): renders the cameras in batched chunks, two chunks in
+    // flight, and calls sink(index, rgb, bytes) for every frame in order, rgb pointing at RGB8
+    // rows top to bottom that stay valid until sink returns. splatScales: empty (the options'
+    // scale throughout) or one per camera. A sink that throws stops the path; the exception
+    // reaches the caller after the GPU work has drained.
+    template <class Sink>
+    void renderPath(const std::vector<Camera>& cameras, const std::vector<float>& splatScales,
+                    const RenderOptions& options, Sink&& sink, int chunk = 16) {
+        if (!splatScales.empty() && splatScales.size() != cameras.size())
+            throw std::runtime_error("aniso_amd: one splat scale per camera");
+        std::vector<ans_camera> cams(cameras.size());
+        for (std::size_t i = 0; i < cameras.size(); ++i) cams[i] = toC(cameras[i]);
+        ans_options o{};
+        o.sh_degree = options.shDegree;
+        o.splat_scale = options.splatScale;
+        struct Ctx {
+            Sink* sink;
+            std::exception_ptr error;
+        } ctx{&sink, nullptr};
+        auto trampoline = [](void* user, int64_t frame, const uint8_t* rgb, size_t bytes) -> int {
+            Ctx* c = static_cast<Ctx*>(user);
+            try {
+                (*c->sink)(static_cast<std::size_t>(frame), rgb, bytes);
+                return 0;
+            } catch (...) {
+                c->error = std::current_exception();
+                return 1;
+            }
+        };
+        const int status = ans_render_path(handle_, static_cast<int64_t>(cams.size()), cams.data(),
+                                           splatScales.empty() ? nullptr : splatScales.data(), &o, chunk,
+                                           trampoline, &ctx, nullptr);
+        if (ctx.error) std::rethrow_exception(ctx.error);
+        check(status);
+    }
+
 private:
+    static ans_camera toC(const Camera& camera) {
+        ans_camera c{};
+        for (int r = 0; r < 3; ++r)
+            for (int k = 0; k < 3; ++k) c.R[3 * r + k] = camera.R[r][k];
+        for (int k = 0; k < 3; ++k) c.t[k] = camera.t[k];
+        c.fx = camera.fx;
+        c.fy = camera.fy;
+        c.cx = camera.cx;
+        c.cy = camera.cy;
+        c.width = camera.width;
+        c.height = camera.height;
+        return c;
+    }
     static void check(int status) {
         if (status != ANS_OK) throw std::runtime_error(std::string("aniso_amd: ") + ans_last_error());
     }

@@ -54,6 +54,16 @@ class RenderStats:
     retries: int = 0
 
 
+@dataclasses.dataclass
+class PathStats:
+    """ans_path_stats: frames handed to the sink, batched launches, their GPU time, wall time."""
+
+    frames: int = 0
+    chunks: int = 0
+    gpu_ms: float = 0.0
+    total_ms: float = 0.0
+
+
 def camera_c(camera: Camera) -> _lib.Camera_:
     c = _lib.Camera_()
     for i, v in enumerate(np.asarray(camera.R, np.float64).reshape(9)):
@@ -156,6 +166,50 @@ class GpuRenderer:
                                                   _torch_stream(), int(repeats), ctypes.byref(best),
                                                   ctypes.byref(ms)))
         return RenderOptions.from_c(best), ms.value
+
+    def render_path(self, cameras, sink, options: RenderOptions | None = None, chunk: int = 16,
+                    splat_scales=None) -> "PathStats":
+        """Renders a camera path (ans_render_path): chunks of `chunk` views in batched launches, two
+        chunks in flight, every frame handed to `sink` in order as it reaches host memory.
+
+        sink: a callable sink(frame_index, rgb) with rgb a uint8 [H, W, 3] array that is only
+        valid during the call (copy it to keep it), or a binary file object (each frame is
+        written, as `aniso path` writes raw RGB to stdout, src/main.cpp:243). A callable that
+        returns False stops the path (ValueError); an exception it raises stops the path and is
+        raised again here. splat_scales: None (options.splat_scale throughout) or one scale per
+        frame, the reference path file's optional tenth column."""
+        k = len(cameras)
+        cams = (_lib.Camera_ * max(k, 1))(*[camera_c(c) for c in cameras])
+        sizes = [(c.height, c.width) for c in cameras]
+        scales = None
+        if splat_scales is not None:
+            scales = np.ascontiguousarray(splat_scales, np.float32)
+            if scales.shape != (k,):
+                raise ValueError("one splat scale per camera")
+        write = getattr(sink, "write", None)
+        failure = []
+
+        def deliver(_user, frame, rgb, nbytes):
+            try:
+                buf = np.ctypeslib.as_array(rgb, shape=(nbytes,))
+                if write is not None:
+                    write(buf.tobytes())
+                    return 0
+                h, w = sizes[frame]
+                return 1 if sink(frame, buf.reshape(h, w, 3)) is False else 0
+            except BaseException as e:  # handed back to the caller after the library drains
+                failure.append(e)
+                return 1
+
+        cb = _lib.FrameSink(deliver)
+        st = _lib.PathStats_()
+        opts = (options or RenderOptions()).c()
+        rc = _lib.load().ans_render_path(self._h, k, cams, None if scales is None else _ptr(scales),
+                                         ctypes.byref(opts), int(chunk), cb, None, ctypes.byref(st))
+        if failure:
+            raise failure[0]
+        _lib.check(rc)
+        return PathStats(st.frames, st.chunks, st.gpu_ms, st.total_ms)
 
     def views_dump(self, view: int):
         """Lists of one view of the last batched render: dict with visible, pairs, values u32[P],

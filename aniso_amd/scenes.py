@@ -244,6 +244,22 @@ def view_batch_cameras(count: int = 16, seed: int = 1, hfov_deg: float = 60.0):
     return cams
 
 
+def flythrough_cameras(count: int, width: int = 1920, height: int = 1080, hfov_deg: float = 70.0):
+    """A smooth camera path of `count` frames through the synthetic scenes, the kind of path
+    `aniso path` renders for a fly-through video (tools/flythrough.py: 1920x1080, 70 degrees):
+    the camera dollies from (-0.4, 0, -0.4) to (0.4, 0.1, 0.4) while its yaw sweeps from -25 to
+    25 degrees and back, eased at both ends."""
+    cams = []
+    for k in range(count):
+        t = k / max(count - 1, 1)
+        e = 0.5 - 0.5 * math.cos(math.pi * t)  # ease in and out
+        pos = (-0.4 + 0.8 * e, 0.1 * e, -0.4 + 0.8 * e)
+        yaw = math.radians(-25.0 * math.cos(2.0 * math.pi * t))
+        fwd = (math.sin(yaw), 0.0, math.cos(yaw))
+        cams.append(pose_camera(pos, fwd, (0.0, -1.0, 0.0), width, height, hfov_deg))
+    return cams
+
+
 # The benchmark workloads, by what they are.
 WORKLOADS = {
     "synthetic-200k-640x360": dict(n=200_000, width=640, height=360, clustered=False),

@@ -127,6 +127,7 @@ def main():
         extra["fwd_clustered_2.2M_1080p"] = bench_forward("clustered-2.2M-1920x1080", max(10, args.steps // 2),
                                                          args.warmup)
         extra["batched_16_views"] = bench_views("batched-1M-16views", max(5, args.steps // 10), args.warmup)
+        extra["path_stream"] = bench_path(raw, 128)
 
     cpu = None
     if not args.skip_cpu:
@@ -293,6 +294,40 @@ def bench_views(workload, steps, warmup):
             "speedup_vs_pipelined": round(plain_ms / batch_ms, 3),
             "autotuned": {"tile": f"{best.tile[0]}x{best.tile[1]}", "batch_depth": best.batch_depth,
                           "gpu_ms": round(tuned_ms, 4)}, "steps": steps}
+
+
+def bench_path(raw, frames):
+    """Frames/s of camera-path streaming: a `frames`-frame fly-through at 1920x1080, 70 degrees
+    (scenes.flythrough_cameras, the size and field of view of tools/flythrough.py) of the
+    headline scene, every RGB8 frame delivered in order into host memory. ans_render_path
+    (chunks of 16 views in batched launches, two chunks in flight, the copy of one overlapping the
+    render of the next) against the reference's loop (src/main.cpp:234-244): one synchronous
+    ans_render per frame returning the host image."""
+    from aniso_amd import _lib, scenes
+    from aniso_amd.renderer import GpuRenderer, RenderOptions, camera_c
+
+    cams = scenes.flythrough_cameras(frames)
+    L = _lib.load()
+    with GpuRenderer(raw) as r:
+        opts = RenderOptions()
+        r.render_path(cams[:32], lambda i, rgb: None, opts, chunk=16)  # graphs captured, buffers sized
+        st = r.render_path(cams, lambda i, rgb: None, opts, chunk=16)
+        path_ms = st.total_ms
+        img = np.empty((cams[0].height, cams[0].width, 3), np.uint8)
+        c_opt = opts.c()
+        c_cams = [camera_c(c) for c in cams]
+        _lib.check(L.ans_render(r._h, c_cams[0], c_opt, img.ctypes.data_as(ctypes.c_void_p), img.size, None))
+        t0 = time.perf_counter()
+        for c in c_cams:
+            _lib.check(L.ans_render(r._h, c, c_opt, img.ctypes.data_as(ctypes.c_void_p), img.size, None))
+        loop_ms = (time.perf_counter() - t0) * 1e3
+    return {"frames": frames, "size": f"{cams[0].width}x{cams[0].height}", "gaussians": raw.n, "chunk": 16,
+            "path_frames_per_s": round(frames / (path_ms * 1e-3), 1), "path_ms_per_frame": round(path_ms / frames, 4),
+            "path_gpu_ms_per_frame": round(st.gpu_ms / frames, 4),
+            "per_frame_loop_frames_per_s": round(frames / (loop_ms * 1e-3), 1),
+            "per_frame_loop_ms_per_frame": round(loop_ms / frames, 4),
+            "speedup": round(loop_ms / path_ms, 3),
+            "includes": "every frame's RGB8 image in host memory, in order (pinned, PCIe)"}
 
 
 def bench_fwd_bwd(raw, cam, steps, warmup):

@@ -148,6 +148,27 @@ int ans_views_autotune(ans_renderer* r, int32_t n_views, const ans_camera* cams,
 int ans_views_frame_counts(ans_renderer* r, int32_t view, int64_t* visible, int64_t* pairs, int64_t* tiles);
 int ans_views_frame_dump(ans_renderer* r, int32_t view, uint32_t* values, uint32_t* ranges);
 
+/* ---- Camera-path streaming. Replaces the frame loop of the reference's `aniso path` verb
+ * (src/main.cpp:234-244: one GpuRenderer::render per pose, each RGB8 frame fwrite to stdout).
+ * Renders n_frames cameras in chunks of `chunk` views (1..256) with the batched launch above and
+ * hands every frame, in order, to `sink` as RGB8 rows top to bottom (width * height * 3 bytes in
+ * pinned host memory, valid until sink returns). Two chunks are in flight: while the host runs
+ * sink on chunk k, chunk k + 1 renders and the copy of chunk k to the host overlaps it.
+ * splat_scales: NULL (options->splat_scale for every frame) or n_frames positive scales, the
+ * reference's optional tenth path column (main.cpp:219, :237). Each frame equals ans_render of
+ * its camera with its splat scale. sink returns 0 to continue; nonzero stops the path and the
+ * call returns ANS_ERR_ARGUMENT after its GPU work has drained. stats may be NULL. */
+typedef int (*ans_frame_sink)(void* user, int64_t frame, const uint8_t* rgb, size_t bytes);
+typedef struct ans_path_stats {
+    int64_t frames;    /* frames handed to sink */
+    int64_t chunks;    /* batched launches, re-renders after pair-arena growth included */
+    double gpu_ms;     /* GPU time of the batched launches */
+    double total_ms;   /* wall time of the call */
+} ans_path_stats;
+int ans_render_path(ans_renderer* r, int64_t n_frames, const ans_camera* cams, const float* splat_scales,
+                    const ans_options* options, int32_t chunk, ans_frame_sink sink, void* user,
+                    ans_path_stats* stats);
+
 /* ---- Differentiable path (no counterpart in the reference, whose training calls a
  * third-party rasterizer at pipeline/train_gsplat.py:96-102; this is what a
  * torch.autograd.Function binds in its place). All pointers below are DEVICE pointers to

@@ -1,6 +1,6 @@
 """The header-only C++ drop-in (aniso_amd/gpu_render.hpp) compiles against the reference's own
 headers and links against libaniso_amd.so, with every call the reference's callers make
-(constructor, render with stats and options, available). Runs where the reference tree is present
+(constructor, render with stats and options, available) and the path-streaming addition. Runs where the reference tree is present
 (it is read for its headers only) and g++ exists; it does not touch a GPU."""
 import os
 import shutil
@@ -24,6 +24,10 @@ int main(int argc, char**) {
         options.shDegree = 1;
         aniso::Image image = renderer.render(camera, &stats, options);
         std::printf("%d\n", image.width);
+        std::vector<aniso::Camera> path(3, camera);
+        renderer.renderPath(path, {}, options, [](std::size_t, const uint8_t* rgb, std::size_t bytes) {
+            std::fwrite(rgb, 1, bytes, stdout);
+        });
     }
     std::printf("available=%d\n", aniso::GpuRenderer::available() ? 1 : 0);
     return 0;
