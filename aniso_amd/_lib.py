@@ -63,6 +63,12 @@ SIGNATURES = {
     "ans_renderer_create": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, P, P, P, P, P, ctypes.POINTER(P)]),
     "ans_renderer_create_raw": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, P, P, P, P, P, ctypes.POINTER(P)]),
     "ans_renderer_destroy": (None, [P]),
+    "ans_ply_inspect": (ctypes.c_int, [P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int64),
+                                       ctypes.POINTER(ctypes.c_int32)]),
+    "ans_renderer_create_ply": (ctypes.c_int, [P, ctypes.c_size_t, ctypes.POINTER(P)]),
+    "ans_renderer_load_ply": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(P)]),
+    "ans_renderer_scene": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32), P, P,
+                                          P, P, P]),
     "ans_render": (ctypes.c_int, [P, ctypes.POINTER(Camera_), ctypes.POINTER(Options_), P, ctypes.c_size_t,
                                   ctypes.POINTER(Stats_)]),
     "ans_render_float": (ctypes.c_int, [P, ctypes.POINTER(Camera_), ctypes.POINTER(Options_), P, P,
@@ -91,6 +97,7 @@ SIGNATURES = {
     "ans_diff_screen_grads": (ctypes.c_int, [P, ctypes.c_int64, P, P]),
     "ans_diff_frame_counts": (ctypes.c_int, [P, ctypes.c_int64, P, P, P]),
     "ans_diff_frame_dump": (ctypes.c_int, [P, ctypes.c_int64, P, P]),
+    "ans_ply_write": (ctypes.c_int, [ctypes.POINTER(Params_), P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), P]),
 }
 
 _lib = None

@@ -207,3 +207,29 @@ def forward_backward(means, log_scales, quats, opacity_logits, sh, camera: Camer
     finally:
         L.ans_diff_release(handle, fid.value)
     return image, grads, splat
+
+
+def save_ply(means, log_scales, quats, opacity_logits, sh, path=None) -> bytes:
+    """The raw parameters (float32 CUDA tensors, as for `rasterize`) as a 3DGS PLY file image,
+    the layout of the training pipeline's writer (pipeline/train_gsplat.py:109-128); the rows are
+    interleaved on the device (ans_ply_write) on torch's current stream. Writes `path` if given
+    and returns the bytes."""
+    n = means.shape[0]
+    K = sh.shape[1]
+    for t, name, shape in ((means, "means", (n, 3)), (log_scales, "log_scales", (n, 3)), (quats, "quats", (n, 4)),
+                           (opacity_logits, "opacity_logits", (n,)), (sh, "sh", (n, K, 3))):
+        _check(t, name, shape)
+    tensors = [t.contiguous() for t in (means, log_scales, quats, opacity_logits, sh)]
+    p = _params(*tensors)
+    size = ctypes.c_size_t()
+    L = _lib.load()
+    _lib.check(L.ans_ply_write(ctypes.byref(p), None, 0, ctypes.byref(size), None))
+    out = bytearray(size.value)
+    buf = (ctypes.c_char * len(out)).from_buffer(out)
+    with torch.cuda.device(means.device):
+        _lib.check(L.ans_ply_write(ctypes.byref(p), buf, len(out), ctypes.byref(size), _stream()))
+    data = bytes(out)
+    if path is not None:
+        with open(path, "wb") as f:
+            f.write(data)
+    return data

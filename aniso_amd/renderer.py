@@ -12,6 +12,7 @@ from __future__ import annotations
 
 import ctypes
 import dataclasses
+import os
 
 import numpy as np
 
@@ -113,6 +114,36 @@ class GpuRenderer:
         self._h = h
         self.n = n
         self.sh_degree = degree
+
+    @classmethod
+    def from_ply(cls, source) -> "GpuRenderer":
+        """A renderer from a 3DGS PLY: a path (ans_renderer_load_ply, streamed from the file) or
+        the file's bytes (ans_renderer_create_ply). The header is read on the host with the
+        reference loader's rules; activation and the f_rest reorder run on the device."""
+        L = _lib.load()
+        h = ctypes.c_void_p()
+        if isinstance(source, (bytes, bytearray, memoryview)):
+            buf = np.frombuffer(source, np.uint8)
+            _lib.check(L.ans_renderer_create_ply(_ptr(buf), buf.size, ctypes.byref(h)))
+        else:
+            _lib.check(L.ans_renderer_load_ply(os.fsencode(source), ctypes.byref(h)))
+        self = cls.__new__(cls)
+        self._h = h
+        n, degree = ctypes.c_int64(), ctypes.c_int32()
+        _lib.check(L.ans_renderer_scene(h, ctypes.byref(n), ctypes.byref(degree), None, None, None, None, None))
+        self.n, self.sh_degree = n.value, degree.value
+        return self
+
+    def scene_arrays(self):
+        """The device scene copied back, activated: dict of pos [n,3], scale [n,3], rot [n,4]
+        (w, x, y, z), opacity [n], sh [n, (degree+1)^2, 3]."""
+        k = (self.sh_degree + 1) ** 2
+        out = dict(pos=np.empty((self.n, 3), np.float32), scale=np.empty((self.n, 3), np.float32),
+                   rot=np.empty((self.n, 4), np.float32), opacity=np.empty(self.n, np.float32),
+                   sh=np.empty((self.n, k, 3), np.float32))
+        _lib.check(_lib.load().ans_renderer_scene(self._h, None, None, *[_ptr(out[key]) for key in
+                                                                          ("pos", "scale", "rot", "opacity", "sh")]))
+        return out
 
     @staticmethod
     def available() -> bool:
@@ -255,6 +286,15 @@ class GpuRenderer:
 
     def __exit__(self, *exc):
         self.close()
+
+
+def ply_inspect(data) -> tuple:
+    """(Gaussian count, SH degree) of a PLY file image, from its header alone (ans_ply_inspect;
+    no GPU). Malformed headers raise ValueError with the reference loader's message."""
+    buf = np.frombuffer(data, np.uint8)
+    n, degree = ctypes.c_int64(), ctypes.c_int32()
+    _lib.check(_lib.load().ans_ply_inspect(_ptr(buf), buf.size, ctypes.byref(n), ctypes.byref(degree)))
+    return n.value, degree.value
 
 
 def activate(scene: RawScene):

@@ -128,10 +128,15 @@ def main():
                                                          args.warmup)
         extra["batched_16_views"] = bench_views("batched-1M-16views", max(5, args.steps // 10), args.warmup)
         extra["path_stream"] = bench_path(raw, 128)
+        extra["ply_ingest"] = bench_ingest(raw)
 
     cpu = None
     if not args.skip_cpu:
         cpu = cpu_baseline(raw, cam, args.cpu_seconds)
+
+    if cpu and "ply_loader" in cpu and "ply_ingest" in extra:
+        extra["ply_ingest"]["speedup_vs_reference_loader"] = round(
+            cpu["ply_loader"]["value"] / extra["ply_ingest"]["load_ply_ms"], 2)
 
     out = {
         "metric": METRIC,
@@ -330,6 +335,42 @@ def bench_path(raw, frames):
             "includes": "every frame's RGB8 image in host memory, in order (pinned, PCIe)"}
 
 
+def bench_ingest(raw):
+    """Scene ingest from a 3DGS PLY file of the headline scene (written to a temporary file, so it
+    is in the page cache): ans_renderer_load_ply (header on the host, the vertex block streamed
+    through pinned memory, activation and reorder on the device) and ans_renderer_create_ply
+    (from the file's bytes in memory), best of 3 each. The reference loader's time on the same
+    file is part of the CPU baseline (cpu_baseline.ply_loader)."""
+    import tempfile
+
+    from aniso_amd import scenes
+    from aniso_amd.renderer import GpuRenderer
+
+    fd, path = tempfile.mkstemp(suffix=".ply")
+    os.close(fd)
+    try:
+        scenes.write_ply(path, raw)
+        size = os.path.getsize(path)
+        data = open(path, "rb").read()
+
+        def best(fn, k=3):
+            out = 1e30
+            for _ in range(k):
+                t0 = time.perf_counter()
+                r = fn()
+                out = min(out, (time.perf_counter() - t0) * 1e3)
+                r.close()
+            return out
+
+        load_ms = best(lambda: GpuRenderer.from_ply(path))
+        bytes_ms = best(lambda: GpuRenderer.from_ply(data))
+    finally:
+        os.remove(path)
+    return {"gaussians": raw.n, "file_bytes": size, "load_ply_ms": round(load_ms, 2),
+            "load_ply_GBs": round(size / (load_ms * 1e-3) / 1e9, 2), "create_ply_from_bytes_ms": round(bytes_ms, 2),
+            "includes": "file read (page cache) or bytes in host memory -> renderer ready on the device"}
+
+
 def bench_fwd_bwd(raw, cam, steps, warmup):
     """One differentiable training step on the headline scene: forward (unclamped float image), L1
     against a target rendered after perturbing means by N(0, 0.01^2) and SH DC by N(0, 0.05^2),
@@ -390,8 +431,25 @@ def cpu_baseline(raw, cam, seconds):
         t = time.perf_counter()
         fn()
         times.append((time.perf_counter() - t) * 1e3)
-    return {"value": round(statistics.median(times), 2), "unit": "ms/frame", "cores": os.cpu_count(),
-            "kind": kind, "sample": f"{len(times)} frames of the same workload after 1 warm-up, median"}
+    out = {"value": round(statistics.median(times), 2), "unit": "ms/frame", "cores": os.cpu_count(),
+           "kind": kind, "sample": f"{len(times)} frames of the same workload after 1 warm-up, median"}
+    if oracle.ref_available():
+        # The reference's PLY loader (aniso::loadPly, one thread) on the scene's file, beside the
+        # device ingest leg.
+        import tempfile
+
+        from aniso_amd import scenes
+
+        fd, path = tempfile.mkstemp(suffix=".ply")
+        os.close(fd)
+        try:
+            scenes.write_ply(path, raw)
+            ms = min(oracle.ref_load_ply(path)[1] for _ in range(2))
+        finally:
+            os.remove(path)
+        out["ply_loader"] = {"value": round(ms, 2), "unit": "ms/file", "cores": 1, "kind": "reference",
+                             "sample": f"aniso::loadPly of the {raw.n}-Gaussian SH3 PLY, best of 2"}
+    return out
 
 
 if __name__ == "__main__":

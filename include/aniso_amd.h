@@ -88,6 +88,31 @@ int ans_renderer_create_raw(int64_t n, int32_t sh_degree, const float* pos_xyz, 
 
 void ans_renderer_destroy(ans_renderer* r);
 
+/* ---- Scene ingest on the device. The reference loads a 3DGS PLY on the CPU
+ * (aniso::loadPly, src/ply_loader.cpp:77-141: parse, activate, reorder f_rest) and then
+ * uploads (gpu_render.cu:267-292). Here the host reads only the header, with the same rules and
+ * messages (ply_loader.cpp:20-66, :86-110: binary_little_endian, float properties, columns by
+ * name, SH degree from the f_rest count, length check); the vertex block goes to the device as
+ * it is and one kernel activates it (exp, sigmoid, q/|q| in FP32, bit-identical to the
+ * reference loader), drops the normals and writes the renderer's layout. Degree-4 files are
+ * accepted and keep coefficients up to degree 3, the most either renderer evaluates
+ * (sh.cpp:18-58). Malformed files: ANS_ERR_ARGUMENT with the reference loader's message. */
+
+/* Header only, no GPU needed: the Gaussian count and SH degree of the file image data[bytes]. */
+int ans_ply_inspect(const void* data, size_t bytes, int64_t* n, int32_t* sh_degree);
+
+/* A renderer from a PLY file image in host memory (data[bytes], the whole file). */
+int ans_renderer_create_ply(const void* data, size_t bytes, ans_renderer** out);
+
+/* A renderer from a PLY file (loadPly(path), ply_loader.h:27): the vertex block is read in
+ * chunks into pinned memory, each chunk's copy to the device overlapping the read of the next. */
+int ans_renderer_load_ply(const char* path, ans_renderer** out);
+
+/* The handle's scene copied back to host arrays in ans_renderer_create's layout (any array may
+ * be NULL: call with all NULL to get n and sh_degree first). */
+int ans_renderer_scene(ans_renderer* r, int64_t* n, int32_t* sh_degree, float* pos_xyz, float* scale_xyz,
+                       float* rot_wxyz, float* opacity, float* sh);
+
 /* Renders one frame into rgb_out (width * height * 3 bytes, RGB8, rows top to bottom), like
  * GpuRenderer::render (gpu_render.cu:299-375). options and stats may be NULL. An empty view
  * (nothing visible) gives a black image and status 0. */
@@ -236,6 +261,14 @@ int ans_diff_screen_grads(ans_diff* d, int64_t frame, float* grad_uv, void* stre
  * CPU reference evaluated on the same lists: values [pairs], ranges [tiles][2]. */
 int ans_diff_frame_counts(ans_diff* d, int64_t frame, int64_t* visible, int64_t* pairs, int64_t* tiles);
 int ans_diff_frame_dump(ans_diff* d, int64_t frame, uint32_t* values, uint32_t* ranges);
+
+/* The raw parameters (device arrays, as for ans_diff_forward) as a 3DGS PLY file image in host
+ * memory, the training pipeline's writer (pipeline/train_gsplat.py:109-128): header, then per
+ * Gaussian x y z, nx ny nz = 0, f_dc_0..2, f_rest channel-major, opacity logit, log-scales,
+ * quaternion / max(|q|, 1e-12). out NULL: *written = the file size, nothing else done. Otherwise
+ * out must hold capacity >= that size; the rows are interleaved by a kernel on `stream` and the
+ * call waits for them. */
+int ans_ply_write(const ans_params* params, void* out, size_t capacity, size_t* written, void* stream);
 
 /* Host activation used by ans_renderer_create_raw, exposed for parity tests. */
 void ans_activate(int64_t n, const float* log_scale_xyz, const float* opacity_logit, const float* quat_wxyz,

@@ -94,6 +94,10 @@ def ref_lib():
         R.ref_render.restype = ctypes.c_int
         R.ref_render.argtypes = [ctypes.c_int64, ctypes.c_int, P, P, P, P, P, ctypes.POINTER(_Camera),
                                  ctypes.c_int, ctypes.c_float, P, ctypes.POINTER(_Stats)]
+        R.ref_load_ply.restype = P
+        R.ref_load_ply.argtypes = [ctypes.c_char_p, P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_double)]
+        R.ref_scene_arrays.argtypes = [P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int), P, P, P, P, P]
+        R.ref_scene_free.argtypes = [P]
         _ref = R
     return _ref
 
@@ -239,6 +243,35 @@ def ref_render(scene: ActivatedScene, camera, sh_degree=3, splat_scale=1.0):
     if rc != 0:
         raise RuntimeError("reference render failed")
     return out, st
+
+
+class RefPlyError(ValueError):
+    """aniso::PlyError raised by the compiled reference loader; the message is its what()."""
+
+
+def ref_load_ply(path):
+    """Loads a PLY with the compiled reference loader (aniso::loadPly). Returns (arrays, ms):
+    arrays = dict(pos, scale, rot (w, x, y, z), opacity, sh [n, coeffs, 3], sh_degree) and the
+    load's wall time in ms. Raises RefPlyError with the loader's message."""
+    R = ref_lib()
+    err = ctypes.create_string_buffer(1024)
+    ms = ctypes.c_double()
+    h = R.ref_load_ply(os.fsencode(path), err, len(err), ctypes.byref(ms))
+    if not h:
+        raise RefPlyError(err.value.decode(errors="replace"))
+    try:
+        n, degree = ctypes.c_int64(), ctypes.c_int()
+        R.ref_scene_arrays(h, ctypes.byref(n), ctypes.byref(degree), None, None, None, None, None)
+        n, degree = n.value, degree.value
+        k = (degree + 1) ** 2
+        out = dict(pos=np.empty((n, 3), np.float32), scale=np.empty((n, 3), np.float32),
+                   rot=np.empty((n, 4), np.float32), opacity=np.empty(n, np.float32),
+                   sh=np.empty((n, k, 3), np.float32))
+        R.ref_scene_arrays(h, None, None, *[_ptr(out[key]) for key in ("pos", "scale", "rot", "opacity", "sh")])
+        out["sh_degree"] = degree
+        return out, ms.value
+    finally:
+        R.ref_scene_free(h)
 
 
 # --- FP64 gradients (grad.cpp) ------------------------------------------------------------------

@@ -1,11 +1,15 @@
-// Drives the reference CPU renderer, compiled from its own sources (src/render.cpp, src/sh.cpp,
-// src/camera.cpp), through a C entry point the tests can call. TEST INFRASTRUCTURE ONLY: the
+// Drives the reference CPU renderer and PLY loader, compiled from their own sources
+// (src/render.cpp, src/sh.cpp, src/camera.cpp, src/ply_loader.cpp), through C entry points the
+// tests can call. TEST INFRASTRUCTURE ONLY: the
 // library built from this file is the compiled reference, used to validate the oracle's
 // restatement and as the CPU baseline; the product path never loads it.
+#include "aniso/ply_loader.h"
 #include "aniso/render.h"
 
 #include <cstdint>
+#include <chrono>
 #include <cstring>
+#include <exception>
 
 extern "C" {
 
@@ -62,5 +66,43 @@ int ref_render(int64_t n, int scene_sh_degree, const float* pos, const float* sc
     }
     return 0;
 }
+
+// Loads a PLY with aniso::loadPly (ply_loader.h:27). Returns the scene (free with
+// ref_scene_free) or NULL with the loader's message in err[errCap]; *ms = the load's wall time.
+void* ref_load_ply(const char* path, char* err, size_t errCap, double* ms) {
+    try {
+        const auto t0 = std::chrono::steady_clock::now();
+        auto* s = new aniso::Scene(aniso::loadPly(path));
+        if (ms) *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        return s;
+    } catch (const std::exception& e) {
+        if (err && errCap) {
+            std::strncpy(err, e.what(), errCap - 1);
+            err[errCap - 1] = 0;
+        }
+        return nullptr;
+    }
+}
+
+// Size and arrays of a loaded scene (any array may be NULL): pos/scale [n][3], rot [n][4]
+// (w, x, y, z), opacity [n], sh [n][coeffs][3].
+void ref_scene_arrays(const void* scene, int64_t* n, int* sh_degree, float* pos, float* scale, float* rot,
+                      float* opacity, float* sh) {
+    const auto& s = *static_cast<const aniso::Scene*>(scene);
+    if (n) *n = static_cast<int64_t>(s.size());
+    if (sh_degree) *sh_degree = s.shDegree;
+    for (size_t i = 0; i < s.size(); ++i) {
+        const aniso::Gaussian& g = s.gaussians[i];
+        if (pos) pos[3 * i] = g.position.x, pos[3 * i + 1] = g.position.y, pos[3 * i + 2] = g.position.z;
+        if (scale) scale[3 * i] = g.scale.x, scale[3 * i + 1] = g.scale.y, scale[3 * i + 2] = g.scale.z;
+        if (rot)
+            rot[4 * i] = g.rotation.w, rot[4 * i + 1] = g.rotation.x, rot[4 * i + 2] = g.rotation.y,
+            rot[4 * i + 3] = g.rotation.z;
+        if (opacity) opacity[i] = g.opacity;
+    }
+    if (sh) std::memcpy(sh, s.sh.data(), s.sh.size() * sizeof(float));
+}
+
+void ref_scene_free(void* scene) { delete static_cast<aniso::Scene*>(scene); }
 
 }  // extern "C"
