@@ -16,10 +16,16 @@
 #include "engine.hpp"
 #include "ply.hpp"
 
+#include <fcntl.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <algorithm>
-#include <cstdio>
+#include <atomic>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace ans {
@@ -171,77 +177,111 @@ std::unique_ptr<ans_renderer> newRenderer(const PlyLayout& L) {
     return r;
 }
 
-// Pinned staging for streamed file reads: two buffers, each with an event marking its last copy.
+// Process-wide pinned staging for file reads: two buffers, each with an event marking its last
+// copy. Allocated on the first load and kept until the process ends (pinning memory costs more
+// than a load of a typical scene); loads that share it are serialized by stagingMutex.
 struct Staging {
+    static constexpr size_t kBytes = size_t(16) << 20;
     uint8_t* buf[2] = {};
     hipEvent_t done[2] = {};
-    size_t cap = 0;
-    explicit Staging(size_t bytes) : cap(bytes) {
+    Staging() {
         for (int k = 0; k < 2; ++k) {
-            hipCheck(hipHostMalloc(reinterpret_cast<void**>(&buf[k]), bytes, hipHostMallocDefault), "hipHostMalloc");
+            hipCheck(hipHostMalloc(reinterpret_cast<void**>(&buf[k]), kBytes, hipHostMallocDefault), "hipHostMalloc");
             hipCheck(hipEventCreate(&done[k]), "hipEventCreate");
-        }
-    }
-    ~Staging() {
-        for (int k = 0; k < 2; ++k) {
-            if (done[k]) {
-                (void)hipEventSynchronize(done[k]);
-                (void)hipEventDestroy(done[k]);
-            }
-            if (buf[k]) (void)hipHostFree(buf[k]);
         }
     }
 };
 
-struct FileCloser {
-    void operator()(std::FILE* f) const { std::fclose(f); }
+std::mutex stagingMutex;
+
+Staging& staging() {
+    static Staging* s = new Staging();  // never freed: lives as long as the process
+    return *s;
+}
+
+struct Fd {
+    int fd = -1;
+    ~Fd() {
+        if (fd >= 0) ::close(fd);
+    }
 };
+
+// Reads bytes at `offset` into dst with `threads` concurrent preads (the page-cache copy of one
+// thread is slower than the copy to the device). False if the file ends early.
+bool readAt(int fd, uint8_t* dst, size_t bytes, uint64_t offset, int threads) {
+    std::atomic<bool> ok{true};
+    const size_t part = (bytes + threads - 1) / threads;
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; ++t) {
+        const size_t b0 = std::min(bytes, t * part), b1 = std::min(bytes, b0 + part);
+        pool.emplace_back([&, b0, b1] {
+            for (size_t b = b0; b < b1;) {
+                const ssize_t got = ::pread(fd, dst + b, b1 - b, static_cast<off_t>(offset + b));
+                if (got <= 0) {
+                    ok = false;
+                    return;
+                }
+                b += static_cast<size_t>(got);
+            }
+        });
+    }
+    for (auto& th : pool) th.join();
+    return ok;
+}
 
 ans_renderer* loadPly(const char* path) {
     if (!path) throw std::invalid_argument("path is null");
     const std::string name(path);
-    std::unique_ptr<std::FILE, FileCloser> f(std::fopen(path, "rb"));
-    if (!f) throw PlyError(name + ": cannot open");
-    if (std::fseek(f.get(), 0, SEEK_END) != 0) throw PlyError(name + ": cannot seek");
-    const long size = std::ftell(f.get());
-    if (size < 0) throw PlyError(name + ": cannot tell its size");
-    std::rewind(f.get());
+    Fd f;
+    f.fd = ::open(path, O_RDONLY);
+    if (f.fd < 0) throw PlyError(name + ": cannot open");
+    struct stat sb {};
+    if (::fstat(f.fd, &sb) != 0) throw PlyError(name + ": cannot stat");
+    const size_t size = static_cast<size_t>(sb.st_size);
     // The header: read until its end_header line is in memory (headers are a few KB).
     std::vector<char> head;
     PlyLayout L;
     for (size_t want = 4096;; want *= 4) {
-        head.resize(std::min<size_t>(want, static_cast<size_t>(size)));
-        std::rewind(f.get());
-        const size_t got = std::fread(head.data(), 1, head.size(), f.get());
+        head.resize(std::min(want, size));
+        size_t got = 0;
+        while (got < head.size()) {
+            const ssize_t k = ::pread(f.fd, head.data() + got, head.size() - got, static_cast<off_t>(got));
+            if (k <= 0) break;
+            got += static_cast<size_t>(k);
+        }
         head.resize(got);
         const std::string_view text(head.data(), head.size());
         const size_t end = text.find("end_header");
-        if ((end != std::string_view::npos && text.find('\n', end) != std::string_view::npos) ||
-            got == static_cast<size_t>(size)) {
-            L = parsePlyHeader(head.data(), head.size(), name, static_cast<size_t>(size));
+        if ((end != std::string_view::npos && text.find('\n', end) != std::string_view::npos) || got == size ||
+            got < std::min(want, size)) {
+            L = parsePlyHeader(head.data(), head.size(), name, size);
             break;
         }
     }
     auto r = newRenderer(L);
-    const size_t rowBytes = static_cast<size_t>(L.stride) * 4;
-    const size_t bodyBytes = static_cast<size_t>(L.n) * rowBytes;
+    const size_t bodyBytes = static_cast<size_t>(L.n) * static_cast<size_t>(L.stride) * 4;
     DevBuf<uint8_t> body;
     body.reserve(std::max<size_t>(bodyBytes, 1));
     if (bodyBytes) {
         hipStream_t st = r->engine->stream;
-        const size_t chunkRows = std::max<size_t>(1, (size_t(32) << 20) / rowBytes);
-        Staging stage(chunkRows * rowBytes);
-        if (std::fseek(f.get(), static_cast<long>(L.body), SEEK_SET) != 0) throw PlyError(name + ": cannot seek");
+        std::lock_guard<std::mutex> lock(stagingMutex);
+        Staging& stage = staging();
         size_t done = 0;
-        for (int k = 0; done < bodyBytes; k ^= 1) {
-            const size_t bytes = std::min(stage.cap, bodyBytes - done);
-            hipCheck(hipEventSynchronize(stage.done[k]), "hipEventSynchronize");  // this buffer's last copy
-            if (std::fread(stage.buf[k], 1, bytes, f.get()) != bytes)
-                throw PlyError(name + ": file is shorter than the header promises (" + std::to_string(L.n) +
-                               " Gaussians)");
-            hipCheck(hipMemcpyAsync(body.p + done, stage.buf[k], bytes, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
-            hipCheck(hipEventRecord(stage.done[k], st), "hipEventRecord");
-            done += bytes;
+        try {
+            for (int k = 0; done < bodyBytes; k ^= 1) {
+                const size_t bytes = std::min(Staging::kBytes, bodyBytes - done);
+                hipCheck(hipEventSynchronize(stage.done[k]), "hipEventSynchronize");  // this buffer's last copy
+                if (!readAt(f.fd, stage.buf[k], bytes, L.body + done, 4))
+                    throw PlyError(name + ": file is shorter than the header promises (" + std::to_string(L.n) +
+                                   " Gaussians)");
+                hipCheck(hipMemcpyAsync(body.p + done, stage.buf[k], bytes, hipMemcpyHostToDevice, st),
+                         "hipMemcpyAsync");
+                hipCheck(hipEventRecord(stage.done[k], st), "hipEventRecord");
+                done += bytes;
+            }
+        } catch (...) {
+            (void)hipStreamSynchronize(st);  // the staging buffers are not reused while copies read them
+            throw;
         }
     }
     return ingestBody(L, reinterpret_cast<const float*>(body.p), std::move(r));
