@@ -130,8 +130,9 @@ __global__ __launch_bounds__(TW* TH) void k_blendBackward(
             const uint4 meta = rec[g].meta;
             const uint2 rp = make_uint2(meta.x, meta.y);
             const uint32_t first = slotStart[g];
-            s0[k] = make_float4(a.x, a.y, -0.5f * a.z, a.w);  // the blend's staged (-A/2, B, -C/2)
-            s1[k] = make_float4(-0.5f * bb.x, bb.y, bb.z, bb.w);
+            const float3 q = stagedConic(a.z, a.w, bb.x);  // the blend's staged conic
+            s0[k] = make_float4(a.x, a.y, q.x, q.y);
+            s1[k] = make_float4(q.z, bb.y, bb.z, bb.w);
             s2[k] = rec[g].col;
             const uint32_t x0 = rp.x & 0xffffu, y0 = rp.x >> 16, w = (rp.y & 0xffffu) - x0 + 1;
             sSlot[k] = first + (static_cast<uint32_t>(tileY) - y0) * w + (static_cast<uint32_t>(tileX) - x0);
@@ -196,11 +197,11 @@ __global__ __launch_bounds__(TW* TH) void k_blendBackward(
     // operations), so every blend/skip decision matches the forward's; selects replace branches.
     auto visit = [&](const float4& a, const float4& bb, const float4& c, uint32_t pos, float (&d)[9]) -> bool {
         const float dx = px - a.x, dy = py - a.y;
-        const float power = a.z * dx * dx + bb.x * dy * dy - a.w * dx * dy;
-        const float G = __expf(power);
+        const float p = splatPower2(a.z, a.w, bb.x, dx, dy);
+        const float G = __builtin_amdgcn_exp2f(p);
         const float raw = bb.y * G;
         const float alpha = fminf(0.99f, raw);
-        const bool use = pos < lastContrib && !(power > 0.0f) && alpha >= 1.0f / 255.0f;
+        const bool use = pos < lastContrib && !(p > 0.0f) && alpha >= 1.0f / 255.0f;
         const float oneMinus = 1.0f - alpha;
         const float Ti = Tcur * __builtin_amdgcn_rcpf(oneMinus);
         const float w = use ? alpha * Ti : 0.0f;
@@ -212,9 +213,11 @@ __global__ __launch_bounds__(TW* TH) void k_blendBackward(
         const bool open = use && raw <= 0.99f;
         const float gPow = open ? gAlpha * raw : 0.0f;
         d[5] = open ? gAlpha * G : 0.0f;
-        // Staged a.z = -A/2, bb.x = -C/2, a.w = B.
-        d[0] = gPow * (-2.0f * a.z * dx + a.w * dy);
-        d[1] = gPow * (-2.0f * bb.x * dy + a.w * dx);
+        // Staged a.z = -log2(e) A/2, a.w = -log2(e) B, bb.x = -log2(e) C/2: A dx + B dy and
+        // C dy + B dx are the staged forms times ln 2.
+        const float gPowLn2 = gPow * (1.0f / kLog2e);
+        d[0] = gPowLn2 * (-2.0f * a.z * dx - a.w * dy);
+        d[1] = gPowLn2 * (-2.0f * bb.x * dy - a.w * dx);
         d[2] = gPow * (-0.5f * dx * dx);
         d[3] = gPow * (-dx * dy);
         d[4] = gPow * (-0.5f * dy * dy);

@@ -625,12 +625,32 @@ __device__ inline float quadMinOverBox(float A, float B, float C, float rB_C, fl
     return fminf(fminf(a, b), fminf(c, d));
 }
 
+// log2(e): the blend works in log2 units, so the exponential is one v_exp_f32.
+constexpr float kLog2e = 1.4426950408889634f;
+
+// A splat's conic staged for the blend and its adjoint: log2(e) power = qa dx^2 + qb dx dy +
+// qc dy^2 with qa = -log2(e) A / 2, qb = -log2(e) B, qc = -log2(e) C / 2.
+__device__ inline float3 stagedConic(float A, float B, float C) {
+    return make_float3(-0.5f * kLog2e * A, -kLog2e * B, -0.5f * kLog2e * C);
+}
+
+// log2(e) power at offset (dx, dy) from the centre, as (qa dx + qb dy) dx + (qc dy) dy in two
+// fused multiply-adds and three products. Every term rounds relative to itself, so the value is
+// positive (the reference's power > 0 skip) only where the true power is zero to within
+// rounding, as in the reference's own evaluation.
+__device__ inline float splatPower2(float qa, float qb, float qc, float dx, float dy) {
+    const float t = __builtin_fmaf(qa, dx, qb * dy);
+    const float r = qc * dy;
+    return __builtin_fmaf(t, dx, r * dy);
+}
+
 // Front-to-back blend, one workgroup per TW x TH tile, one lane per pixel; each wavefront owns
 // an 8 x 8 block of the tile. The per-pixel rule is the reference's (src/gpu_render.cu:180-233,
 // src/render.cpp:193-221): skip power > 0; alpha = min(0.99, o exp(power)); skip
-// alpha < 1/255; stop once T (1 - alpha) < 1e-4; C += rgb alpha T. power is evaluated in the
-// same operation order as the CPU renderer, so every skip/stop decision matches it except where
-// the exponential's last bits move alpha across 1/255 or T across 1e-4.
+// alpha < 1/255; stop once T (1 - alpha) < 1e-4; C += rgb alpha T. Here power is evaluated in
+// log2 units with fused multiply-adds (splatPower2), exp as exp2, and T (1 - alpha) as
+// T - alpha T; these differ from the reference's rounding in the last bits, which moves alpha
+// across 1/255 or T across 1e-4 only at the margins (at most 1/255 on a pixel).
 //
 // Per round, up to BATCH splat records of the tile's depth-sorted list are staged in LDS. With
 // CULL, the staging lane also tests the splat against each wavefront's 8 x 8 block: a splat
@@ -639,13 +659,13 @@ __device__ inline float quadMinOverBox(float A, float B, float C, float rB_C, fl
 // wavefront compacts the indices it must visit with ballots, walks them in order, and stops as
 // soon as all of its pixels are done; the workgroup stops when all wavefronts are.
 //
+// A pixel that is done carries T <= 0 (-T once it stops, -1 outside the image): its alpha T is
+// then <= 0 and T - alpha T below 1e-4, so every later splat leaves it unchanged without a
+// separate done flag, and |T| is its final transmittance.
+//
 // SAVE (frames the backward pass will read): pixState receives, per pixel, the final
 // transmittance's bits and the position + 1 in the tile's list of the last splat blended into it
 // (0: none).
-//
-// The staged conic is (-A/2, B, -C/2): power = (-A/2 dx) dx + (-C/2 dy) dy - (B dx) dy equals the
-// reference's -0.5 (A dx dx + C dy dy) - B dx dy bit for bit, since scaling by a power of two
-// commutes with rounding.
 template <int TW, int TH, int BATCH, bool CULL, bool SAVE>
 __device__ inline void blendTile(int tileX, int tileY, uint2 range, const uint32_t* __restrict__ sortedGauss,
                                  const SplatRec* __restrict__ rec, int width, int height, uint8_t* __restrict__ out8,
@@ -654,7 +674,9 @@ __device__ inline void blendTile(int tileX, int tileY, uint2 range, const uint32
     constexpr int kWaves = kThreads / kWave;
     constexpr int kWavesX = TW / 8;
     static_assert(kWaves <= 8, "at most eight 8x8 blocks per tile");
-    __shared__ float4 s0[BATCH], s1[BATCH], s2[BATCH];
+    // Staged per splat, only what the walk reads: (u, v, qa, qb), (qc, opacity, r, g), b.
+    __shared__ float4 s0[BATCH], s1[BATCH];
+    __shared__ float s2[BATCH];
     __shared__ uint8_t sMask[BATCH];
     __shared__ uint8_t sList[kWaves][BATCH];
 
@@ -666,20 +688,21 @@ __device__ inline void blendTile(int tileX, int tileY, uint2 range, const uint32
     const float px = x + 0.5f, py = y + 0.5f;
     const uint64_t below = lanesBelow();
 
-    float T = 1.0f;
+    float T = inside ? 1.0f : -1.0f;
     float cr = 0.0f, cg = 0.0f, cb = 0.0f;
     uint32_t last = 0;
-    bool done = !inside;
     for (uint32_t start = range.x; start < range.y; start += BATCH) {
-        if (__syncthreads_count(done) == kThreads) break;
+        if (__syncthreads_count(T <= 0.0f) == kThreads) break;
         const int cnt = min(static_cast<uint32_t>(BATCH), range.y - start);
         for (int k = tid; k < cnt; k += kThreads) {
             const uint32_t g = sortedGauss[start + k];
             const float4 a = rec[g].geo;
             const float4 b = rec[g].geo2;
-            s0[k] = make_float4(a.x, a.y, -0.5f * a.z, a.w);
-            s1[k] = make_float4(-0.5f * b.x, b.y, b.z, b.w);
-            s2[k] = rec[g].col;
+            const float4 col = rec[g].col;
+            const float3 q = stagedConic(a.z, a.w, b.x);
+            s0[k] = make_float4(a.x, a.y, q.x, q.y);
+            s1[k] = make_float4(q.z, b.y, col.x, col.y);
+            s2[k] = col.z;
             if constexpr (CULL) {
                 uint32_t m = 0;
                 const float op = b.y;
@@ -699,7 +722,7 @@ __device__ inline void blendTile(int tileX, int tileY, uint2 range, const uint32
             }
         }
         __syncthreads();
-        if (__ballot(!done) == 0) continue;
+        if (__ballot(T > 0.0f) == 0) continue;
         int count = cnt;
         if constexpr (CULL) {
             count = 0;
@@ -715,31 +738,32 @@ __device__ inline void blendTile(int tileX, int tileY, uint2 range, const uint32
         // Software-pipelined walk over the list, two splats per step with two register sets:
         // while one splat is composited, the other set receives the next splat's records and the
         // list index after it is read. The per-pixel rule is applied with selects instead of
-        // divergent branches; a skipped or stopping splat changes nothing, so every value equals
-        // the branching form's.
-        auto composite = [&](const float4& a, const float4& b, const float4& c, int j) {
-            const float dx = px - a.x, dy = py - a.y;
-            const float power = a.z * dx * dx + b.x * dy * dy - a.w * dx * dy;
-            const float alpha = fminf(0.99f, b.y * __expf(power));
-            const float nextT = T * (1.0f - alpha);
-            const bool hit = !done && !(power > 0.0f) && alpha >= 1.0f / 255.0f;
-            const bool stop = hit && nextT < 1e-4f;
-            const bool use = hit && !stop;
-            done = done || stop;
+        // divergent branches; a skipped splat and a done pixel change nothing.
+        auto composite = [&](const float4& a, const float4& b, float blue, int j) {
+            const float p = splatPower2(a.z, a.w, b.x, px - a.x, py - a.y);
+            const float alpha = fminf(0.99f, b.y * __builtin_amdgcn_exp2f(p));
+            const float aT = alpha * T;
+            const float nT = T - aT;
+            const bool hit = !(p > 0.0f) && alpha >= 1.0f / 255.0f;
+            const bool keep = nT >= 1e-4f;
+            const bool use = hit && keep;
             // A weight of +0 leaves the (non-negative, finite) colour sums unchanged exactly.
-            const float w = use ? alpha * T : 0.0f;
-            cr = __builtin_fmaf(c.x, w, cr);
-            cg = __builtin_fmaf(c.y, w, cg);
-            cb = __builtin_fmaf(c.z, w, cb);
-            T = use ? nextT : T;
+            const float w = use ? aT : 0.0f;
+            cr = __builtin_fmaf(b.z, w, cr);
+            cg = __builtin_fmaf(b.w, w, cg);
+            cb = __builtin_fmaf(blue, w, cb);
+            const float stopped = keep ? nT : -__builtin_fabsf(T);
+            T = hit ? stopped : T;
             if constexpr (SAVE) last = use ? start - range.x + static_cast<uint32_t>(j) + 1 : last;
         };
         auto listAt = [&](int i) { return CULL ? static_cast<int>(sList[wave][i]) : i; };
         if (count > 0) {
             int jA = listAt(0);
-            float4 aA = s0[jA], bA = s1[jA], cA = s2[jA];
+            float4 aA = s0[jA], bA = s1[jA];
+            float cA = s2[jA];
             int jB = count > 1 ? listAt(1) : 0;
-            float4 aB, bB, cB;
+            float4 aB, bB;
+            float cB;
             for (int i = 0; i < count; i += 2) {
                 const bool more1 = i + 1 < count;
                 if (more1) {
@@ -759,7 +783,7 @@ __device__ inline void blendTile(int tileX, int tileY, uint2 range, const uint32
                 const int jNextB = i + 3 < count ? listAt(i + 3) : 0;
                 composite(aB, bB, cB, jB);
                 jB = jNextB;
-                if ((i & 7) == 6 && __ballot(!done) == 0) break;
+                if ((i & 7) == 6 && __ballot(T > 0.0f) == 0) break;
             }
         }
     }
@@ -775,7 +799,7 @@ __device__ inline void blendTile(int tileX, int tileY, uint2 range, const uint32
             outf[3 * p + 1] = cg;
             outf[3 * p + 2] = cb;
         }
-        if constexpr (SAVE) pixState[p] = make_uint2(__float_as_uint(T), last);
+        if constexpr (SAVE) pixState[p] = make_uint2(__float_as_uint(__builtin_fabsf(T)), last);
     }
 }
 
