@@ -130,9 +130,9 @@ __global__ __launch_bounds__(TW* TH) void k_blendBackward(
             const uint4 meta = rec[g].meta;
             const uint2 rp = make_uint2(meta.x, meta.y);
             const uint32_t first = slotStart[g];
-            const float3 q = stagedConic(a.z, a.w, bb.x);  // the blend's staged conic
+            const float3 q = stagedConic(a.z, a.w, bb.x);  // the blend's staged conic and log2 o
             s0[k] = make_float4(a.x, a.y, q.x, q.y);
-            s1[k] = make_float4(q.z, bb.y, bb.z, bb.w);
+            s1[k] = make_float4(q.z, bb.y, stagedLogOpacity(bb.y), 1.0f / bb.y);
             s2[k] = rec[g].col;
             const uint32_t x0 = rp.x & 0xffffu, y0 = rp.x >> 16, w = (rp.y & 0xffffu) - x0 + 1;
             sSlot[k] = first + (static_cast<uint32_t>(tileY) - y0) * w + (static_cast<uint32_t>(tileX) - x0);
@@ -197,11 +197,11 @@ __global__ __launch_bounds__(TW* TH) void k_blendBackward(
     // operations), so every blend/skip decision matches the forward's; selects replace branches.
     auto visit = [&](const float4& a, const float4& bb, const float4& c, uint32_t pos, float (&d)[9]) -> bool {
         const float dx = px - a.x, dy = py - a.y;
-        const float p = splatPower2(a.z, a.w, bb.x, dx, dy);
-        const float G = __builtin_amdgcn_exp2f(p);
-        const float raw = bb.y * G;
+        const float e = splatExponent(a.z, a.w, bb.x, bb.z, dx, dy);
+        const float raw = __builtin_amdgcn_exp2f(e);  // o G
+        const float G = raw * bb.w;                     // bb.w = 1 / o
         const float alpha = fminf(0.99f, raw);
-        const bool use = pos < lastContrib && !(p > 0.0f) && alpha >= 1.0f / 255.0f;
+        const bool use = pos < lastContrib && !(e > bb.z) && alpha >= 1.0f / 255.0f;
         const float oneMinus = 1.0f - alpha;
         const float Ti = Tcur * __builtin_amdgcn_rcpf(oneMinus);
         const float w = use ? alpha * Ti : 0.0f;

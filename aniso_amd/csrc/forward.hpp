@@ -634,23 +634,28 @@ __device__ inline float3 stagedConic(float A, float B, float C) {
     return make_float3(-0.5f * kLog2e * A, -kLog2e * B, -0.5f * kLog2e * C);
 }
 
-// log2(e) power at offset (dx, dy) from the centre, as (qa dx + qb dy) dx + (qc dy) dy in two
-// fused multiply-adds and three products. Every term rounds relative to itself, so the value is
-// positive (the reference's power > 0 skip) only where the true power is zero to within
-// rounding, as in the reference's own evaluation.
-__device__ inline float splatPower2(float qa, float qb, float qc, float dx, float dy) {
+// log2 of a splat's opacity, staged beside its conic: alpha = min(0.99, exp2(e)) with
+// e = log2(e) power + log2(o), so the opacity costs no multiply in the walk.
+__device__ inline float stagedLogOpacity(float o) { return __builtin_amdgcn_logf(o); }
+
+// e = log2(e) power + log2(o) at offset (dx, dy) from the centre, as
+// (qa dx + qb dy) dx + ((qc dy) dy + lo) in three fused multiply-adds and two products. Every
+// term of the power rounds relative to itself, so e > lo (the reference's power > 0 skip) holds
+// only where the true power is zero to within rounding, as in the reference's own evaluation.
+__device__ inline float splatExponent(float qa, float qb, float qc, float lo, float dx, float dy) {
     const float t = __builtin_fmaf(qa, dx, qb * dy);
     const float r = qc * dy;
-    return __builtin_fmaf(t, dx, r * dy);
+    return __builtin_fmaf(t, dx, __builtin_fmaf(r, dy, lo));
 }
 
 // Front-to-back blend, one workgroup per TW x TH tile, one lane per pixel; each wavefront owns
 // an 8 x 8 block of the tile. The per-pixel rule is the reference's (src/gpu_render.cu:180-233,
 // src/render.cpp:193-221): skip power > 0; alpha = min(0.99, o exp(power)); skip
 // alpha < 1/255; stop once T (1 - alpha) < 1e-4; C += rgb alpha T. Here power is evaluated in
-// log2 units with fused multiply-adds (splatPower2), exp as exp2, and T (1 - alpha) as
-// T - alpha T; these differ from the reference's rounding in the last bits, which moves alpha
-// across 1/255 or T across 1e-4 only at the margins (at most 1/255 on a pixel).
+// log2 units with fused multiply-adds and the opacity folded in (splatExponent), exp as exp2,
+// and T (1 - alpha) as T - alpha T; these differ from the reference's rounding in the last bits,
+// which moves alpha across 1/255 or T across 1e-4 only at the margins (at most 1/255 on a
+// pixel).
 //
 // Per round, up to BATCH splat records of the tile's depth-sorted list are staged in LDS. With
 // CULL, the staging lane also tests the splat against each wavefront's 8 x 8 block: a splat
@@ -674,11 +679,14 @@ __device__ inline void blendTile(int tileX, int tileY, uint2 range, const uint32
     constexpr int kWaves = kThreads / kWave;
     constexpr int kWavesX = TW / 8;
     static_assert(kWaves <= 8, "at most eight 8x8 blocks per tile");
-    // Staged per splat, only what the walk reads: (u, v, qa, qb), (qc, opacity, r, g), b.
-    __shared__ float4 s0[BATCH], s1[BATCH];
-    __shared__ float s2[BATCH];
+    // Staged per splat in three arrays of 16-byte rows read at one byte offset (the visit lists
+    // hold offsets, so the walk computes no addresses): (u, v, qa, qb), (qc, log2 o, r, g), (b).
+    __shared__ float4 sRec[3][BATCH];
     __shared__ uint8_t sMask[BATCH];
-    __shared__ uint8_t sList[kWaves][BATCH];
+    __shared__ uint16_t sList[kWaves][BATCH + 4];  // padded: the walk reads up to 3 entries ahead
+    auto row = [&](int r, uint32_t off) -> const float4& {
+        return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(sRec[r]) + off);
+    };
 
     const int tid = threadIdx.x;
     const int wave = tid / kWave, lane = __lane_id();
@@ -700,9 +708,9 @@ __device__ inline void blendTile(int tileX, int tileY, uint2 range, const uint32
             const float4 b = rec[g].geo2;
             const float4 col = rec[g].col;
             const float3 q = stagedConic(a.z, a.w, b.x);
-            s0[k] = make_float4(a.x, a.y, q.x, q.y);
-            s1[k] = make_float4(q.z, b.y, col.x, col.y);
-            s2[k] = col.z;
+            sRec[0][k] = make_float4(a.x, a.y, q.x, q.y);
+            sRec[1][k] = make_float4(q.z, stagedLogOpacity(b.y), col.x, col.y);
+            sRec[2][k].x = col.z;
             if constexpr (CULL) {
                 uint32_t m = 0;
                 const float op = b.y;
@@ -730,7 +738,7 @@ __device__ inline void blendTile(int tileX, int tileY, uint2 range, const uint32
                 const int j = g0 + lane;
                 const bool hit = j < cnt && ((sMask[j] >> wave) & 1u);
                 const uint64_t bal = __ballot(hit);
-                if (hit) sList[wave][count + __popcll(bal & below)] = static_cast<uint8_t>(j);
+                if (hit) sList[wave][count + __popcll(bal & below)] = static_cast<uint16_t>(j * 16);
                 count += __popcll(bal);
             }
             waveSync();
@@ -739,12 +747,12 @@ __device__ inline void blendTile(int tileX, int tileY, uint2 range, const uint32
         // while one splat is composited, the other set receives the next splat's records and the
         // list index after it is read. The per-pixel rule is applied with selects instead of
         // divergent branches; a skipped splat and a done pixel change nothing.
-        auto composite = [&](const float4& a, const float4& b, float blue, int j) {
-            const float p = splatPower2(a.z, a.w, b.x, px - a.x, py - a.y);
-            const float alpha = fminf(0.99f, b.y * __builtin_amdgcn_exp2f(p));
+        auto composite = [&](const float4& a, const float4& b, float blue, uint32_t off) {
+            const float e = splatExponent(a.z, a.w, b.x, b.y, px - a.x, py - a.y);
+            const float alpha = fminf(0.99f, __builtin_amdgcn_exp2f(e));
             const float aT = alpha * T;
             const float nT = T - aT;
-            const bool hit = !(p > 0.0f) && alpha >= 1.0f / 255.0f;
+            const bool hit = !(e > b.y) && alpha >= 1.0f / 255.0f;
             const bool keep = nT >= 1e-4f;
             const bool use = hit && keep;
             // A weight of +0 leaves the (non-negative, finite) colour sums unchanged exactly.
@@ -754,33 +762,33 @@ __device__ inline void blendTile(int tileX, int tileY, uint2 range, const uint32
             cb = __builtin_fmaf(blue, w, cb);
             const float stopped = keep ? nT : -__builtin_fabsf(T);
             T = hit ? stopped : T;
-            if constexpr (SAVE) last = use ? start - range.x + static_cast<uint32_t>(j) + 1 : last;
+            if constexpr (SAVE) last = use ? start - range.x + (off >> 4) + 1 : last;
         };
-        auto listAt = [&](int i) { return CULL ? static_cast<int>(sList[wave][i]) : i; };
+        auto listAt = [&](int i) -> uint32_t { return CULL ? sList[wave][i] : static_cast<uint32_t>(i) * 16u; };
         if (count > 0) {
-            int jA = listAt(0);
-            float4 aA = s0[jA], bA = s1[jA];
-            float cA = s2[jA];
-            int jB = count > 1 ? listAt(1) : 0;
+            uint32_t jA = listAt(0);
+            float4 aA = row(0, jA), bA = row(1, jA);
+            float cA = row(2, jA).x;
+            uint32_t jB = listAt(1);  // entries past count are read, never used
             float4 aB, bB;
             float cB;
             for (int i = 0; i < count; i += 2) {
                 const bool more1 = i + 1 < count;
                 if (more1) {
-                    aB = s0[jB];
-                    bB = s1[jB];
-                    cB = s2[jB];
+                    aB = row(0, jB);
+                    bB = row(1, jB);
+                    cB = row(2, jB).x;
                 }
-                const int jNextA = i + 2 < count ? listAt(i + 2) : 0;
+                const uint32_t jNextA = listAt(i + 2);
                 composite(aA, bA, cA, jA);
                 if (!more1) break;
                 jA = jNextA;
                 if (i + 2 < count) {
-                    aA = s0[jA];
-                    bA = s1[jA];
-                    cA = s2[jA];
+                    aA = row(0, jA);
+                    bA = row(1, jA);
+                    cA = row(2, jA).x;
                 }
-                const int jNextB = i + 3 < count ? listAt(i + 3) : 0;
+                const uint32_t jNextB = listAt(i + 3);
                 composite(aB, bB, cB, jB);
                 jB = jNextB;
                 if ((i & 7) == 6 && __ballot(T > 0.0f) == 0) break;
