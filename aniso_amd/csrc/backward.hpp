@@ -197,33 +197,37 @@ __global__ __launch_bounds__(TW* TH) void k_blendBackward(
     // operations), so every blend/skip decision matches the forward's; selects replace branches.
     auto visit = [&](const float4& a, const float4& bb, const float4& c, uint32_t pos, float (&d)[9]) -> bool {
         const float dx = px - a.x, dy = py - a.y;
-        const float e = splatExponent(a.z, a.w, bb.x, bb.z, dx, dy);
-        const float raw = __builtin_amdgcn_exp2f(e);  // o G
-        const float G = raw * bb.w;                     // bb.w = 1 / o
+        const SplatExponent x = splatExponentParts(a.z, a.w, bb.x, bb.z, dx, dy);
+        const float raw = __builtin_amdgcn_exp2f(x.e);  // o G
+        const float G = raw * bb.w;                       // bb.w = 1 / o
         const float alpha = fminf(0.99f, raw);
-        const bool use = pos < lastContrib && !(e > bb.z) && alpha >= 1.0f / 255.0f;
-        const float oneMinus = 1.0f - alpha;
-        const float Ti = Tcur * __builtin_amdgcn_rcpf(oneMinus);
+        const bool use = pos < lastContrib && !(x.e > bb.z) && alpha >= 1.0f / 255.0f;
+        const float Ti = Tcur * __builtin_amdgcn_rcpf(1.0f - alpha);
         const float w = use ? alpha * Ti : 0.0f;
         d[6] = gr * w;
         d[7] = gg * w;
         d[8] = gb * w;
-        const float gAlpha = Ti * (gr * (c.x - ar) + gg * (c.y - ag) + gb * (c.z - ab));
+        const float er = c.x - ar, eg = c.y - ag, eb = c.z - ab;
+        const float gAlpha = Ti * __builtin_fmaf(gb, eb, __builtin_fmaf(gg, eg, gr * er));
         // No gradient where the 0.99 clamp is active; gPow = 0 zeroes the conic and centre terms.
         const bool open = use && raw <= 0.99f;
         const float gPow = open ? gAlpha * raw : 0.0f;
         d[5] = open ? gAlpha * G : 0.0f;
-        // Staged a.z = -log2(e) A/2, a.w = -log2(e) B, bb.x = -log2(e) C/2: A dx + B dy and
-        // C dy + B dx are the staged forms times ln 2.
+        // With the staged qa = -log2(e) A/2, qb = -log2(e) B, qc = -log2(e) C/2 and
+        // t = qa dx + qb dy, r = qc dy: A dx + B dy = -(qa dx + t) ln 2 and
+        // C dy + B dx = -(qb dx + 2 r) ln 2.
         const float gPowLn2 = gPow * (1.0f / kLog2e);
-        d[0] = gPowLn2 * (-2.0f * a.z * dx - a.w * dy);
-        d[1] = gPowLn2 * (-2.0f * bb.x * dy - a.w * dx);
-        d[2] = gPow * (-0.5f * dx * dx);
-        d[3] = gPow * (-dx * dy);
-        d[4] = gPow * (-0.5f * dy * dy);
-        ar = use ? alpha * c.x + oneMinus * ar : ar;
-        ag = use ? alpha * c.y + oneMinus * ag : ag;
-        ab = use ? alpha * c.z + oneMinus * ab : ab;
+        d[0] = -gPowLn2 * __builtin_fmaf(a.z, dx, x.t);
+        d[1] = -gPowLn2 * __builtin_fmaf(a.w, dx, x.r + x.r);
+        const float h = -0.5f * gPow;
+        const float hx = h * dx;
+        d[2] = hx * dx;
+        d[3] = (hx + hx) * dy;
+        d[4] = (h * dy) * dy;
+        // accum <- alpha rgb + (1 - alpha) accum, as accum + alpha (rgb - accum).
+        ar = use ? __builtin_fmaf(alpha, er, ar) : ar;
+        ag = use ? __builtin_fmaf(alpha, eg, ag) : ag;
+        ab = use ? __builtin_fmaf(alpha, eb, ab) : ab;
         Tcur = use ? Ti : Tcur;
         return use;
     };
@@ -235,43 +239,21 @@ __global__ __launch_bounds__(TW* TH) void k_blendBackward(
         int count;
         const int cnt = stage(b, count);
         if (__ballot(lastContrib > static_cast<uint32_t>(b) * BATCH) != 0 && count > 0) {
-            // Pairs (i, i - 1) back to front; the next pair's records are read from LDS while
-            // this pair's contributions are reduced.
+            // Pairs (i, i - 1) back to front. Records are read at the top of each step: carried
+            // across steps they cost more register moves than the LDS latency other waves hide.
             const uint32_t base = static_cast<uint32_t>(b) * BATCH;
-            int i = count - 1;
-            int j1 = sList[wave][i];
-            int j2 = i >= 1 ? static_cast<int>(sList[wave][i - 1]) : BATCH;
-            float4 a1 = s0[j1], b1 = s1[j1], c1 = s2[j1];
-            float4 a2 = a1, b2 = b1, c2 = c1;
-            if (i >= 1) {
-                a2 = s0[j2];
-                b2 = s1[j2];
-                c2 = s2[j2];
-            }
-            for (; i >= 0; i -= 2) {
-                const int n1 = i >= 2 ? static_cast<int>(sList[wave][i - 2]) : BATCH;
-                const int n2 = i >= 3 ? static_cast<int>(sList[wave][i - 3]) : BATCH;
+            for (int i = count - 1; i >= 0; i -= 2) {
+                const int j1 = sList[wave][i];
+                const int j2 = i >= 1 ? static_cast<int>(sList[wave][i - 1]) : BATCH;
                 float d1[9], d2[9];
-                bool any = visit(a1, b1, c1, base + j1, d1);
+                bool any = visit(s0[j1], s1[j1], s2[j1], base + j1, d1);
                 if (i >= 1) {
-                    any |= visit(a2, b2, c2, base + j2, d2);
+                    any |= visit(s0[j2], s1[j2], s2[j2], base + j2, d2);
                 } else {
 #pragma unroll
                     for (int c = 0; c < 9; ++c) d2[c] = 0.0f;
                 }
-                if (i >= 2) {
-                    a1 = s0[n1];
-                    b1 = s1[n1];
-                    c1 = s2[n1];
-                }
-                if (i >= 3) {
-                    a2 = s0[n2];
-                    b2 = s1[n2];
-                    c2 = s2[n2];
-                }
                 if (__ballot(any) != 0) pairSumStore(d1, d2, &sGrad[wave][secondSplat ? j2 : j1][rowHalf * 5]);
-                j1 = n1;
-                j2 = n2;
             }
         }
         __syncthreads();
