@@ -105,6 +105,7 @@ def main():
         "traffic": measured_traffic("k_blend", args.workload),
         "algorithmic_bytes": b_blend,
         "kernel_ms": round(blend, 4),
+        "valu": valu_bound(measured_traffic("k_blend", args.workload, "valu_instructions_per_launch"), blend),
         "timing": f"HIP events around the blend on the library's stream, mean of the last "
                   f"{min(args.steps, 64)} timed frames",
     }
@@ -182,16 +183,35 @@ def copy_calibration():
     del src, dst
 
 
-def measured_traffic(kernel, workload):
-    """HBM bytes per launch of `kernel` on `workload` from the committed counter summary
-    profiles/<kernel>_<workload>.md (its `traffic_bytes_per_launch:` line), or None."""
+def measured_traffic(kernel, workload, field="traffic_bytes_per_launch"):
+    """A per-launch counter figure of `kernel` on `workload` from the committed counter summary
+    profiles/<kernel>_<workload>.md (its `<field>:` line, by default the HBM bytes), or None."""
     import re
 
     path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}.md")
     if not os.path.exists(path):
         return None
-    m = re.search(r"^traffic_bytes_per_launch:\s*(\d+)", open(path).read(), re.M)
+    m = re.search(rf"^{field}:\s*(\d+)", open(path).read(), re.M)
     return int(m.group(1)) if m else None
+
+
+# The vector ALU of MI355X without packed FP32 (which this build keeps off): 256 CUs x 4 SIMDs,
+# a wave64 instruction occupying a 16-lane SIMD for 4 cycles, 2.4 GHz peak engine clock.
+SIMDS = 1024
+VALU_CYCLES_PER_INST = 4
+CLOCK_HZ = 2.4e9
+
+
+def valu_bound(instructions, kernel_ms):
+    """The blend's compute side beside the HBM roofline: wave instructions per launch (the
+    profile's SQ_INSTS_VALU pass) over the chip's VALU issue capacity during the event-timed
+    kernel, at the peak clock (a lower bound on the busy fraction if the clock runs lower)."""
+    if not instructions:
+        return None
+    capacity = SIMDS * kernel_ms * 1e-3 * CLOCK_HZ / VALU_CYCLES_PER_INST
+    return {"instructions_per_launch": instructions, "capacity_per_launch": int(capacity),
+            "busy_frac": round(instructions / capacity, 3),
+            "model": "wave64 FP32 instruction = 4 cycles on a 16-lane SIMD (no packed FP32), 1024 SIMDs, 2.4 GHz"}
 
 
 def bench_forward(workload, steps, warmup):
