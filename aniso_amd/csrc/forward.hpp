@@ -674,9 +674,11 @@ __device__ inline float splatExponent(float qa, float qb, float qc, float lo, fl
 // wavefront compacts the indices it must visit with ballots, walks them in order, and stops as
 // soon as all of its pixels are done; the workgroup stops when all wavefronts are.
 //
-// A pixel that is done carries T <= 0 (-T once it stops, -1 outside the image): its alpha T is
-// then <= 0 and T - alpha T below 1e-4, so every later splat leaves it unchanged without a
-// separate done flag, and |T| is its final transmittance.
+// A done pixel needs no flag of its own: its T makes every later splat a no-op. Without SAVE a
+// stopping splat leaves T (1 - alpha) < 1e-4 in T (0 outside the image), so T (1 - alpha) of any
+// later splat stays below 1e-4 and is never kept. With SAVE, where the final transmittance must
+// survive, a stop stores -T (-1 outside the image): alpha T <= 0, T - alpha T < 1e-4, and |T| is
+// the final transmittance.
 //
 // SAVE (frames the backward pass will read): pixState receives, per pixel, the final
 // transmittance's bits and the position + 1 in the tile's list of the last splat blended into it
@@ -706,11 +708,13 @@ __device__ inline void blendTile(int tileX, int tileY, uint2 range, const uint32
     const float px = x + 0.5f, py = y + 0.5f;
     const uint64_t below = lanesBelow();
 
-    float T = inside ? 1.0f : -1.0f;
+    float T = inside ? 1.0f : (SAVE ? -1.0f : 0.0f);
+    // Whether a pixel still blends (see the note on done pixels above).
+    auto live = [](float t) { return SAVE ? t > 0.0f : t >= 1e-4f; };
     float cr = 0.0f, cg = 0.0f, cb = 0.0f;
     uint32_t last = 0;
     for (uint32_t start = range.x; start < range.y; start += BATCH) {
-        if (__syncthreads_count(T <= 0.0f) == kThreads) break;
+        if (__syncthreads_count(!live(T)) == kThreads) break;
         const int cnt = min(static_cast<uint32_t>(BATCH), range.y - start);
         for (int k = tid; k < cnt; k += kThreads) {
             const uint32_t g = sortedGauss[start + k];
@@ -740,7 +744,7 @@ __device__ inline void blendTile(int tileX, int tileY, uint2 range, const uint32
             }
         }
         __syncthreads();
-        if (__ballot(T > 0.0f) == 0) continue;
+        if (__ballot(live(T)) == 0) continue;
         int count = cnt;
         if constexpr (CULL) {
             count = 0;
@@ -770,9 +774,13 @@ __device__ inline void blendTile(int tileX, int tileY, uint2 range, const uint32
             cr = __builtin_fmaf(b.z, w, cr);
             cg = __builtin_fmaf(b.w, w, cg);
             cb = __builtin_fmaf(blue, w, cb);
-            const float stopped = keep ? nT : -__builtin_fabsf(T);
-            T = hit ? stopped : T;
-            if constexpr (SAVE) last = use ? start - range.x + (off >> 4) + 1 : last;
+            if constexpr (SAVE) {
+                const float stopped = keep ? nT : -__builtin_fabsf(T);
+                T = hit ? stopped : T;
+                last = use ? start - range.x + (off >> 4) + 1 : last;
+            } else {
+                T = hit ? nT : T;
+            }
         };
         auto listAt = [&](int i) -> uint32_t { return CULL ? sList[wave][i] : static_cast<uint32_t>(i) * 16u; };
         if (count > 0) {
@@ -801,7 +809,7 @@ __device__ inline void blendTile(int tileX, int tileY, uint2 range, const uint32
                 const uint32_t jNextB = listAt(i + 3);
                 composite(aB, bB, cB, jB);
                 jB = jNextB;
-                if ((i & 7) == 6 && __ballot(T > 0.0f) == 0) break;
+                if ((i & 7) == 6 && __ballot(live(T)) == 0) break;
             }
         }
     }
