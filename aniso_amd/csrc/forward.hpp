@@ -9,20 +9,6 @@
 
 namespace ans {
 
-// Clears the per-frame control block and the tile ranges, and bumps the epoch.
-static __global__ __launch_bounds__(256) void k_initFrame(FrameCtl* __restrict__ ctl, uint2* __restrict__ ranges,
-                                                   uint32_t tiles) {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i == 0) {
-        ctl->epoch += 1;
-        ctl->visible = 0;
-        ctl->pairs = 0;
-        ctl->overflow = 0;
-        ctl->big = 0;
-    }
-    if (i < tiles) ranges[i] = make_uint2(0, 0);
-}
-
 struct F3 {
     float x, y, z;
 };
@@ -306,8 +292,19 @@ __global__ __launch_bounds__(256) void k_preprocess(int n, const float* __restri
                                                     const float* __restrict__ opacity, const float* __restrict__ sh,
                                                     CamParams cam, SplatRec* __restrict__ rec,
                                                     uint32_t* __restrict__ tilesTouched,
-                                                    uint32_t* __restrict__ depthKey) {
+                                                    uint32_t* __restrict__ depthKey, FrameCtl* __restrict__ ctl,
+                                                    uint2* __restrict__ ranges, uint32_t tiles) {
     const int i = blockIdx.x * 256 + threadIdx.x;
+    // The frame starts here: its control block (all but sticky) and tile ranges are cleared by
+    // the first max(n, tiles) lanes; no later kernel of the frame runs before this one ends.
+    if (i == 0) {
+        ctl->epoch += 1;
+        ctl->visible = 0;
+        ctl->pairs = 0;
+        ctl->overflow = 0;
+        ctl->big = 0;
+    }
+    if (static_cast<uint32_t>(i) < tiles) ranges[i] = make_uint2(0, 0);
     if (i >= n) return;
     GaussProj g;
     const float* shBase = PLANAR ? sh + i : sh + static_cast<size_t>(i) * cam.shCoeffs * 3;

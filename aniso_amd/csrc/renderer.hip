@@ -157,19 +157,23 @@ void Engine::enqueueCount(FrameState& f, const SceneDev& s, const CamParams& cp,
     evPending = std::min(evPending + 1, kEventRing);
 
     hipCheck(hipEventRecord(E[0], stream), "hipEventRecord");
-    k_initFrame<<<(std::max<uint32_t>(tiles, 1) + 255) / 256, 256, 0, stream>>>(ctl, f.ranges.p, tiles);
-    if (n > 0) {
-        const uint32_t grid = (n + 255) / 256;
+    {
+        // The preprocess also clears the frame's control block and tile ranges.
+        const uint32_t grid = (std::max<uint32_t>(std::max<uint32_t>(n, tiles), 1) + 255) / 256;
         if (s.raw)
             k_preprocess<false, true><<<grid, 256, 0, stream>>>(static_cast<int>(n), s.pos, s.scale, s.rot, s.opacity,
-                                                                s.sh, cp, f.rec.p, f.tilesTouched.p, depthKey.p);
+                                                                s.sh, cp, f.rec.p, f.tilesTouched.p, depthKey.p, ctl,
+                                                                f.ranges.p, tiles);
         else if (s.shPlanar)
             k_preprocess<true, false><<<grid, 256, 0, stream>>>(static_cast<int>(n), s.pos, s.scale, s.rot, s.opacity,
-                                                                s.sh, cp, f.rec.p, f.tilesTouched.p, depthKey.p);
+                                                                s.sh, cp, f.rec.p, f.tilesTouched.p, depthKey.p, ctl,
+                                                                f.ranges.p, tiles);
         else
             k_preprocess<false, false><<<grid, 256, 0, stream>>>(static_cast<int>(n), s.pos, s.scale, s.rot,
                                                                  s.opacity, s.sh, cp, f.rec.p, f.tilesTouched.p,
-                                                                 depthKey.p);
+                                                                 depthKey.p, ctl, f.ranges.p, tiles);
+    }
+    if (n > 0) {
         // Depth sort: four stable 8-bit passes over all Gaussians (culled ones carry the largest
         // key); ties keep ascending Gaussian index.
         const uint32_t blocks = (n + kSortBlock * kDepthItems - 1) / (kSortBlock * kDepthItems);
