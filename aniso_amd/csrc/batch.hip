@@ -51,7 +51,6 @@ struct ViewBatch {
     DevBuf<SplatRec> rec;
     DevBuf<uint32_t> tilesTouched, depthKey, order, kA, vA, kB, vB, ttByRank;
     DevBuf<uint2> rectByRank;
-    DevBuf<uint4> bigPairs;
     DevBuf<uint32_t> hist, rowTotal, blockPairs, blockVisible;
     DevBuf<uint32_t> tileA, gaussA, tileB, gaussB;
     DevBuf<uint2> ranges;
@@ -155,7 +154,7 @@ void reserveArena(ViewBatch& B, const Layout& L) {
     moved |= grow(B.rec, vn);
     moved |= grow(B.tilesTouched, vn) | grow(B.depthKey, vn) | grow(B.order, vn);
     moved |= grow(B.kA, vn) | grow(B.vA, vn) | grow(B.kB, vn) | grow(B.vB, vn);
-    moved |= grow(B.ttByRank, vn) | grow(B.rectByRank, vn) | grow(B.bigPairs, vn);
+    moved |= grow(B.ttByRank, vn) | grow(B.rectByRank, vn);
     const size_t histD = V * kRadix * size_t(L.blocksD), histP = V * kRadix * size_t(L.blocksP),
                  histI = size_t(kRadix) * L.blocksI;
     moved |= grow(B.hist, std::max({histD, histP, histI}));
@@ -238,9 +237,7 @@ const uint32_t* enqueueBatch(ViewBatch& B, const SceneDev& s, const Layout& L, h
         k_scanPairs<<<dim3(1, V), kDupBlock, 0, st>>>(B.blockPairs.p, B.blockVisible.p, L.blocksDup, cap, ctl,
                                                       L.blocksDup);
         k_expandPairs<<<gDup, kDupBlock, 0, st>>>(B.order.p, B.ttByRank.p, B.rectByRank.p, n, 0, B.blockPairs.p,
-                                                  B.blockVisible.p, B.tileA.p, B.gaussA.p, ctl, B.bigPairs.p, n,
-                                                  L.blocksDup, cap, tilesX);
-        k_expandBig<<<dim3(128, V), kBigBlock, 0, st>>>(B.bigPairs.p, 0, B.tileA.p, B.gaussA.p, ctl, n, cap, tilesX);
+                                                  B.tileA.p, B.gaussA.p, ctl, n, L.blocksDup, cap, tilesX);
     }
     // Stable sort of each view's pairs by tile id.
     uint32_t* tk[2] = {B.tileA.p, B.tileB.p};

@@ -43,8 +43,7 @@ struct FrameCtl {
     uint32_t pairs;       // Gaussian-tile pairs (sum of tiles touched)
     uint32_t overflow;    // 1 if pairs exceeded the pair buffers' capacity
     uint32_t sticky;      // overflow of any frame since the host last cleared it (not cleared by init)
-    uint32_t big;         // Gaussians whose pairs the wide expansion writes
-    uint32_t pad[2];
+    uint32_t pad[3];
 };
 
 // Orders LDS accesses of different lanes of one wavefront: writes before the call are visible

@@ -96,11 +96,10 @@ struct Engine {
     DevBuf<uint32_t> depthKey, sortKA, sortVA, sortKB, sortVB;
     DevBuf<uint32_t> ttByRank;      // tile counts in depth order
     DevBuf<uint2> rectByRank;       // tile rects in depth order
-    DevBuf<uint4> bigPairs;         // Gaussians with many tiles: first slot, index, rect
     DevBuf<uint32_t> hist;          // per-block digit histograms, scanned in place
     DevBuf<uint32_t> rowTotal;      // per-digit totals of one pass
     DevBuf<uint32_t> blockPairs;    // pairs per expansion block, scanned in place
-    DevBuf<uint32_t> blockVisible;  // per expansion block: visible | wide << 16, then wide scanned
+    DevBuf<uint32_t> blockVisible;  // visible Gaussians per expansion block
     DevBuf<float> partial;          // backward: per-pair partial gradients, 12-float records
     DevBuf<float> splatGrad;        // backward: per-Gaussian splat-record gradients, 9 floats
     DevBuf<uint32_t> slotStart;     // backward: first pair slot per Gaussian, index order

@@ -302,7 +302,6 @@ __global__ __launch_bounds__(256) void k_preprocess(int n, const float* __restri
         ctl->visible = 0;
         ctl->pairs = 0;
         ctl->overflow = 0;
-        ctl->big = 0;
     }
     if (static_cast<uint32_t>(i) < tiles) ranges[i] = make_uint2(0, 0);
     if (i >= n) return;
@@ -323,11 +322,10 @@ constexpr int kDupTile = kDupBlock;                  // depth ranks per block
 // totals by view * blockStride, pair arrays by view * pairStride and use ctl[view]; a single frame
 // is gridDim.y = 1.
 
-// Gaussians with more tiles than this are expanded one wavefront each (k_expandBig).
+// Gaussians with more tiles than this are expanded by their whole wavefront, one at a time.
 constexpr uint32_t kBigPairs = 32;
 
-// Pairs, visible Gaussians and Gaussians with more than kBigPairs tiles per block of 256 depth
-// ranks (blockVisible = visible | wide << 16). Also gathers each rank's tile count
+// Pairs and visible Gaussians per block of 256 depth ranks. Also gathers each rank's tile count
 // and tile rect (one line of its splat record) into depth order (ttByRank, rectByRank; rect left
 // unwritten for culled ranks), so that the expansion reads them with coalesced loads.
 static __global__ __launch_bounds__(kDupBlock) void k_pairTotals(const uint32_t* __restrict__ sortedIdx,
@@ -353,20 +351,18 @@ static __global__ __launch_bounds__(kDupBlock) void k_pairTotals(const uint32_t*
         ttByRank[r] = c;
         if (c) rectByRank[r] = make_uint2(m.x, m.y);
     }
-    uint32_t total, visible;
+    uint32_t total;
     blockExclusiveScan<kDupBlock>(c, sTmp, &total);
-    blockExclusiveScan<kDupBlock>(c ? 1u : 0u, sTmp, &visible);
-    const uint32_t wide = static_cast<uint32_t>(__syncthreads_count(c > kBigPairs));
+    const uint32_t visible = static_cast<uint32_t>(__syncthreads_count(c != 0));
     if (threadIdx.x == 0) {
         blockPairs[blockIdx.x] = total;
-        blockVisible[blockIdx.x] = visible | (wide << 16);
+        blockVisible[blockIdx.x] = visible;
     }
 }
 
-// One workgroup: exclusive scans of the per-block pair counts and wide-Gaussian counts (the
-// latter replacing blockVisible); publishes the pair total (the sum gpu_render.cu:324-329 reads
-// back to the host), the visible and wide counts, and flags a pair total above the pair buffers'
-// capacity.
+// One workgroup: exclusive scan of the per-block pair counts; publishes the pair total (the sum
+// gpu_render.cu:324-329 reads back to the host) and the visible count, and flags a pair total
+// above the pair buffers' capacity.
 static __global__ __launch_bounds__(kDupBlock) void k_scanPairs(uint32_t* __restrict__ blockPairs,
                                                                 uint32_t* __restrict__ blockVisible,
                                                                 uint32_t blocks, uint32_t cap,
@@ -377,41 +373,31 @@ static __global__ __launch_bounds__(kDupBlock) void k_scanPairs(uint32_t* __rest
     ctl += blockIdx.y;
     constexpr int kPer = 8;
     uint64_t carry = 0;
-    uint32_t visible = 0, wideCarry = 0;
+    uint32_t visible = 0;
     for (uint32_t base = 0; base < blocks; base += kDupBlock * kPer) {
         const uint32_t first = base + threadIdx.x * kPer;
-        uint32_t v[kPer], wd[kPer];
-        uint32_t sum = 0, vis = 0, wsum = 0;
+        uint32_t v[kPer];
+        uint32_t sum = 0, vis = 0;
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const bool in = first + k < blocks;
             v[k] = in ? blockPairs[first + k] : 0u;
-            const uint32_t packed = in ? blockVisible[first + k] : 0u;
-            vis += packed & 0xffffu;
-            wd[k] = packed >> 16;
-            wsum += wd[k];
+            vis += in ? blockVisible[first + k] : 0u;
             sum += v[k];
         }
-        uint32_t total, visTotal, wideTotal;
+        uint32_t total, visTotal;
         uint32_t run = static_cast<uint32_t>(carry) + blockExclusiveScan<kDupBlock>(sum, sTmp, &total);
         blockExclusiveScan<kDupBlock>(vis, sTmp, &visTotal);
-        uint32_t wrun = wideCarry + blockExclusiveScan<kDupBlock>(wsum, sTmp, &wideTotal);
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
-            if (first + k < blocks) {
-                blockPairs[first + k] = run;
-                blockVisible[first + k] = wrun;
-            }
+            if (first + k < blocks) blockPairs[first + k] = run;
             run += v[k];
-            wrun += wd[k];
         }
         carry += total;
         visible += visTotal;
-        wideCarry += wideTotal;
     }
     if (threadIdx.x == 0) {
         ctl->visible = visible;
-        ctl->big = wideCarry;
         ctl->pairs = carry > 0xffffffffull ? 0xffffffffu : static_cast<uint32_t>(carry);
         if (carry > cap) {
             ctl->overflow = 1;
@@ -435,22 +421,41 @@ __device__ inline int waveInclusiveMax(int v) {
 // pairs (tile id, Gaussian index) contiguously, each Gaussian's tiles row-major (the order
 // gpu_render.cu:155-156 writes them), 64 pairs per step with one pair per lane. A step finds
 // the Gaussian owning each slot by marking the slots where Gaussians start (LDS head flags) and
-// a wavefront max-scan. Gaussians with more than kBigPairs tiles would serialize their wavefront
-// (depth order puts the nearest, largest splats next to each other), so they are only given
-// their slots here and listed in `big`, and k_expandBig writes their pairs, one wavefront per
-// Gaussian; the head-flag walk runs over the remaining Gaussians' slots.
+// a wavefront max-scan. Gaussians with more than kBigPairs tiles would make that walk spend many
+// steps on one owner (depth order puts the nearest, largest splats next to each other), so the
+// walk covers only the other Gaussians' slots, and afterwards the wavefront writes each of its
+// wide Gaussians with all 64 lanes (writeRectPairs) into the slots reserved for it.
 
-// tilesXView (batched views, else null) holds each view's tile-grid width; big entries are
-// {first slot, Gaussian, rect.x, rect.y} at view * gaussStride, in rank order from blockBig (the
-// scanned wide counts).
+// The pairs of one Gaussian with a w x h tile rect at (x0, y0), written by a whole wavefront from
+// slot `first`: lane l writes tiles l, l + 64, ... row-major, carrying (row, col) instead of
+// dividing.
+__device__ inline void writeRectPairs(uint32_t first, uint32_t g, uint32_t rectLo, uint32_t rectHi, int tilesX,
+                                      uint32_t* __restrict__ tileOut, uint32_t* __restrict__ gaussOut) {
+    const uint32_t lane = __lane_id();
+    const uint32_t x0 = rectLo & 0xffffu, y0 = rectLo >> 16;
+    const uint32_t w = (rectHi & 0xffffu) - x0 + 1, h = (rectHi >> 16) - y0 + 1;
+    uint32_t slot = first + lane;
+    uint32_t row = lane / w, col = lane - (lane / w) * w;
+    const uint32_t stepRows = kWave / w, stepCols = kWave - stepRows * w;
+    for (uint32_t k = lane; k < w * h; k += kWave, slot += kWave) {
+        tileOut[slot] = (y0 + row) * static_cast<uint32_t>(tilesX) + x0 + col;
+        gaussOut[slot] = g;
+        row += stepRows;
+        col += stepCols;
+        if (col >= w) {
+            col -= w;
+            row += 1;
+        }
+    }
+}
+
+// tilesXView (batched views, else null) holds each view's tile-grid width.
 static __global__ __launch_bounds__(kDupBlock) void k_expandPairs(
     const uint32_t* __restrict__ sortedIdx, const uint32_t* __restrict__ ttByRank,
     const uint2* __restrict__ rectByRank, uint32_t n, int tilesX, const uint32_t* __restrict__ blockBase,
-    const uint32_t* __restrict__ blockBig, uint32_t* __restrict__ tileOut, uint32_t* __restrict__ gaussOut,
-    const FrameCtl* __restrict__ ctl, uint4* __restrict__ big, uint32_t gaussStride, uint32_t blockStride,
-    uint32_t pairStride, const int* __restrict__ tilesXView) {
+    uint32_t* __restrict__ tileOut, uint32_t* __restrict__ gaussOut, const FrameCtl* __restrict__ ctl,
+    uint32_t gaussStride, uint32_t blockStride, uint32_t pairStride, const int* __restrict__ tilesXView) {
     __shared__ uint32_t sWave[kDupBlock / kWave];
-    __shared__ uint32_t sWide[kDupBlock / kWave];
     __shared__ int sHead[kDupBlock / kWave][kWave];
     {
         const uint32_t view = blockIdx.y;
@@ -459,9 +464,7 @@ static __global__ __launch_bounds__(kDupBlock) void k_expandPairs(
         sortedIdx += go;
         ttByRank += go;
         rectByRank += go;
-        big += go;
         blockBase += static_cast<size_t>(view) * blockStride;
-        blockBig += static_cast<size_t>(view) * blockStride;
         tileOut += po;
         gaussOut += po;
         if (tilesXView) tilesX = tilesXView[view];
@@ -487,18 +490,10 @@ static __global__ __launch_bounds__(kDupBlock) void k_expandPairs(
     const uint32_t waveTotal = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), kWave - 1));
     const uint32_t waveSmall = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(inclS), kWave - 1));
     const uint64_t bigLanes = __ballot(isBig);
-    if (lane == 0) {
-        sWave[wave] = waveTotal;
-        sWide[wave] = static_cast<uint32_t>(__popcll(bigLanes));
-    }
+    if (lane == 0) sWave[wave] = waveTotal;
     __syncthreads();
     uint32_t out = blockBase[blockIdx.x];
-    uint32_t wideBase = blockBig[blockIdx.x];
-    for (int w = 0; w < wave; ++w) {
-        out += sWave[w];
-        wideBase += sWide[w];
-    }
-    if (isBig) big[wideBase + __popcll(bigLanes & lanesBelow())] = make_uint4(out + excl, g, rp.x, rp.y);
+    for (int w = 0; w < wave; ++w) out += sWave[w];
 
     const uint32_t x0 = rp.x & 0xffffu, y0 = rp.x >> 16;
     const uint32_t w = (rp.y & 0xffffu) - x0 + 1;
@@ -535,50 +530,20 @@ static __global__ __launch_bounds__(kDupBlock) void k_expandPairs(
             gaussOut[slot] = oG;
         }
     }
-}
-
-// Pairs of the Gaussians k_expandPairs listed in `big`: one wavefront per Gaussian (grid-strided
-// over the list), lane l writing tiles l, l + 64, ... of its rect, row-major, from its first slot.
-constexpr int kBigBlock = 256;
-static __global__ __launch_bounds__(kBigBlock) void k_expandBig(const uint4* __restrict__ big, int tilesX,
-                                                                uint32_t* __restrict__ tileOut,
-                                                                uint32_t* __restrict__ gaussOut,
-                                                                const FrameCtl* __restrict__ ctl, uint32_t gaussStride,
-                                                                uint32_t pairStride,
-                                                                const int* __restrict__ tilesXView) {
-    const uint32_t view = blockIdx.y;
-    ctl += view;
-    if (ctl->overflow) return;
-    big += static_cast<size_t>(view) * gaussStride;
-    tileOut += static_cast<size_t>(view) * pairStride;
-    gaussOut += static_cast<size_t>(view) * pairStride;
-    if (tilesXView) tilesX = tilesXView[view];
-    const uint32_t count = ctl->big;
-    const uint32_t lane = __lane_id();
-    const uint32_t wavesPerBlock = kBigBlock / kWave;
-    for (uint32_t e = blockIdx.x * wavesPerBlock + threadIdx.x / kWave; e < count; e += gridDim.x * wavesPerBlock) {
-        const uint4 b = big[e];
-        const uint32_t x0 = b.z & 0xffffu, y0 = b.z >> 16;
-        const uint32_t w = (b.w & 0xffffu) - x0 + 1, h = (b.w >> 16) - y0 + 1;
-        uint32_t slot = b.x + lane;
-        // Walk the rect row-major 64 tiles at a time, carrying (row, col) instead of dividing.
-        uint32_t row = lane / w, col = lane - (lane / w) * w;
-        const uint32_t stepRows = kWave / w, stepCols = kWave - stepRows * w;
-        for (uint32_t k = lane; k < w * h; k += kWave, slot += kWave) {
-            tileOut[slot] = (y0 + row) * static_cast<uint32_t>(tilesX) + x0 + col;
-            gaussOut[slot] = b.y;
-            row += stepRows;
-            col += stepCols;
-            if (col >= w) {
-                col -= w;
-                row += 1;
-            }
-        }
+    // The wavefront's wide Gaussians, one at a time with all 64 lanes, into the slots the scan
+    // reserved for them in depth order.
+    for (uint64_t wide = bigLanes; wide != 0; wide &= wide - 1) {
+        const int L = __builtin_ctzll(wide);
+        writeRectPairs(out + static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(excl), L)),
+                       static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(g), L)),
+                       static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(rp.x), L)),
+                       static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(rp.y), L)), tilesX, tileOut,
+                       gaussOut);
     }
 }
 
 // Start and end of each tile's run in the sorted pairs (gpu_render.cu:164-177); ranges were
-// cleared to [0, 0) by the frame init. Grid-stride over groups of four pairs.
+// cleared to [0, 0) by the preprocess. Grid-stride over groups of four pairs.
 static __global__ __launch_bounds__(256) void k_tileRanges(const uint32_t* __restrict__ sortedTile,
                                                            const FrameCtl* __restrict__ ctl,
                                                            uint2* __restrict__ ranges, uint32_t pairStride,

@@ -120,7 +120,7 @@ Engine::~Engine() {
 void Engine::reserveScratch(size_t n, size_t pairCap) {
     const size_t m = std::max<size_t>(n, 1);
     depthKey.reserve(m); sortKA.reserve(m); sortVA.reserve(m); sortKB.reserve(m); sortVB.reserve(m);
-    ttByRank.reserve(m); rectByRank.reserve(m); bigPairs.reserve(m);
+    ttByRank.reserve(m); rectByRank.reserve(m);
     const size_t blocksDepth = (n + kSortBlock * kDepthItems - 1) / (kSortBlock * kDepthItems);
     const size_t blocksDup = (n + kDupTile - 1) / kDupTile;
     const size_t blocksPair = (pairCap + kSortBlock * kPairItems - 1) / (kSortBlock * kPairItems);
@@ -220,9 +220,8 @@ void Engine::enqueueRest(FrameState& f, uint8_t* out8, float* outf) {
         const uint32_t blocksDup = (n + kDupTile - 1) / kDupTile;
         const uint32_t cap = static_cast<uint32_t>(f.pairCap);
         k_expandPairs<<<blocksDup, kDupBlock, 0, stream>>>(f.order.p, ttByRank.p, rectByRank.p, n, cp.tilesX,
-                                                           blockPairs.p, blockVisible.p, f.tileA.p, f.gaussA.p, ctl,
-                                                           bigPairs.p, 0, 0, 0, nullptr);
-        k_expandBig<<<512, kBigBlock, 0, stream>>>(bigPairs.p, cp.tilesX, f.tileA.p, f.gaussA.p, ctl, 0, 0, nullptr);
+                                                           blockPairs.p, f.tileA.p, f.gaussA.p, ctl, 0, 0, 0,
+                                                           nullptr);
         const uint32_t blocksPair =
             static_cast<uint32_t>((f.pairCap + kSortBlock * kPairItems - 1) / (kSortBlock * kPairItems));
         uint32_t* tk[2] = {f.tileA.p, f.tileB.p};

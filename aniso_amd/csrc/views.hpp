@@ -17,7 +17,6 @@ static __global__ __launch_bounds__(256) void k_initViews(FrameCtl* __restrict__
         ctl[i].visible = 0;
         ctl[i].pairs = 0;
         ctl[i].overflow = 0;
-        ctl[i].big = 0;
     }
     if (i < rangeCount) ranges[i] = make_uint2(0, 0);
 }
