@@ -285,32 +285,35 @@ __global__ __launch_bounds__(TW* TH) void k_blendBackward(
     }
 }
 
-// d basis_k / d(x, y, z) for the basis of shBasis (product rule on each polynomial).
-__host__ __device__ inline void shBasisGrad(int degree, float x, float y, float z, float g[16][3]) {
-#pragma unroll
-    for (int k = 0; k < 16; ++k) g[k][0] = g[k][1] = g[k][2] = 0.0f;
-    if (degree >= 1) {
-        g[1][1] = -kShC1;
-        g[2][2] = kShC1;
-        g[3][0] = -kShC1;
-    }
-    if (degree >= 2) {
-        g[4][0] = kShC2[0] * y; g[4][1] = kShC2[0] * x;
-        g[5][1] = kShC2[1] * z; g[5][2] = kShC2[1] * y;
-        g[6][0] = kShC2[2] * (-2 * x); g[6][1] = kShC2[2] * (-2 * y); g[6][2] = kShC2[2] * (4 * z);
-        g[7][0] = kShC2[3] * z; g[7][2] = kShC2[3] * x;
-        g[8][0] = kShC2[4] * (2 * x); g[8][1] = kShC2[4] * (-2 * y);
-        if (degree >= 3) {
-            const float xx = x * x, yy = y * y, zz = z * z;
-            g[9][0] = kShC3[0] * y * 6 * x; g[9][1] = kShC3[0] * (3 * xx - 3 * yy);
-            g[10][0] = kShC3[1] * y * z; g[10][1] = kShC3[1] * x * z; g[10][2] = kShC3[1] * x * y;
-            g[11][0] = kShC3[2] * y * (-2 * x); g[11][1] = kShC3[2] * (4 * zz - xx - 3 * yy); g[11][2] = kShC3[2] * y * 8 * z;
-            g[12][0] = kShC3[3] * z * (-6 * x); g[12][1] = kShC3[3] * z * (-6 * y);
-            g[12][2] = kShC3[3] * (6 * zz - 3 * xx - 3 * yy);
-            g[13][0] = kShC3[4] * (4 * zz - 3 * xx - yy); g[13][1] = kShC3[4] * x * (-2 * y); g[13][2] = kShC3[4] * x * 8 * z;
-            g[14][0] = kShC3[5] * z * 2 * x; g[14][1] = kShC3[5] * z * (-2 * y); g[14][2] = kShC3[5] * (xx - yy);
-            g[15][0] = kShC3[6] * (3 * xx - 3 * yy); g[15][1] = kShC3[6] * x * (-6 * y);
-        }
+// Row k of the SH basis Jacobian: d basis_k / d (x, y, z) for the real SH of sh.cpp:18-58
+// (rows past the degree the caller evaluates are never asked for). With k a constant in an
+// unrolled loop the switch folds away, so the 16 x 3 Jacobian is never held at once.
+__host__ __device__ inline void shBasisGradRow(int k, float x, float y, float z, float g[3]) {
+    const float xx = x * x, yy = y * y, zz = z * z;
+    g[0] = g[1] = g[2] = 0.0f;
+    switch (k) {
+        case 1: g[1] = -kShC1; break;
+        case 2: g[2] = kShC1; break;
+        case 3: g[0] = -kShC1; break;
+        case 4: g[0] = kShC2[0] * y; g[1] = kShC2[0] * x; break;
+        case 5: g[1] = kShC2[1] * z; g[2] = kShC2[1] * y; break;
+        case 6: g[0] = kShC2[2] * (-2 * x); g[1] = kShC2[2] * (-2 * y); g[2] = kShC2[2] * (4 * z); break;
+        case 7: g[0] = kShC2[3] * z; g[2] = kShC2[3] * x; break;
+        case 8: g[0] = kShC2[4] * (2 * x); g[1] = kShC2[4] * (-2 * y); break;
+        case 9: g[0] = kShC3[0] * y * 6 * x; g[1] = kShC3[0] * (3 * xx - 3 * yy); break;
+        case 10: g[0] = kShC3[1] * y * z; g[1] = kShC3[1] * x * z; g[2] = kShC3[1] * x * y; break;
+        case 11:
+            g[0] = kShC3[2] * y * (-2 * x); g[1] = kShC3[2] * (4 * zz - xx - 3 * yy); g[2] = kShC3[2] * y * 8 * z;
+            break;
+        case 12:
+            g[0] = kShC3[3] * z * (-6 * x); g[1] = kShC3[3] * z * (-6 * y); g[2] = kShC3[3] * (6 * zz - 3 * xx - 3 * yy);
+            break;
+        case 13:
+            g[0] = kShC3[4] * (4 * zz - 3 * xx - yy); g[1] = kShC3[4] * x * (-2 * y); g[2] = kShC3[4] * x * 8 * z;
+            break;
+        case 14: g[0] = kShC3[5] * z * 2 * x; g[1] = kShC3[5] * z * (-2 * y); g[2] = kShC3[5] * (xx - yy); break;
+        case 15: g[0] = kShC3[6] * (3 * xx - 3 * yy); g[1] = kShC3[6] * x * (-6 * y); break;
+        default: break;
     }
 }
 
@@ -386,9 +389,8 @@ __host__ __device__ inline void gaussAdjoint(const GaussProj& g, const CamParams
     // SH colour, clamped at zero: d sh_k = basis_k d rgb; d dir through the basis.
     const float grgb[3] = {g.rgbRaw.x > 0.0f ? gs[6] : 0.0f, g.rgbRaw.y > 0.0f ? gs[7] : 0.0f,
                            g.rgbRaw.z > 0.0f ? gs[8] : 0.0f};
-    float bgrad[16][3];
-    shBasisGrad(cam.shDegree, g.dir[0], g.dir[1], g.dir[2], bgrad);
     float gdir[3] = {0, 0, 0};
+#pragma unroll
     for (int k = 0; k < 16; ++k) {
         if (k < g.shCount) {
             float gbk = 0.0f;
@@ -396,7 +398,9 @@ __host__ __device__ inline void gaussAdjoint(const GaussProj& g, const CamParams
                 gbk += grgb[c] * shi[k * 3 + c];  // read before the write below: gsh may alias shi
                 gsh[k * 3 + c] = grgb[c] * g.basis[k];
             }
-            for (int j = 0; j < 3; ++j) gdir[j] += gbk * bgrad[k][j];
+            float row[3];
+            shBasisGradRow(k, g.dir[0], g.dir[1], g.dir[2], row);
+            for (int j = 0; j < 3; ++j) gdir[j] += gbk * row[j];
         }
     }
     for (int j = g.shCount * 3; j < cam.shCoeffs * 3; ++j) gsh[j] = 0.0f;

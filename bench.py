@@ -121,7 +121,8 @@ def main():
     host_ms = (time.perf_counter() - t0) * 1e3 / host_steps
 
     extra = {"host_output": {"ms_per_frame": round(host_ms, 4), "steps": host_steps,
-                             "includes": "synchronous ans_render: frame, pair-count readback, RGB8 copy to host"}}
+                             "includes": "synchronous ans_render: frame, pair-count readback, RGB8 copy to host"},
+             "single_frame": bench_single_frames(r, L, c_cam, c_opt, dev_img)}
     if not args.skip_extra:
         extra["fwd_bwd"] = bench_fwd_bwd(raw, cam, max(5, args.steps // 4), args.warmup)
         extra["fwd_clustered_2.2M"] = bench_forward("clustered-2.2M-1280x720", max(10, args.steps // 2), args.warmup)
@@ -200,6 +201,40 @@ def measured_traffic(kernel, workload, field="traffic_bytes_per_launch"):
 SIMDS = 1024
 VALU_CYCLES_PER_INST = 4
 CLOCK_HZ = 2.4e9
+
+
+def bench_single_frames(r, L, c_cam, c_opt, dev_img, warm=25, cold=10):
+    """Frames rendered one at a time, each waited for: the median of `warm` frames after one
+    warm-up, and the median of `cold` frames each preceded by writing 1 GiB of scratch (more than
+    the last-level cache, so the scene and the frame's buffers come from HBM). Wall time per frame
+    and the frame's event-timed GPU time."""
+    import torch
+
+    from aniso_amd import _lib
+
+    def one():
+        t0 = time.perf_counter()
+        _lib.check(L.ans_render_async(r._h, c_cam, c_opt, dev_img.data_ptr(), None))
+        st = _lib.Stats_()
+        _lib.check(L.ans_render_wait(r._h, ctypes.byref(st)))
+        wall = (time.perf_counter() - t0) * 1e3
+        return wall, st.project_ms + st.sort_ms + st.blend_ms
+
+    one()
+    warm_runs = [one() for _ in range(warm)]
+    scratch = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")
+    cold_runs = []
+    for k in range(cold):
+        scratch.fill_(k & 0xff)
+        torch.cuda.synchronize()
+        cold_runs.append(one())
+    del scratch
+    med = lambda xs: round(statistics.median(xs), 4)
+    return {"warm_median_ms": med([w for w, _ in warm_runs]), "warm_gpu_median_ms": med([g for _, g in warm_runs]),
+            "cold_median_ms": med([w for w, _ in cold_runs]), "cold_gpu_median_ms": med([g for _, g in cold_runs]),
+            "warm_runs": warm, "cold_runs": cold,
+            "cold_flush": "1 GiB device write before each cold frame (evicts the last-level cache)",
+            "includes": "ans_render_async + ans_render_wait per frame; gpu = event-timed project + sort + blend"}
 
 
 def valu_bound(instructions, kernel_ms):
