@@ -43,7 +43,16 @@ struct FrameCtl {
     uint32_t pairs;       // Gaussian-tile pairs (sum of tiles touched)
     uint32_t overflow;    // 1 if pairs exceeded the pair buffers' capacity
     uint32_t sticky;      // overflow of any frame since the host last cleared it (not cleared by init)
-    uint32_t pad[3];
+    uint32_t finished;    // blend workgroups that have ended (stage timing)
+    uint32_t pad[2];
+};
+
+// Stage timestamps of one frame, read from the GPU's constant-rate wall clock (s_memrealtime) by
+// the kernels themselves: [0] preprocess start, [1] pair counting start (end of the depth sort),
+// [2] blend start, [3] blend end. Stamps taken in kernels cost nothing between launches; events
+// recorded between two kernels separate them by several microseconds.
+struct FrameStamps {
+    uint64_t t[4];
 };
 
 // Orders LDS accesses of different lanes of one wavefront: writes before the call are visible

@@ -85,13 +85,16 @@ struct Engine {
     int device = 0;
     hipStream_t ownStream = nullptr;
     hipStream_t stream = nullptr;
-    // Stage events of the last kEventRing frames (start, after project, after sort, end); stats
+    // Stage timestamps of the last kStampRing frames, written by the frames' kernels; stats
     // average the frames enqueued since the previous readout.
-    static constexpr int kEventRing = 64;
-    hipEvent_t ev[kEventRing][4] = {};
-    int evNext = 0, evPending = 0;
-    hipEvent_t* curEv = nullptr;  // events of the frame being enqueued
-    FrameCtl* ctlHost = nullptr;  // pinned
+    static constexpr int kStampRing = 64;
+    DevBuf<FrameStamps> stamps;
+    FrameStamps* stampsHost = nullptr;  // pinned
+    bool stampNoPairs[kStampRing] = {};
+    int stampNext = 0, stampPending = 0;
+    FrameStamps* curStamps = nullptr;  // stamps of the frame being enqueued
+    double clockKHz = 0;               // wall clock rate
+    FrameCtl* ctlHost = nullptr;       // pinned
 
     DevBuf<uint32_t> depthKey, sortKA, sortVA, sortKB, sortVB;
     DevBuf<uint32_t> ttByRank;      // tile counts in depth order
@@ -115,11 +118,12 @@ struct Engine {
                       float* outf);
     void enqueueCount(FrameState& f, const SceneDev& s, const CamParams& cp, const ans_options& o);
     void enqueueRest(FrameState& f, uint8_t* out8, float* outf);
-    void launchBlend(FrameState& f, uint8_t* out8, float* outf);
+    // The blend launch; the kernel writes the blend stage's start and end stamps into st.
+    void launchBlend(FrameState& f, uint8_t* out8, float* outf, FrameStamps* st);
     template <int TW, int TH>
-    void launchBlendT(FrameState& f, bool cull, uint8_t* out8, float* outf);
+    void launchBlendT(FrameState& f, bool cull, uint8_t* out8, float* outf, FrameStamps* st);
     template <int TW, int TH, int BATCH>
-    void launchBlendB(FrameState& f, bool cull, uint8_t* out8, float* outf);
+    void launchBlendB(FrameState& f, bool cull, uint8_t* out8, float* outf, FrameStamps* st);
     // Copies the frame's control block to the host and waits; true if the pair buffers overflowed.
     bool readBack(FrameState& f);
     int renderCounted(FrameState& f, const SceneDev& s, const CamParams& cp, const ans_options& o, uint8_t* out8,

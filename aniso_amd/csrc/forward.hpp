@@ -293,15 +293,19 @@ __global__ __launch_bounds__(256) void k_preprocess(int n, const float* __restri
                                                     CamParams cam, SplatRec* __restrict__ rec,
                                                     uint32_t* __restrict__ tilesTouched,
                                                     uint32_t* __restrict__ depthKey, FrameCtl* __restrict__ ctl,
-                                                    uint2* __restrict__ ranges, uint32_t tiles) {
+                                                    uint2* __restrict__ ranges, uint32_t tiles,
+                                                    FrameStamps* __restrict__ stamps) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     // The frame starts here: its control block (all but sticky) and tile ranges are cleared by
     // the first max(n, tiles) lanes; no later kernel of the frame runs before this one ends.
+    // Workgroup 0 is dispatched first, so its clock reading is the kernel's start.
     if (i == 0) {
+        if (stamps) stamps->t[0] = wall_clock64();
         ctl->epoch += 1;
         ctl->visible = 0;
         ctl->pairs = 0;
         ctl->overflow = 0;
+        ctl->finished = 0;
     }
     if (static_cast<uint32_t>(i) < tiles) ranges[i] = make_uint2(0, 0);
     if (i >= n) return;
@@ -334,8 +338,10 @@ static __global__ __launch_bounds__(kDupBlock) void k_pairTotals(const uint32_t*
                                                                  uint2* __restrict__ rectByRank,
                                                                  uint32_t* __restrict__ blockPairs,
                                                                  uint32_t* __restrict__ blockVisible,
-                                                                 uint32_t gaussStride, uint32_t blockStride) {
+                                                                 uint32_t gaussStride, uint32_t blockStride,
+                                                                 FrameStamps* __restrict__ stamps) {
     __shared__ uint32_t sTmp[kDupBlock / kWave];
+    if (stamps && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) stamps->t[1] = wall_clock64();
     const size_t go = static_cast<size_t>(blockIdx.y) * gaussStride;
     sortedIdx += go;
     rec += go;
@@ -798,11 +804,18 @@ __global__ __launch_bounds__(TW* TH) void k_blend(const uint2* __restrict__ rang
                                                   const uint32_t* __restrict__ sortedGauss,
                                                   const SplatRec* __restrict__ rec, int width, int height, int tilesX,
                                                   uint8_t* __restrict__ out8, float* __restrict__ outf,
-                                                  uint2* __restrict__ pixState, const FrameCtl* __restrict__ ctl) {
-    if (ctl->overflow) return;
+                                                  uint2* __restrict__ pixState, FrameCtl* __restrict__ ctl,
+                                                  FrameStamps* __restrict__ stamps) {
     const int tile = blockIdx.x;
-    blendTile<TW, TH, BATCH, CULL, SAVE>(tile % tilesX, tile / tilesX, ranges[tile], sortedGauss, rec, width, height,
-                                         out8, outf, pixState);
+    if (stamps && tile == 0 && threadIdx.x == 0) stamps->t[2] = wall_clock64();
+    if (!ctl->overflow)
+        blendTile<TW, TH, BATCH, CULL, SAVE>(tile % tilesX, tile / tilesX, ranges[tile], sortedGauss, rec, width,
+                                             height, out8, outf, pixState);
+    if (stamps) {
+        // The workgroup that ends last reads the clock for the kernel's end.
+        __syncthreads();
+        if (threadIdx.x == 0 && atomicAdd(&ctl->finished, 1u) == gridDim.x - 1) stamps->t[3] = wall_clock64();
+    }
 }
 
 }  // namespace ans

@@ -103,17 +103,22 @@ Engine::Engine() {
     hipCheck(hipGetDevice(&device), "hipGetDevice");
     hipCheck(hipStreamCreateWithFlags(&ownStream, hipStreamNonBlocking), "hipStreamCreate");
     stream = ownStream;
-    for (auto& f : ev)
-        for (auto& e : f) hipCheck(hipEventCreate(&e), "hipEventCreate");
+    int khz = 0;
+    hipCheck(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device), "hipDeviceGetAttribute");
+    if (khz <= 0) throw std::runtime_error("device reports no wall clock rate");
+    clockKHz = khz;
     hipCheck(hipHostMalloc(reinterpret_cast<void**>(&ctlHost), sizeof(FrameCtl), hipHostMallocDefault),
              "hipHostMalloc");
+    hipCheck(hipHostMalloc(reinterpret_cast<void**>(&stampsHost), sizeof(FrameStamps) * kStampRing,
+                           hipHostMallocDefault),
+             "hipHostMalloc");
+    stamps.reserve(kStampRing);
+    hipCheck(hipMemset(stamps.p, 0, sizeof(FrameStamps) * kStampRing), "hipMemset");
 }
 
 Engine::~Engine() {
     if (ctlHost) (void)hipHostFree(ctlHost);
-    for (auto& f : ev)
-        for (auto& e : f)
-            if (e) (void)hipEventDestroy(e);
+    if (stampsHost) (void)hipHostFree(stampsHost);
     if (ownStream) (void)hipStreamDestroy(ownStream);
 }
 
@@ -151,27 +156,27 @@ void Engine::enqueueCount(FrameState& f, const SceneDev& s, const CamParams& cp,
     f.cp = cp;
     f.opts = o;
     FrameCtl* ctl = f.ctl.p;
-    hipEvent_t* E = ev[evNext];
-    curEv = E;
-    evNext = (evNext + 1) % kEventRing;
-    evPending = std::min(evPending + 1, kEventRing);
+    FrameStamps* S = stamps.p + stampNext;
+    curStamps = S;
+    stampNoPairs[stampNext] = n == 0;  // no pair-counting kernel: the sort stage is empty
+    stampNext = (stampNext + 1) % kStampRing;
+    stampPending = std::min(stampPending + 1, kStampRing);
 
-    hipCheck(hipEventRecord(E[0], stream), "hipEventRecord");
     {
         // The preprocess also clears the frame's control block and tile ranges.
         const uint32_t grid = (std::max<uint32_t>(std::max<uint32_t>(n, tiles), 1) + 255) / 256;
         if (s.raw)
             k_preprocess<false, true><<<grid, 256, 0, stream>>>(static_cast<int>(n), s.pos, s.scale, s.rot, s.opacity,
                                                                 s.sh, cp, f.rec.p, f.tilesTouched.p, depthKey.p, ctl,
-                                                                f.ranges.p, tiles);
+                                                                f.ranges.p, tiles, S);
         else if (s.shPlanar)
             k_preprocess<true, false><<<grid, 256, 0, stream>>>(static_cast<int>(n), s.pos, s.scale, s.rot, s.opacity,
                                                                 s.sh, cp, f.rec.p, f.tilesTouched.p, depthKey.p, ctl,
-                                                                f.ranges.p, tiles);
+                                                                f.ranges.p, tiles, S);
         else
             k_preprocess<false, false><<<grid, 256, 0, stream>>>(static_cast<int>(n), s.pos, s.scale, s.rot,
                                                                  s.opacity, s.sh, cp, f.rec.p, f.tilesTouched.p,
-                                                                 depthKey.p, ctl, f.ranges.p, tiles);
+                                                                 depthKey.p, ctl, f.ranges.p, tiles, S);
     }
     if (n > 0) {
         // Depth sort: four stable 8-bit passes over all Gaussians (culled ones carry the largest
@@ -192,11 +197,10 @@ void Engine::enqueueCount(FrameState& f, const SceneDev& s, const CamParams& cp,
             vin = vout[pass];
         }
     }
-    hipCheck(hipEventRecord(E[1], stream), "hipEventRecord");
     if (n > 0) {
         const uint32_t blocksDup = (n + kDupTile - 1) / kDupTile;
         k_pairTotals<<<blocksDup, kDupBlock, 0, stream>>>(f.order.p, f.rec.p, n, ttByRank.p, rectByRank.p,
-                                                          blockPairs.p, blockVisible.p, 0, 0);
+                                                          blockPairs.p, blockVisible.p, 0, 0, S);
         k_scanPairs<<<1, kDupBlock, 0, stream>>>(blockPairs.p, blockVisible.p, blocksDup,
                                                  static_cast<uint32_t>(f.pairCap), ctl, 0);
     }
@@ -209,7 +213,6 @@ void Engine::enqueueRest(FrameState& f, uint8_t* out8, float* outf) {
     const uint32_t tiles = static_cast<uint32_t>(f.tiles);
     const CamParams& cp = f.cp;
     FrameCtl* ctl = f.ctl.p;
-    hipEvent_t* E = curEv;
     reserveScratch(n, f.pairCap);
     const int tileBits = std::max(1, bitsFor(tiles - 1));
     const int passes = (tileBits + 7) / 8;
@@ -243,42 +246,40 @@ void Engine::enqueueRest(FrameState& f, uint8_t* out8, float* outf) {
         k_tileRanges<<<static_cast<uint32_t>((f.pairCap + 1023) / 1024), 256, 0, stream>>>(f.sortedTile, ctl,
                                                                                             f.ranges.p, 0, 0);
     }
-    hipCheck(hipEventRecord(E[2], stream), "hipEventRecord");
-    launchBlend(f, out8, outf);
+    launchBlend(f, out8, outf, curStamps);
     hipCheck(hipGetLastError(), "kernel launch");
-    hipCheck(hipEventRecord(E[3], stream), "hipEventRecord");
 }
 
 template <int TW, int TH, int BATCH>
-void Engine::launchBlendB(FrameState& f, bool cull, uint8_t* out8, float* outf) {
+void Engine::launchBlendB(FrameState& f, bool cull, uint8_t* out8, float* outf, FrameStamps* st) {
     const CamParams& cp = f.cp;
     const uint32_t tiles = static_cast<uint32_t>(cp.tilesX) * cp.tilesY;
     if (f.savePixels) {
         // Frames for the backward pass always cull (culling changes no pixel).
         f.pixState.reserve(static_cast<size_t>(cp.width) * cp.height);
         k_blend<TW, TH, BATCH, true, true><<<tiles, TW * TH, 0, stream>>>(
-            f.ranges.p, f.sortedGauss, f.rec.p, cp.width, cp.height, cp.tilesX, out8, outf, f.pixState.p, f.ctl.p);
+            f.ranges.p, f.sortedGauss, f.rec.p, cp.width, cp.height, cp.tilesX, out8, outf, f.pixState.p, f.ctl.p, st);
     } else if (cull) {
         k_blend<TW, TH, BATCH, true, false><<<tiles, TW * TH, 0, stream>>>(
-            f.ranges.p, f.sortedGauss, f.rec.p, cp.width, cp.height, cp.tilesX, out8, outf, nullptr, f.ctl.p);
+            f.ranges.p, f.sortedGauss, f.rec.p, cp.width, cp.height, cp.tilesX, out8, outf, nullptr, f.ctl.p, st);
     } else {
         k_blend<TW, TH, BATCH, false, false><<<tiles, TW * TH, 0, stream>>>(
-            f.ranges.p, f.sortedGauss, f.rec.p, cp.width, cp.height, cp.tilesX, out8, outf, nullptr, f.ctl.p);
+            f.ranges.p, f.sortedGauss, f.rec.p, cp.width, cp.height, cp.tilesX, out8, outf, nullptr, f.ctl.p, st);
     }
 }
 
 template <int TW, int TH>
-void Engine::launchBlendT(FrameState& f, bool cull, uint8_t* out8, float* outf) {
-    if (f.opts.batch_depth == 64) launchBlendB<TW, TH, 64>(f, cull, out8, outf);
-    else if (f.opts.batch_depth == 128) launchBlendB<TW, TH, 128>(f, cull, out8, outf);
-    else launchBlendB<TW, TH, 256>(f, cull, out8, outf);
+void Engine::launchBlendT(FrameState& f, bool cull, uint8_t* out8, float* outf, FrameStamps* st) {
+    if (f.opts.batch_depth == 64) launchBlendB<TW, TH, 64>(f, cull, out8, outf, st);
+    else if (f.opts.batch_depth == 128) launchBlendB<TW, TH, 128>(f, cull, out8, outf, st);
+    else launchBlendB<TW, TH, 256>(f, cull, out8, outf, st);
 }
 
-void Engine::launchBlend(FrameState& f, uint8_t* out8, float* outf) {
+void Engine::launchBlend(FrameState& f, uint8_t* out8, float* outf, FrameStamps* st) {
     const bool cull = !(f.opts.flags & ANS_FLAG_NO_CULL);
-    if (f.opts.tile_w == 8) launchBlendT<8, 8>(f, cull, out8, outf);
-    else if (f.opts.tile_w == 32) launchBlendT<32, 16>(f, cull, out8, outf);
-    else launchBlendT<16, 16>(f, cull, out8, outf);
+    if (f.opts.tile_w == 8) launchBlendT<8, 8>(f, cull, out8, outf, st);
+    else if (f.opts.tile_w == 32) launchBlendT<32, 16>(f, cull, out8, outf, st);
+    else launchBlendT<16, 16>(f, cull, out8, outf, st);
 }
 
 bool Engine::readBack(FrameState& f) {
@@ -300,7 +301,7 @@ bool Engine::readBack(FrameState& f) {
 // written, and the rest of the frame is enqueued without waiting for it.
 int Engine::renderCounted(FrameState& f, const SceneDev& s, const CamParams& cp, const ans_options& o,
                           uint8_t* out8, float* outf) {
-    evPending = 0;  // stage times of this frame
+    stampPending = 0;  // stage times of this frame
     enqueueCount(f, s, cp, o);
     hipCheck(hipMemcpyAsync(ctlHost, f.ctl.p, sizeof(FrameCtl), hipMemcpyDeviceToHost, stream), "hipMemcpyAsync ctl");
     hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
@@ -320,25 +321,26 @@ int Engine::renderCounted(FrameState& f, const SceneDev& s, const CamParams& cp,
 
 void Engine::fillStats(const FrameState& f, ans_stats* st) {
     if (!st) {
-        evPending = 0;
+        stampPending = 0;
         return;
     }
-    hipCheck(hipEventSynchronize(ev[(evNext - 1 + kEventRing) % kEventRing][3]), "hipEventSynchronize");
+    hipCheck(hipMemcpyAsync(stampsHost, stamps.p, sizeof(FrameStamps) * kStampRing, hipMemcpyDeviceToHost, stream),
+             "hipMemcpyAsync stamps");
+    hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
     // Stage times averaged over the frames enqueued since the last readout (the newest
-    // kEventRing of them); every one of them has completed by now.
+    // kStampRing of them); every one of them has completed by now.
     double a = 0, b = 0, c = 0;
-    const int frames = std::max(evPending, 1);
+    const int frames = std::max(stampPending, 1);
+    const double msPerTick = 1.0 / clockKHz;
     for (int k = 0; k < frames; ++k) {
-        hipEvent_t* E = ev[(evNext - 1 - k + kEventRing) % kEventRing];
-        float x = 0, y = 0, z = 0;
-        hipCheck(hipEventElapsedTime(&x, E[0], E[1]), "hipEventElapsedTime");
-        hipCheck(hipEventElapsedTime(&y, E[1], E[2]), "hipEventElapsedTime");
-        hipCheck(hipEventElapsedTime(&z, E[2], E[3]), "hipEventElapsedTime");
-        a += x;
-        b += y;
-        c += z;
+        const int slot = (stampNext - 1 - k + kStampRing) % kStampRing;
+        const uint64_t* t = stampsHost[slot].t;
+        const uint64_t t1 = stampNoPairs[slot] ? t[2] : t[1];
+        a += static_cast<double>(t1 - t[0]) * msPerTick;
+        b += static_cast<double>(t[2] - t1) * msPerTick;
+        c += static_cast<double>(t[3] - t[2]) * msPerTick;
     }
-    evPending = 0;
+    stampPending = 0;
     st->visible = f.visible;
     st->tile_pairs = f.pairs;
     st->project_ms = a / frames;

@@ -44,7 +44,7 @@ class RenderOptions:
 
 @dataclasses.dataclass
 class RenderStats:
-    """aniso::RenderStats (render.h:9-13); GPU stage times from HIP events."""
+    """aniso::RenderStats (render.h:9-13); GPU stage times from clock readings the frame's kernels take."""
 
     visible: int = 0
     tile_pairs: int = 0

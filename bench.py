@@ -72,8 +72,8 @@ def main():
         r.render(cam, st, opts)
 
     # Timed region: K frames enqueued back to back, device-resident input and output. The
-    # library's per-frame HIP events (recorded on its stream around each stage) give the stage
-    # times averaged over the timed frames (the newest 64), read by the wait.
+    # library's per-frame clock stamps (taken by the frame's kernels at each stage boundary) give
+    # the stage times averaged over the timed frames (the newest 64), read by the wait.
     torch.cuda.synchronize()
     _lib.check(L.ans_render_wait(r._h, None))
     t0 = time.perf_counter()
@@ -106,8 +106,8 @@ def main():
         "algorithmic_bytes": b_blend,
         "kernel_ms": round(blend, 4),
         "valu": valu_bound(measured_traffic("k_blend", args.workload, "valu_instructions_per_launch"), blend),
-        "timing": f"HIP events around the blend on the library's stream, mean of the last "
-                  f"{min(args.steps, 64)} timed frames",
+        "timing": f"device wall-clock stamps at the blend's start (workgroup 0) and end (last "
+                  f"workgroup to finish), mean of the last {min(args.steps, 64)} timed frames",
     }
 
     # The same frame through the host-output entry (ans_render: synchronous, image copied to host
@@ -207,7 +207,7 @@ def bench_single_frames(r, L, c_cam, c_opt, dev_img, warm=25, cold=10):
     """Frames rendered one at a time, each waited for: the median of `warm` frames after one
     warm-up, and the median of `cold` frames each preceded by writing 1 GiB of scratch (more than
     the last-level cache, so the scene and the frame's buffers come from HBM). Wall time per frame
-    and the frame's event-timed GPU time."""
+    and the frame's GPU time from the library's clock stamps."""
     import torch
 
     from aniso_amd import _lib
@@ -234,12 +234,12 @@ def bench_single_frames(r, L, c_cam, c_opt, dev_img, warm=25, cold=10):
             "cold_median_ms": med([w for w, _ in cold_runs]), "cold_gpu_median_ms": med([g for _, g in cold_runs]),
             "warm_runs": warm, "cold_runs": cold,
             "cold_flush": "1 GiB device write before each cold frame (evicts the last-level cache)",
-            "includes": "ans_render_async + ans_render_wait per frame; gpu = event-timed project + sort + blend"}
+            "includes": "ans_render_async + ans_render_wait per frame; gpu = stamp-timed project + sort + blend"}
 
 
 def valu_bound(instructions, kernel_ms):
     """The blend's compute side beside the HBM roofline: wave instructions per launch (the
-    profile's SQ_INSTS_VALU pass) over the chip's VALU issue capacity during the event-timed
+    profile's SQ_INSTS_VALU pass) over the chip's VALU issue capacity during the stamp-timed
     kernel, at the peak clock (a lower bound on the busy fraction if the clock runs lower)."""
     if not instructions:
         return None

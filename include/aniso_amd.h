@@ -53,8 +53,9 @@ typedef struct ans_options {
 #define ANS_FLAG_NO_GRAPH 2
 
 /* aniso::RenderStats (include/aniso/render.h:9-13) plus the pair-buffer bookkeeping. Times are
- * GPU time per stage from HIP events: project = preprocess + depth sort, sort = pair expansion
- * + pair sort + tile ranges, blend = blend + output. */
+ * GPU time per stage from the device wall clock, read by the frame's own kernels: project =
+ * preprocess + depth sort, sort = pair expansion + pair sort + tile ranges, blend = blend +
+ * output. */
 typedef struct ans_stats {
     int64_t visible;
     int64_t tile_pairs;

@@ -161,3 +161,31 @@ def test_blend_culling_changes_nothing(gpu, tile):
         _, a = r.render_float(cam, None, RenderOptions(tile=tile))
         _, b = r.render_float(cam, None, RenderOptions(tile=tile, cull=False))
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_stage_times(gpu):
+    """Stage times come from clock readings the frame's own kernels take (preprocess start, pair
+    counting start, blend start, blend end): each stage is positive, and the stages of frames
+    rendered one at a time sum to less than the wall time around them; an empty scene has no sort
+    stage."""
+    import time
+
+    raw = scenes.synthetic_scene(200_000, 640, 360, seed=3)
+    cam = scenes.pinhole_camera(640, 360)
+    with GpuRenderer(raw) as r:
+        st = RenderStats()
+        r.render(cam, st)
+        for _ in range(5):
+            t0 = time.perf_counter()
+            r.render(cam, st)
+            wall = (time.perf_counter() - t0) * 1e3
+            print(f"project {st.project_ms:.4f} sort {st.sort_ms:.4f} blend {st.blend_ms:.4f} wall {wall:.4f} ms",
+                  flush=True)
+            assert st.project_ms > 0 and st.sort_ms > 0 and st.blend_ms > 0
+            assert st.project_ms + st.sort_ms + st.blend_ms < wall
+    empty = oracle.ActivatedScene(np.zeros((0, 3)), np.zeros((0, 3)), np.zeros((0, 4)), np.zeros(0),
+                                  np.zeros((0, 16, 3)))
+    with GpuRenderer(empty, activated=True) as r:
+        st = RenderStats()
+        r.render(cam, st)
+        assert st.sort_ms == 0 and st.project_ms > 0 and st.blend_ms > 0
