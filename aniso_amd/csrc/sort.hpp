@@ -101,10 +101,15 @@ __global__ __launch_bounds__(kSortBlock) void k_radixUpsweep(const uint32_t* __r
     for (int i = tid; i < radix; i += kSortBlock) h[i] = 0;
     __syncthreads();
     const uint32_t mask = radix - 1;
+    // Every key is loaded before any is counted (a load per LDS add would wait out each load's
+    // latency in turn); past the end the index is clamped and the add skipped.
+    if (base < n) {  // block 0 of an empty sort writes zeros
+        uint32_t key[ITEMS];
 #pragma unroll
-    for (int j = 0; j < ITEMS; ++j) {
-        const uint32_t idx = base + j * kSortBlock + tid;
-        if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & mask], 1u);
+        for (int j = 0; j < ITEMS; ++j) key[j] = keys[min(base + j * kSortBlock + tid, n - 1)];
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j)
+            if (base + j * kSortBlock + tid < n) atomicAdd(&h[(key[j] >> shift) & mask], 1u);
     }
     __syncthreads();
     for (int i = tid; i < radix; i += kSortBlock) hist[static_cast<size_t>(i) * stride + blockIdx.x] = h[i];
