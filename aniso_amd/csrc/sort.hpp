@@ -87,7 +87,13 @@ __global__ __launch_bounds__(kSortBlock) void k_radixUpsweep(const uint32_t* __r
                                                              uint32_t* __restrict__ hist, uint32_t stride,
                                                              const FrameCtl* __restrict__ ctl, uint32_t itemStride,
                                                              uint32_t histStride) {
-    __shared__ uint32_t h[kRadix];
+    // kCopies histograms, lane l adding into copy l % kCopies: lanes of one LDS add that share a
+    // digit (most of them when the digit is the top byte of nearby depths) update different words
+    // instead of queueing on one. Rows are kRadix + 1 words apart so equal digits of different
+    // copies fall in different banks.
+    constexpr int kCopies = 8;
+    constexpr int kRow = kRadix + 1;
+    __shared__ uint32_t h[kCopies * kRow];
     const uint32_t view = blockIdx.y;
     ctl += view;
     if (ctl->overflow) return;
@@ -98,7 +104,7 @@ __global__ __launch_bounds__(kSortBlock) void k_radixUpsweep(const uint32_t* __r
     if (base >= n && blockIdx.x > 0) return;
     const int tid = threadIdx.x;
     const int radix = 1 << bits;
-    for (int i = tid; i < radix; i += kSortBlock) h[i] = 0;
+    for (int i = tid; i < kCopies * kRow; i += kSortBlock) h[i] = 0;
     __syncthreads();
     const uint32_t mask = radix - 1;
     // Every key is loaded before any is counted (a load per LDS add would wait out each load's
@@ -107,12 +113,18 @@ __global__ __launch_bounds__(kSortBlock) void k_radixUpsweep(const uint32_t* __r
         uint32_t key[ITEMS];
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j) key[j] = keys[min(base + j * kSortBlock + tid, n - 1)];
+        uint32_t* mine = h + (__lane_id() % kCopies) * kRow;
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j)
-            if (base + j * kSortBlock + tid < n) atomicAdd(&h[(key[j] >> shift) & mask], 1u);
+            if (base + j * kSortBlock + tid < n) atomicAdd(&mine[(key[j] >> shift) & mask], 1u);
     }
     __syncthreads();
-    for (int i = tid; i < radix; i += kSortBlock) hist[static_cast<size_t>(i) * stride + blockIdx.x] = h[i];
+    for (int i = tid; i < radix; i += kSortBlock) {
+        uint32_t c = 0;
+#pragma unroll
+        for (int k = 0; k < kCopies; ++k) c += h[k * kRow + i];
+        hist[static_cast<size_t>(i) * stride + blockIdx.x] = c;
+    }
 }
 
 // Exclusive scan of one row hist[bin * stride, + blocks holding items) in place by one workgroup
