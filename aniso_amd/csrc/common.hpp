@@ -47,6 +47,17 @@ struct FrameCtl {
     uint32_t pad[2];
 };
 
+// A frame's pair count as the pair-count scan publishes it to pinned host memory: the fields,
+// then seq (the host's number for the frame) with a system-scope release, so a host that sees
+// seq sees the fields. The host waits on seq instead of synchronizing the stream, and the kernels
+// it enqueued after the count keep the GPU busy meanwhile.
+struct CountMirror {
+    uint32_t seq;
+    uint32_t pairs;
+    uint32_t visible;
+    uint32_t overflow;
+};
+
 // Stage timestamps of one frame, read from the GPU's constant-rate wall clock (s_memrealtime) by
 // the kernels themselves: [0] preprocess start, [1] pair counting start (end of the depth sort),
 // [2] blend start, [3] blend end. Stamps taken in kernels cost nothing between launches; events

@@ -95,6 +95,9 @@ struct Engine {
     FrameStamps* curStamps = nullptr;  // stamps of the frame being enqueued
     double clockKHz = 0;               // wall clock rate
     FrameCtl* ctlHost = nullptr;       // pinned
+    CountMirror* mirror = nullptr;     // pinned, coherent: written by the pair-count scan
+    CountMirror* mirrorDev = nullptr;  // its device address
+    uint32_t countSeq = 0;             // number of the last frame whose count was enqueued
 
     DevBuf<uint32_t> depthKey, sortKA, sortVA, sortKB, sortVB;
     DevBuf<uint32_t> ttByRank;      // tile counts in depth order
@@ -126,6 +129,9 @@ struct Engine {
     void launchBlendB(FrameState& f, bool cull, uint8_t* out8, float* outf, FrameStamps* st);
     // Copies the frame's control block to the host and waits; true if the pair buffers overflowed.
     bool readBack(FrameState& f);
+    // Waits until the pair count of the frame enqueued last has been published (not for the rest
+    // of the frame) and returns it.
+    CountMirror waitCount(const FrameState& f);
     int renderCounted(FrameState& f, const SceneDev& s, const CamParams& cp, const ans_options& o, uint8_t* out8,
                       float* outf);
     void fillStats(const FrameState& f, ans_stats* st);

@@ -368,11 +368,12 @@ static __global__ __launch_bounds__(kDupBlock) void k_pairTotals(const uint32_t*
 
 // One workgroup: exclusive scan of the per-block pair counts; publishes the pair total (the sum
 // gpu_render.cu:324-329 reads back to the host) and the visible count, and flags a pair total
-// above the pair buffers' capacity.
+// above the pair buffers' capacity; with mirror, also to pinned host memory under number seq.
 static __global__ __launch_bounds__(kDupBlock) void k_scanPairs(uint32_t* __restrict__ blockPairs,
                                                                 uint32_t* __restrict__ blockVisible,
                                                                 uint32_t blocks, uint32_t cap,
-                                                                FrameCtl* __restrict__ ctl, uint32_t blockStride) {
+                                                                FrameCtl* __restrict__ ctl, uint32_t blockStride,
+                                                                CountMirror* __restrict__ mirror, uint32_t seq) {
     __shared__ uint32_t sTmp[kDupBlock / kWave];
     blockPairs += static_cast<size_t>(blockIdx.y) * blockStride;
     blockVisible += static_cast<size_t>(blockIdx.y) * blockStride;
@@ -403,11 +404,18 @@ static __global__ __launch_bounds__(kDupBlock) void k_scanPairs(uint32_t* __rest
         visible += visTotal;
     }
     if (threadIdx.x == 0) {
+        const uint32_t pairs = carry > 0xffffffffull ? 0xffffffffu : static_cast<uint32_t>(carry);
         ctl->visible = visible;
-        ctl->pairs = carry > 0xffffffffull ? 0xffffffffu : static_cast<uint32_t>(carry);
+        ctl->pairs = pairs;
         if (carry > cap) {
             ctl->overflow = 1;
             ctl->sticky = 1;
+        }
+        if (mirror) {
+            mirror->pairs = pairs;
+            mirror->visible = visible;
+            mirror->overflow = carry > cap ? 1u : 0u;
+            __hip_atomic_store(&mirror->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
         }
     }
 }

@@ -109,6 +109,11 @@ Engine::Engine() {
     clockKHz = khz;
     hipCheck(hipHostMalloc(reinterpret_cast<void**>(&ctlHost), sizeof(FrameCtl), hipHostMallocDefault),
              "hipHostMalloc");
+    hipCheck(hipHostMalloc(reinterpret_cast<void**>(&mirror), sizeof(CountMirror),
+                           hipHostMallocMapped | hipHostMallocCoherent),
+             "hipHostMalloc");
+    std::memset(mirror, 0, sizeof(CountMirror));
+    hipCheck(hipHostGetDevicePointer(reinterpret_cast<void**>(&mirrorDev), mirror, 0), "hipHostGetDevicePointer");
     hipCheck(hipHostMalloc(reinterpret_cast<void**>(&stampsHost), sizeof(FrameStamps) * kStampRing,
                            hipHostMallocDefault),
              "hipHostMalloc");
@@ -118,6 +123,7 @@ Engine::Engine() {
 
 Engine::~Engine() {
     if (ctlHost) (void)hipHostFree(ctlHost);
+    if (mirror) (void)hipHostFree(mirror);
     if (stampsHost) (void)hipHostFree(stampsHost);
     if (ownStream) (void)hipStreamDestroy(ownStream);
 }
@@ -202,7 +208,7 @@ void Engine::enqueueCount(FrameState& f, const SceneDev& s, const CamParams& cp,
         k_pairTotals<<<blocksDup, kDupBlock, 0, stream>>>(f.order.p, f.rec.p, n, ttByRank.p, rectByRank.p,
                                                           blockPairs.p, blockVisible.p, 0, 0, S);
         k_scanPairs<<<1, kDupBlock, 0, stream>>>(blockPairs.p, blockVisible.p, blocksDup,
-                                                 static_cast<uint32_t>(f.pairCap), ctl, 0);
+                                                 static_cast<uint32_t>(f.pairCap), ctl, 0, mirrorDev, ++countSeq);
     }
 }
 
@@ -296,26 +302,58 @@ bool Engine::readBack(FrameState& f) {
     return false;
 }
 
+CountMirror Engine::waitCount(const FrameState& f) {
+    CountMirror m{};
+    if (f.n == 0) return m;  // no pair-count scan was launched: no pairs
+    const uint32_t want = countSeq;
+    for (uint32_t spin = 1;; ++spin) {
+        if (__atomic_load_n(&mirror->seq, __ATOMIC_ACQUIRE) == want) {
+            m.pairs = mirror->pairs;
+            m.visible = mirror->visible;
+            m.overflow = mirror->overflow;
+            return m;
+        }
+        if ((spin & 1023) == 0) {
+            // Every so often: has the stream drained without the value arriving? Then read the
+            // control block the scan also wrote.
+            const hipError_t q = hipStreamQuery(stream);
+            if (q == hipSuccess && __atomic_load_n(&mirror->seq, __ATOMIC_ACQUIRE) != want) {
+                hipCheck(hipMemcpy(ctlHost, f.ctl.p, sizeof(FrameCtl), hipMemcpyDeviceToHost), "hipMemcpy ctl");
+                m.pairs = ctlHost->pairs;
+                m.visible = ctlHost->visible;
+                m.overflow = ctlHost->overflow;
+                return m;
+            }
+            if (q != hipSuccess && q != hipErrorNotReady) hipCheck(q, "hipStreamQuery");
+        }
+        __builtin_ia32_pause();
+    }
+}
+
 // Renders a frame, waiting only for its pair count (the reference reads the same total back,
-// gpu_render.cu:324-329): if the pairs exceed the pair buffers they grow before any pair is
-// written, and the rest of the frame is enqueued without waiting for it.
+// gpu_render.cu:324-329). The rest of the frame is enqueued before the wait, with the pair
+// buffers as they are, so the GPU does not idle while the host waits: if the pairs exceeded the
+// buffers, those kernels skip their work (the control block's overflow flag), the buffers grow
+// and the rest of the frame is enqueued again.
 int Engine::renderCounted(FrameState& f, const SceneDev& s, const CamParams& cp, const ans_options& o,
                           uint8_t* out8, float* outf) {
     stampPending = 0;  // stage times of this frame
     enqueueCount(f, s, cp, o);
-    hipCheck(hipMemcpyAsync(ctlHost, f.ctl.p, sizeof(FrameCtl), hipMemcpyDeviceToHost, stream), "hipMemcpyAsync ctl");
-    hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
-    f.visible = ctlHost->visible;
-    f.pairs = ctlHost->pairs;
-    f.retries = 0;
-    if (ctlHost->overflow) {
-        if (ctlHost->pairs >= (1u << 30) - 1) throw std::runtime_error("pair count exceeds 2^30");
-        f.reservePairs(static_cast<size_t>(ctlHost->pairs) + ctlHost->pairs / 4 + 1024);
-        // The flags only stop the kernels of this frame; with room now, clear them.
-        hipCheck(hipMemsetAsync(&f.ctl.p->overflow, 0, 8, stream), "hipMemsetAsync");  // overflow, sticky
-        f.retries = 1;
-    }
     enqueueRest(f, out8, outf);
+    const CountMirror m = waitCount(f);
+    f.visible = m.visible;
+    f.pairs = m.pairs;
+    f.retries = 0;
+    if (m.overflow) {
+        if (m.pairs >= (1u << 30) - 1) throw std::runtime_error("pair count exceeds 2^30");
+        hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");  // the skipped kernels have ended
+        f.reservePairs(static_cast<size_t>(m.pairs) + m.pairs / 4 + 1024);
+        // The flags only stopped the kernels of this frame; with room now, clear them (and the
+        // blend's workgroup counter, which the skipped blend advanced).
+        hipCheck(hipMemsetAsync(&f.ctl.p->overflow, 0, 12, stream), "hipMemsetAsync");  // overflow, sticky, finished
+        f.retries = 1;
+        enqueueRest(f, out8, outf);
+    }
     return f.retries;
 }
 
