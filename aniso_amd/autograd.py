@@ -78,7 +78,8 @@ def _stream():
 class RasterizeFunction(torch.autograd.Function):
     """Inputs: the five raw parameter tensors, an optional `means2d` leaf ([N, 2], only its
     gradient is used: it receives dL/d(u, v), the screen-space mean gradient), the camera, the
-    options, optional stats. Outputs: the image and the per-Gaussian radii (not differentiable)."""
+    options, optional stats. Outputs: the image and the per-Gaussian radii (not differentiable;
+    empty unless means2d is given)."""
 
     @staticmethod
     def forward(ctx, means, log_scales, quats, opacity_logits, sh, means2d, camera: Camera, options: RenderOptions,
@@ -102,8 +103,10 @@ class RasterizeFunction(torch.autograd.Function):
                                       ctypes.byref(camera_c(camera)), ctypes.byref(options.c()), image.data_ptr(),
                                       _stream(), ctypes.byref(fid), ctypes.byref(st) if st is not None else None))
         frame = _Frame(handle, fid.value)
-        radii = torch.empty(n, dtype=torch.int32, device=means.device)
-        _lib.check(L.ans_diff_radii(handle, fid.value, radii.data_ptr(), _stream()))
+        # Radii only for the info path (means2d given); a plain render skips their launch.
+        radii = torch.empty(n if means2d is not None else 0, dtype=torch.int32, device=means.device)
+        if means2d is not None:
+            _lib.check(L.ans_diff_radii(handle, fid.value, radii.data_ptr(), _stream()))
         if stats is not None:
             for f in ("visible", "tile_pairs", "project_ms", "sort_ms", "blend_ms", "pair_capacity", "retries"):
                 setattr(stats, f, getattr(st, f))
