@@ -49,7 +49,7 @@ struct ViewBatch {
     DevBuf<uint8_t> table;
     DevBuf<FrameCtl> ctl;  // one per view, plus one for the item-order sort
     DevBuf<SplatRec> rec;
-    DevBuf<uint32_t> tilesTouched, depthKey, order, kA, vA, kB, vB, ttByRank;
+    DevBuf<uint32_t> depthKey, order, kA, vA, kB, vB, ttByRank;
     DevBuf<uint2> rectByRank;
     DevBuf<uint32_t> hist, rowTotal, blockPairs, blockVisible;
     DevBuf<uint32_t> tileA, gaussA, tileB, gaussB;
@@ -152,7 +152,7 @@ void reserveArena(ViewBatch& B, const Layout& L) {
     moved |= grow(B.table, TableLayout(L.views).bytes);
     moved |= grow(B.ctl, V + 1);
     moved |= grow(B.rec, vn);
-    moved |= grow(B.tilesTouched, vn) | grow(B.depthKey, vn) | grow(B.order, vn);
+    moved |= grow(B.depthKey, vn) | grow(B.order, vn);
     moved |= grow(B.kA, vn) | grow(B.vA, vn) | grow(B.kB, vn) | grow(B.vB, vn);
     moved |= grow(B.ttByRank, vn) | grow(B.rectByRank, vn);
     const size_t histD = V * kRadix * size_t(L.blocksD), histP = V * kRadix * size_t(L.blocksP),
@@ -211,7 +211,7 @@ const uint32_t* enqueueBatch(ViewBatch& B, const SceneDev& s, const Layout& L, h
     if (n > 0) {
         k_preprocessViews<<<(n + 255) / 256, 256, 0, st>>>(static_cast<int>(n), L.views, s.pos, s.scale, s.rot,
                                                            s.opacity, s.sh, (s.shDegree + 1) * (s.shDegree + 1),
-                                                           cams, B.rec.p, B.tilesTouched.p, B.depthKey.p);
+                                                           cams, B.rec.p, nullptr, B.depthKey.p);
         // Depth sort of each view's Gaussians: four stable 8-bit passes.
         const uint32_t histD = kRadix * L.blocksD;  // depth sort: kPairItems keys per thread
         const uint32_t* kin = B.depthKey.p;

@@ -252,12 +252,13 @@ __host__ __device__ inline bool finishProjection(const F3 m, float opacityIn, co
 __device__ inline void storeCulled(size_t o, SplatRec* __restrict__ rec, uint32_t* __restrict__ tilesTouched,
                                    uint32_t* __restrict__ depthKey) {
     rec[o].meta = make_uint4(0u, 0u, 0u, 0u);
-    tilesTouched[o] = 0;
+    if (tilesTouched) tilesTouched[o] = 0;
     depthKey[o] = 0xffffffffu;
 }
 
-// Writes a visible Gaussian's splat record (with its tile rect and count), tile count and depth
-// key at index o.
+// Writes a visible Gaussian's splat record (with its tile rect and count), tile count (unless
+// tilesTouched is null: the batched views read the count from the record) and depth key at
+// index o.
 __device__ inline void storeSplat(const GaussProj& g, size_t o, SplatRec* __restrict__ rec,
                                   uint32_t* __restrict__ tilesTouched, uint32_t* __restrict__ depthKey) {
     // Region where op * exp(-q/2) >= 1/255: q <= 2 ln(255 op); half-extents sqrt(k a), sqrt(k c)
@@ -277,7 +278,7 @@ __device__ inline void storeSplat(const GaussProj& g, size_t o, SplatRec* __rest
     r.meta = make_uint4(static_cast<uint32_t>(g.x0) | (static_cast<uint32_t>(g.y0) << 16),
                         static_cast<uint32_t>(g.x1) | (static_cast<uint32_t>(g.y1) << 16), tt,
                         static_cast<uint32_t>(g.radius));
-    tilesTouched[o] = tt;
+    if (tilesTouched) tilesTouched[o] = tt;
     depthKey[o] = __float_as_uint(g.depth);
 }
 
