@@ -56,12 +56,14 @@ struct CountMirror {
     uint32_t pairs;
     uint32_t visible;
     uint32_t overflow;
+    uint32_t sticky;  // set with overflow and left set: the host clears it when it has read it
 };
 
 // Stage timestamps of one frame, read from the GPU's constant-rate wall clock (s_memrealtime) by
-// the kernels themselves: [0] preprocess start, [1] pair counting start (end of the depth sort),
-// [2] blend start, [3] blend end. Stamps taken in kernels cost nothing between launches; events
-// recorded between two kernels separate them by several microseconds.
+// the kernels themselves and stored straight to pinned host memory: [0] preprocess start, [1]
+// pair counting start (end of the depth sort), [2] blend start, [3] blend end. Stamps taken in
+// kernels cost nothing between launches; events recorded between two kernels separate them by
+// several microseconds.
 struct FrameStamps {
     uint64_t t[4];
 };

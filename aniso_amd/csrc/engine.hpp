@@ -85,11 +85,11 @@ struct Engine {
     int device = 0;
     hipStream_t ownStream = nullptr;
     hipStream_t stream = nullptr;
-    // Stage timestamps of the last kStampRing frames, written by the frames' kernels; stats
-    // average the frames enqueued since the previous readout.
+    // Stage timestamps of the last kStampRing frames, written by the frames' kernels into pinned
+    // host memory; stats average the frames enqueued since the previous readout.
     static constexpr int kStampRing = 64;
-    DevBuf<FrameStamps> stamps;
-    FrameStamps* stampsHost = nullptr;  // pinned
+    FrameStamps* stampsHost = nullptr;  // pinned, coherent
+    FrameStamps* stampsDev = nullptr;   // its device address
     bool stampNoPairs[kStampRing] = {};
     int stampNext = 0, stampPending = 0;
     FrameStamps* curStamps = nullptr;  // stamps of the frame being enqueued
@@ -127,14 +127,17 @@ struct Engine {
     void launchBlendT(FrameState& f, bool cull, uint8_t* out8, float* outf, FrameStamps* st);
     template <int TW, int TH, int BATCH>
     void launchBlendB(FrameState& f, bool cull, uint8_t* out8, float* outf, FrameStamps* st);
-    // Copies the frame's control block to the host and waits; true if the pair buffers overflowed.
+    // Waits for the stream; true if the pair buffers overflowed in any frame since the last call
+    // (they have then been grown).
     bool readBack(FrameState& f);
     // Waits until the pair count of the frame enqueued last has been published (not for the rest
     // of the frame) and returns it.
     CountMirror waitCount(const FrameState& f);
     int renderCounted(FrameState& f, const SceneDev& s, const CamParams& cp, const ans_options& o, uint8_t* out8,
                       float* outf);
-    void fillStats(const FrameState& f, ans_stats* st);
+    // Fills st (if not null) for frame f; synced: the caller has waited for the stream, else this
+    // waits.
+    void fillStats(const FrameState& f, ans_stats* st, bool synced = false);
     void enqueueBackward(FrameState& f, const SceneDev& s, const float* gradImage, float* gMeans, float* gScales,
                          float* gQuats, float* gOpacity, float* gSh);
 };

@@ -416,6 +416,7 @@ static __global__ __launch_bounds__(kDupBlock) void k_scanPairs(uint32_t* __rest
             mirror->pairs = pairs;
             mirror->visible = visible;
             mirror->overflow = carry > cap ? 1u : 0u;
+            if (carry > cap) mirror->sticky = 1u;
             __hip_atomic_store(&mirror->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
         }
     }

@@ -115,10 +115,10 @@ Engine::Engine() {
     std::memset(mirror, 0, sizeof(CountMirror));
     hipCheck(hipHostGetDevicePointer(reinterpret_cast<void**>(&mirrorDev), mirror, 0), "hipHostGetDevicePointer");
     hipCheck(hipHostMalloc(reinterpret_cast<void**>(&stampsHost), sizeof(FrameStamps) * kStampRing,
-                           hipHostMallocDefault),
+                           hipHostMallocMapped | hipHostMallocCoherent),
              "hipHostMalloc");
-    stamps.reserve(kStampRing);
-    hipCheck(hipMemset(stamps.p, 0, sizeof(FrameStamps) * kStampRing), "hipMemset");
+    std::memset(stampsHost, 0, sizeof(FrameStamps) * kStampRing);
+    hipCheck(hipHostGetDevicePointer(reinterpret_cast<void**>(&stampsDev), stampsHost, 0), "hipHostGetDevicePointer");
 }
 
 Engine::~Engine() {
@@ -162,7 +162,7 @@ void Engine::enqueueCount(FrameState& f, const SceneDev& s, const CamParams& cp,
     f.cp = cp;
     f.opts = o;
     FrameCtl* ctl = f.ctl.p;
-    FrameStamps* S = stamps.p + stampNext;
+    FrameStamps* S = stampsDev + stampNext;
     curStamps = S;
     stampNoPairs[stampNext] = n == 0;  // no pair-counting kernel: the sort stage is empty
     stampNext = (stampNext + 1) % kStampRing;
@@ -289,14 +289,16 @@ void Engine::launchBlend(FrameState& f, uint8_t* out8, float* outf, FrameStamps*
 }
 
 bool Engine::readBack(FrameState& f) {
-    hipCheck(hipMemcpyAsync(ctlHost, f.ctl.p, 32, hipMemcpyDeviceToHost, stream), "hipMemcpyAsync ctl");
-    hipCheck(hipMemsetAsync(&f.ctl.p->sticky, 0, 4, stream), "hipMemsetAsync");
     hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
-    f.visible = ctlHost->visible;
-    f.pairs = ctlHost->pairs;
-    if (ctlHost->sticky) {
-        if (ctlHost->pairs >= (1u << 30) - 1) throw std::runtime_error("pair count exceeds 2^30");
-        f.reservePairs(static_cast<size_t>(ctlHost->pairs) + ctlHost->pairs / 4 + 1024);
+    // The pair-count scan of each frame published its counts to the mirror (an empty scene has
+    // no scan and no pairs).
+    f.visible = f.n ? mirror->visible : 0;
+    f.pairs = f.n ? mirror->pairs : 0;
+    if (mirror->sticky) {
+        mirror->sticky = 0;
+        const uint32_t pairs = mirror->pairs;
+        if (pairs >= (1u << 30) - 1) throw std::runtime_error("pair count exceeds 2^30");
+        f.reservePairs(static_cast<size_t>(pairs) + pairs / 4 + 1024);
         return true;
     }
     return false;
@@ -347,6 +349,7 @@ int Engine::renderCounted(FrameState& f, const SceneDev& s, const CamParams& cp,
     if (m.overflow) {
         if (m.pairs >= (1u << 30) - 1) throw std::runtime_error("pair count exceeds 2^30");
         hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");  // the skipped kernels have ended
+        mirror->sticky = 0;  // handled here, not by a later wait
         f.reservePairs(static_cast<size_t>(m.pairs) + m.pairs / 4 + 1024);
         // The flags only stopped the kernels of this frame; with room now, clear them (and the
         // blend's workgroup counter, which the skipped blend advanced).
@@ -357,14 +360,12 @@ int Engine::renderCounted(FrameState& f, const SceneDev& s, const CamParams& cp,
     return f.retries;
 }
 
-void Engine::fillStats(const FrameState& f, ans_stats* st) {
+void Engine::fillStats(const FrameState& f, ans_stats* st, bool synced) {
     if (!st) {
         stampPending = 0;
         return;
     }
-    hipCheck(hipMemcpyAsync(stampsHost, stamps.p, sizeof(FrameStamps) * kStampRing, hipMemcpyDeviceToHost, stream),
-             "hipMemcpyAsync stamps");
-    hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    if (!synced) hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
     // Stage times averaged over the frames enqueued since the last readout (the newest
     // kStampRing of them); every one of them has completed by now.
     double a = 0, b = 0, c = 0;
@@ -556,7 +557,7 @@ int ans_render_float(ans_renderer* r, const ans_camera* cam, const ans_options* 
                                     e.stream),
                      "hipMemcpyAsync image");
         hipCheck(hipStreamSynchronize(e.stream), "hipStreamSynchronize");
-        e.fillStats(e.frame, stats);
+        e.fillStats(e.frame, stats, true);
         return ANS_OK;
     });
 }
@@ -590,7 +591,7 @@ int ans_render_wait(ans_renderer* r, ans_stats* stats) {
         Engine& e = *r->engine;
         const bool overflow = e.readBack(e.frame);
         e.frame.retries = 0;
-        e.fillStats(e.frame, stats);
+        e.fillStats(e.frame, stats, true);
         if (overflow) {
             g_lastError = "pair count exceeded the pair buffers; buffers grown, render again";
             return ANS_ERR_CAPACITY;
