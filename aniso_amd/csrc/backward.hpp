@@ -35,25 +35,30 @@ __device__ inline float waveSum(float v) {
     return addSwapped(r32[0], r32[1]);
 }
 
-// Sums over the 64 lanes of two splats' 10 per-pixel components at once (a reduce-scatter):
+// Sums over the 64 lanes of two splats' 9 gradient components at once (a reduce-scatter):
 // permlane32_swap leaves lanes 0-31 holding splat 1's pairwise sums and lanes 32-63 splat 2's;
 // permlane16_swap pairs components (k, k + 5) so that rows 0..3 of register k hold
 // (splat 1, k), (splat 1, k + 5), (splat 2, k), (splat 2, k + 5); a row butterfly of DPP adds
 // (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror, row_mirror) completes each row.
-// The last lane of each row then writes its value to out[(splat 1 or 2) row base + k (+ 5)]
-// (rows of 10 floats). The order of additions is fixed.
-__device__ inline void pairSumStore(float (&d1)[10], float (&d2)[10], float* rowBase) {
-    float t[10];
+// Component 4 has no partner: its rows 1 and 3 repeat rows 0 and 2. The last lane of each row
+// then writes its value to out[(splat 1 or 2) row base + k (+ 5)], where the caller's row of 10
+// floats takes component 4's repeat in slot 9. The order of additions is fixed.
+__device__ inline void pairSumStore(float (&d1)[9], float (&d2)[9], float* rowBase) {
+    float t[9];
 #pragma unroll
-    for (int c = 0; c < 10; ++c) {
+    for (int c = 0; c < 9; ++c) {
         const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(d1[c]), __float_as_uint(d2[c]), false, false);
         t[c] = addSwapped(r[0], r[1]);
     }
     float u[5];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) {
+    for (int k = 0; k < 4; ++k) {
         const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(t[k]), __float_as_uint(t[k + 5]), false, false);
         u[k] = addSwapped(r[0], r[1]);
+    }
+    {
+        const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(t[4]), __float_as_uint(t[4]), false, false);
+        u[4] = addSwapped(r[0], r[1]);
     }
 #pragma unroll
     for (int k = 0; k < 5; ++k) u[k] = dppAdd<0xB1>(u[k]);
@@ -84,12 +89,6 @@ constexpr int kPartialStride = 12;  // floats per pair record: u v A B C opacity
 // block, two at a time: pairSumStore reduces the two splats' per-pixel contributions over the
 // wavefront, and the wavefronts' sums are added in fixed order into one partial record per pair:
 // partial[slot * 12 + {u, v, A, B, C, opacity, r, g, b}].
-// The centre and conic terms are linear in g = dL/dpower per pixel times 1, dx, dy, dx^2,
-// dx dy, dy^2. With dx = lx - u', dy = ly - v' (pixel and splat centre relative to the tile
-// centre) a pixel contributes only the moments g (1, lx, ly, lx^2, lx ly, ly^2) of its own fixed
-// tile-local position (five products instead of thirteen operations), and the pair's sums of
-// g dx, g dy, g dx^2, g dx dy, g dy^2 are formed once per pair from the tile's moment sums;
-// the tile-local coordinates keep the terms of that expansion within a few hundred pixels^2.
 template <int TW, int TH, int BATCH>
 __global__ __launch_bounds__(TW* TH) void k_blendBackward(
     const uint2* __restrict__ ranges, const uint32_t* __restrict__ sortedGauss, const SplatRec* __restrict__ rec,
@@ -114,9 +113,6 @@ __global__ __launch_bounds__(TW* TH) void k_blendBackward(
     const int x = bx + (lane & 7), y = by + (lane >> 3);
     const bool inside = x < width && y < height;
     const float px = x + 0.5f, py = y + 0.5f;
-    const float cx = tileX * TW + 0.5f * TW, cy = tileY * TH + 0.5f * TH;  // tile centre
-    const float lx = px - cx, ly = py - cy;                                // exact (half-integers)
-    const float lxx = lx * lx, lxy = lx * ly, lyy = ly * ly;
     const uint64_t below = lanesBelow();
 
     const uint2 range = ranges[tile];
@@ -199,7 +195,7 @@ __global__ __launch_bounds__(TW* TH) void k_blendBackward(
     // position pos of the tile (zero if it does not blend there); steps T and accum back past it.
     // True if it contributed. power and alpha are the blend's, bit for bit (same staged conic, same
     // operations), so every blend/skip decision matches the forward's; selects replace branches.
-    auto visit = [&](const float4& a, const float4& bb, const float4& c, uint32_t pos, float (&d)[10]) -> bool {
+    auto visit = [&](const float4& a, const float4& bb, const float4& c, uint32_t pos, float (&d)[9]) -> bool {
         const float dx = px - a.x, dy = py - a.y;
         const SplatExponent x = splatExponentParts(a.z, a.w, bb.x, bb.z, dx, dy);
         const float raw = __builtin_amdgcn_exp2f(x.e);  // o G
@@ -215,15 +211,19 @@ __global__ __launch_bounds__(TW* TH) void k_blendBackward(
         const float gAlpha = Ti * __builtin_fmaf(gb, eb, __builtin_fmaf(gg, eg, gr * er));
         // No gradient where the 0.99 clamp is active; gPow = 0 zeroes the conic and centre terms.
         const bool open = use && raw <= 0.99f;
-        const float gPow = open ? gAlpha * raw : 0.0f;  // dL/dpower
+        const float gPow = open ? gAlpha * raw : 0.0f;
         d[5] = open ? gAlpha * G : 0.0f;
-        // Moments of dL/dpower at this pixel's tile-local position (components 0-4 and 9).
-        d[0] = gPow;
-        d[1] = gPow * lx;
-        d[2] = gPow * ly;
-        d[3] = gPow * lxx;
-        d[4] = gPow * lxy;
-        d[9] = gPow * lyy;
+        // With the staged qa = -log2(e) A/2, qb = -log2(e) B, qc = -log2(e) C/2 and
+        // t = qa dx + qb dy, r = qc dy: A dx + B dy = -(qa dx + t) ln 2 and
+        // C dy + B dx = -(qb dx + 2 r) ln 2.
+        const float gPowLn2 = gPow * (1.0f / kLog2e);
+        d[0] = -gPowLn2 * __builtin_fmaf(a.z, dx, x.t);
+        d[1] = -gPowLn2 * __builtin_fmaf(a.w, dx, x.r + x.r);
+        const float h = -0.5f * gPow;
+        const float hx = h * dx;
+        d[2] = hx * dx;
+        d[3] = (hx + hx) * dy;
+        d[4] = (h * dy) * dy;
         // accum <- alpha rgb + (1 - alpha) accum, as accum + alpha (rgb - accum).
         ar = use ? __builtin_fmaf(alpha, er, ar) : ar;
         ag = use ? __builtin_fmaf(alpha, eg, ag) : ag;
@@ -245,47 +245,27 @@ __global__ __launch_bounds__(TW* TH) void k_blendBackward(
             for (int i = count - 1; i >= 0; i -= 2) {
                 const int j1 = sList[wave][i];
                 const int j2 = i >= 1 ? static_cast<int>(sList[wave][i - 1]) : BATCH;
-                float d1[10], d2[10];
+                float d1[9], d2[9];
                 bool any = visit(s0[j1], s1[j1], s2[j1], base + j1, d1);
                 if (i >= 1) {
                     any |= visit(s0[j2], s1[j2], s2[j2], base + j2, d2);
                 } else {
 #pragma unroll
-                    for (int c = 0; c < 10; ++c) d2[c] = 0.0f;
+                    for (int c = 0; c < 9; ++c) d2[c] = 0.0f;
                 }
                 if (__ballot(any) != 0) pairSumStore(d1, d2, &sGrad[wave][secondSplat ? j2 : j1][rowHalf * 5]);
             }
         }
         __syncthreads();
-        // One partial per pair of this batch: the wavefronts' sums in fixed order, then the centre
-        // and conic terms from the moment sums. With the staged qa = -log2(e) A/2,
-        // qb = -log2(e) B, qc = -log2(e) C/2: dL/du = sum g (A dx + B dy) =
-        // -(2 qa Sx + qb Sy) ln 2, dL/dv = -(qb Sx + 2 qc Sy) ln 2, dL/dA = -Sxx / 2,
-        // dL/dB = -Sxy, dL/dC = -Syy / 2 (S: sums over the tile's pixels of g dx, g dy, ...).
+        // One partial per pair of this batch: the wavefronts' sums in fixed order.
         for (int k = tid; k < cnt; k += kThreads) {
-            float m[10];
-#pragma unroll
-            for (int c = 0; c < 10; ++c) {
-                m[c] = 0.0f;
-#pragma unroll
-                for (int wv = 0; wv < kWaves; ++wv) m[c] += sGrad[wv][k][c];
-            }
-            const float4 a = s0[k];
-            const float qc = s1[k].x;
-            const float up = a.x - cx, vp = a.y - cy;
-            const float M0 = m[0], Mx = m[1], My = m[2], Mxx = m[3], Mxy = m[4], Myy = m[9];
-            const float Sx = Mx - up * M0, Sy = My - vp * M0;
-            const float Sxx = Mxx - up * (2.0f * Mx - up * M0);
-            const float Syy = Myy - vp * (2.0f * My - vp * M0);
-            const float Sxy = Mxy - up * My - vp * Sx;
             float s[12];
-            s[0] = -(2.0f * a.z * Sx + a.w * Sy) * (1.0f / kLog2e);
-            s[1] = -(a.w * Sx + 2.0f * qc * Sy) * (1.0f / kLog2e);
-            s[2] = -0.5f * Sxx;
-            s[3] = -Sxy;
-            s[4] = -0.5f * Syy;
 #pragma unroll
-            for (int c = 5; c < 9; ++c) s[c] = m[c];
+            for (int c = 0; c < 9; ++c) {
+                s[c] = 0.0f;
+#pragma unroll
+                for (int wv = 0; wv < kWaves; ++wv) s[c] += sGrad[wv][k][c];
+            }
             s[9] = s[10] = s[11] = 0.0f;
             float4* out = reinterpret_cast<float4*>(partial + static_cast<size_t>(sSlot[k]) * kPartialStride);
             out[0] = make_float4(s[0], s[1], s[2], s[3]);
