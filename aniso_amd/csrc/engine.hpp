@@ -95,6 +95,12 @@ struct Engine {
     FrameStamps* curStamps = nullptr;  // stamps of the frame being enqueued
     double clockKHz = 0;               // wall clock rate
     FrameCtl* ctlHost = nullptr;       // pinned
+    // Ordering of the blends of frames pipelined over two engines (the asynchronous render API):
+    // with waitBeforeBlend set, the next frame's blend waits for that event (the other engine's
+    // previous blend); with recordBlendDone, blendDone is recorded after the blend.
+    hipEvent_t blendDone = nullptr;
+    hipEvent_t waitBeforeBlend = nullptr;
+    bool recordBlendDone = false;
     CountMirror* mirror = nullptr;     // pinned, coherent: written by the pair-count scan
     CountMirror* mirrorDev = nullptr;  // its device address
     uint32_t countSeq = 0;             // number of the last frame whose count was enqueued
@@ -138,6 +144,9 @@ struct Engine {
     // Fills st (if not null) for frame f; synced: the caller has waited for the stream, else this
     // waits.
     void fillStats(const FrameState& f, ans_stats* st, bool synced = false);
+    // Adds the stage times (ms) of the frames enqueued since the last readout to a, b, c and
+    // their number to frames, and starts a new readout.
+    void stageSums(double& a, double& b, double& c, int& frames);
     void enqueueBackward(FrameState& f, const SceneDev& s, const float* gradImage, float* gMeans, float* gScales,
                          float* gQuats, float* gOpacity, float* gSh);
 };
@@ -176,5 +185,12 @@ struct ans_renderer {
     ans::DevBuf<uint8_t> image8;
     ans::DevBuf<float> imagef;
     std::unique_ptr<ans::Engine> engine;
+    // Asynchronous frames alternate between engine and engine2 (created on first use), each with
+    // its own stream and frame buffers, so that one frame's sorts overlap the other's blend;
+    // blends still run in enqueue order.
+    std::unique_ptr<ans::Engine> engine2;
+    int asyncNext = 0;
+    ans::Engine* lastAsync = nullptr;  // engine of the last asynchronous frame not yet waited for
+    ans::Engine* last = nullptr;       // engine of the last frame enqueued (counts, dumps)
     std::unique_ptr<ans::ViewBatch, ans::ViewBatchDeleter> batch;
 };

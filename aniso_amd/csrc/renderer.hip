@@ -103,6 +103,7 @@ Engine::Engine() {
     hipCheck(hipGetDevice(&device), "hipGetDevice");
     hipCheck(hipStreamCreateWithFlags(&ownStream, hipStreamNonBlocking), "hipStreamCreate");
     stream = ownStream;
+    hipCheck(hipEventCreateWithFlags(&blendDone, hipEventDisableTiming), "hipEventCreate");
     int khz = 0;
     hipCheck(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device), "hipDeviceGetAttribute");
     if (khz <= 0) throw std::runtime_error("device reports no wall clock rate");
@@ -125,6 +126,7 @@ Engine::~Engine() {
     if (ctlHost) (void)hipHostFree(ctlHost);
     if (mirror) (void)hipHostFree(mirror);
     if (stampsHost) (void)hipHostFree(stampsHost);
+    if (blendDone) (void)hipEventDestroy(blendDone);
     if (ownStream) (void)hipStreamDestroy(ownStream);
 }
 
@@ -252,8 +254,10 @@ void Engine::enqueueRest(FrameState& f, uint8_t* out8, float* outf) {
         k_tileRanges<<<static_cast<uint32_t>((f.pairCap + 1023) / 1024), 256, 0, stream>>>(f.sortedTile, ctl,
                                                                                             f.ranges.p, 0, 0);
     }
+    if (waitBeforeBlend) hipCheck(hipStreamWaitEvent(stream, waitBeforeBlend, 0), "hipStreamWaitEvent");
     launchBlend(f, out8, outf, curStamps);
     hipCheck(hipGetLastError(), "kernel launch");
+    if (recordBlendDone) hipCheck(hipEventRecord(blendDone, stream), "hipEventRecord");
 }
 
 template <int TW, int TH, int BATCH>
@@ -360,18 +364,11 @@ int Engine::renderCounted(FrameState& f, const SceneDev& s, const CamParams& cp,
     return f.retries;
 }
 
-void Engine::fillStats(const FrameState& f, ans_stats* st, bool synced) {
-    if (!st) {
-        stampPending = 0;
-        return;
-    }
-    if (!synced) hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
-    // Stage times averaged over the frames enqueued since the last readout (the newest
-    // kStampRing of them); every one of them has completed by now.
-    double a = 0, b = 0, c = 0;
-    const int frames = std::max(stampPending, 1);
+void Engine::stageSums(double& a, double& b, double& c, int& frames) {
+    // Every frame enqueued since the last readout (the newest kStampRing of them) has completed.
+    const int count = std::min(stampPending, kStampRing);
     const double msPerTick = 1.0 / clockKHz;
-    for (int k = 0; k < frames; ++k) {
+    for (int k = 0; k < count; ++k) {
         const int slot = (stampNext - 1 - k + kStampRing) % kStampRing;
         const uint64_t* t = stampsHost[slot].t;
         const uint64_t t1 = stampNoPairs[slot] ? t[2] : t[1];
@@ -379,7 +376,20 @@ void Engine::fillStats(const FrameState& f, ans_stats* st, bool synced) {
         b += static_cast<double>(t[2] - t1) * msPerTick;
         c += static_cast<double>(t[3] - t[2]) * msPerTick;
     }
+    frames += count;
     stampPending = 0;
+}
+
+void Engine::fillStats(const FrameState& f, ans_stats* st, bool synced) {
+    if (!st) {
+        stampPending = 0;
+        return;
+    }
+    if (!synced) hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    double a = 0, b = 0, c = 0;
+    int frames = 0;
+    stageSums(a, b, c, frames);
+    frames = std::max(frames, 1);
     st->visible = f.visible;
     st->tile_pairs = f.pairs;
     st->project_ms = a / frames;
@@ -549,6 +559,7 @@ int ans_render_float(ans_renderer* r, const ans_camera* cam, const ans_options* 
         if (rgbf_out) r->imagef.reserve(3 * pixels);
         Engine& e = *r->engine;
         e.renderCounted(e.frame, r->scene, cp, o, r->image8.p, rgbf_out ? r->imagef.p : nullptr);
+        r->last = &e;
         if (rgb_out)
             hipCheck(hipMemcpyAsync(rgb_out, r->image8.p, 3 * pixels, hipMemcpyDeviceToHost, e.stream),
                      "hipMemcpyAsync image");
@@ -579,8 +590,29 @@ int ans_render_async(ans_renderer* r, const ans_camera* cam, const ans_options* 
         const ans_options o = resolveOptions(options);
         const CamParams cp = makeCamParams(*cam, o, r->scene.shDegree);
         if (cp.tilesX > 65535 || cp.tilesY > 65535) throw std::invalid_argument("too many tiles");
-        Engine& e = *r->engine;
-        e.enqueueFrame(e.frame, r->scene, cp, o, dev_rgb, dev_rgbf);
+        // Consecutive frames alternate between two engines (streams and frame buffers), so one
+        // frame's preprocess and sorts run while the previous frame blends; each blend waits for
+        // the previous frame's, so frames finish, and write their outputs, in enqueue order.
+        if (!r->engine2) {
+            r->engine2 = std::make_unique<Engine>();
+            r->engine2->frame.reservePairs(r->engine->frame.pairCap);  // as grown so far
+        }
+        Engine& e = r->asyncNext == 0 ? *r->engine : *r->engine2;
+        Engine* prev = r->lastAsync;
+        e.waitBeforeBlend = prev && prev != &e ? prev->blendDone : nullptr;
+        e.recordBlendDone = true;
+        try {
+            e.enqueueFrame(e.frame, r->scene, cp, o, dev_rgb, dev_rgbf);
+        } catch (...) {
+            e.waitBeforeBlend = nullptr;
+            e.recordBlendDone = false;
+            throw;
+        }
+        e.waitBeforeBlend = nullptr;
+        e.recordBlendDone = false;
+        r->asyncNext ^= 1;
+        r->lastAsync = &e;
+        r->last = &e;
         return ANS_OK;
     });
 }
@@ -588,10 +620,34 @@ int ans_render_async(ans_renderer* r, const ans_camera* cam, const ans_options* 
 int ans_render_wait(ans_renderer* r, ans_stats* stats) {
     return guarded([&] {
         if (!r) throw std::invalid_argument("renderer is null");
-        Engine& e = *r->engine;
-        const bool overflow = e.readBack(e.frame);
-        e.frame.retries = 0;
-        e.fillStats(e.frame, stats, true);
+        bool overflow = false;
+        double a = 0, b = 0, c = 0;
+        int frames = 0;
+        for (Engine* e : {r->engine.get(), r->engine2.get()}) {
+            if (!e) continue;
+            overflow |= e->readBack(e->frame);
+            e->frame.retries = 0;
+            e->stageSums(a, b, c, frames);
+        }
+        r->lastAsync = nullptr;
+        if (overflow && r->engine2) {
+            // Either engine may render the frames again: both get the larger capacity.
+            const size_t cap = std::max(r->engine->frame.pairCap, r->engine2->frame.pairCap);
+            r->engine->frame.reservePairs(cap);
+            r->engine2->frame.reservePairs(cap);
+        }
+        if (stats) {
+            const Engine& e = r->last ? *r->last : *r->engine;
+            frames = std::max(frames, 1);
+            stats->visible = e.frame.visible;
+            stats->tile_pairs = e.frame.pairs;
+            stats->project_ms = a / frames;
+            stats->sort_ms = b / frames;
+            stats->blend_ms = c / frames;
+            stats->pair_capacity = static_cast<int64_t>(e.frame.pairCap);
+            stats->retries = 0;
+            stats->reserved = 0;
+        }
         if (overflow) {
             g_lastError = "pair count exceeded the pair buffers; buffers grown, render again";
             return ANS_ERR_CAPACITY;
@@ -603,7 +659,7 @@ int ans_render_wait(ans_renderer* r, ans_stats* stats) {
 int ans_frame_counts(ans_renderer* r, int64_t* n, int64_t* visible, int64_t* pairs, int64_t* tiles) {
     return guarded([&] {
         if (!r) throw std::invalid_argument("renderer is null");
-        const FrameState& f = r->engine->frame;
+        const FrameState& f = (r->last ? *r->last : *r->engine).frame;
         if (n) *n = static_cast<int64_t>(f.n);
         if (visible) *visible = f.visible;
         if (pairs) *pairs = f.pairs;
@@ -616,7 +672,8 @@ int ans_frame_dump(ans_renderer* r, uint32_t* splat_bits, int32_t* rect, uint64_
                    uint32_t* ranges) {
     return guarded([&] {
         if (!r) throw std::invalid_argument("renderer is null");
-        dumpFrame(r->engine->stream, r->engine->frame, splat_bits, rect, keys, values, ranges);
+        const Engine& e = r->last ? *r->last : *r->engine;
+        dumpFrame(e.stream, e.frame, splat_bits, rect, keys, values, ranges);
         return ANS_OK;
     });
 }
