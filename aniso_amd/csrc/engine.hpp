@@ -5,6 +5,10 @@
 #include "forward.hpp"
 
 #include <cstddef>
+
+#ifndef ANS_ASYNC_LANES
+#define ANS_ASYNC_LANES 3
+#endif
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -185,10 +189,11 @@ struct ans_renderer {
     ans::DevBuf<uint8_t> image8;
     ans::DevBuf<float> imagef;
     std::unique_ptr<ans::Engine> engine;
-    // Asynchronous frames alternate between engine and engine2 (created on first use), each with
-    // its own stream and frame buffers, so that one frame's sorts overlap the other's blend;
-    // blends still run in enqueue order.
-    std::unique_ptr<ans::Engine> engine2;
+    // Asynchronous frames rotate over kLanes engines (engine, then lanes[1..], created on first
+    // use), each with its own stream and frame buffers, so that one frame's preprocess and sorts
+    // overlap the previous frame's blend; blends still run in enqueue order.
+    static constexpr int kLanes = ANS_ASYNC_LANES;
+    std::unique_ptr<ans::Engine> lanes[kLanes];
     int asyncNext = 0;
     ans::Engine* lastAsync = nullptr;  // engine of the last asynchronous frame not yet waited for
     ans::Engine* last = nullptr;       // engine of the last frame enqueued (counts, dumps)

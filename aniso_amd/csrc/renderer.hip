@@ -593,11 +593,13 @@ int ans_render_async(ans_renderer* r, const ans_camera* cam, const ans_options* 
         // Consecutive frames alternate between two engines (streams and frame buffers), so one
         // frame's preprocess and sorts run while the previous frame blends; each blend waits for
         // the previous frame's, so frames finish, and write their outputs, in enqueue order.
-        if (!r->engine2) {
-            r->engine2 = std::make_unique<Engine>();
-            r->engine2->frame.reservePairs(r->engine->frame.pairCap);  // as grown so far
+        Engine* lane = r->asyncNext == 0 ? r->engine.get() : r->lanes[r->asyncNext].get();
+        if (!lane) {
+            r->lanes[r->asyncNext] = std::make_unique<Engine>();
+            lane = r->lanes[r->asyncNext].get();
+            lane->frame.reservePairs(r->engine->frame.pairCap);  // as grown so far
         }
-        Engine& e = r->asyncNext == 0 ? *r->engine : *r->engine2;
+        Engine& e = *lane;
         Engine* prev = r->lastAsync;
         e.waitBeforeBlend = prev && prev != &e ? prev->blendDone : nullptr;
         e.recordBlendDone = true;
@@ -610,7 +612,7 @@ int ans_render_async(ans_renderer* r, const ans_camera* cam, const ans_options* 
         }
         e.waitBeforeBlend = nullptr;
         e.recordBlendDone = false;
-        r->asyncNext ^= 1;
+        r->asyncNext = (r->asyncNext + 1) % ans_renderer::kLanes;
         r->lastAsync = &e;
         r->last = &e;
         return ANS_OK;
@@ -623,18 +625,22 @@ int ans_render_wait(ans_renderer* r, ans_stats* stats) {
         bool overflow = false;
         double a = 0, b = 0, c = 0;
         int frames = 0;
-        for (Engine* e : {r->engine.get(), r->engine2.get()}) {
+        for (int k = 0; k < ans_renderer::kLanes; ++k) {
+            Engine* e = k == 0 ? r->engine.get() : r->lanes[k].get();
             if (!e) continue;
             overflow |= e->readBack(e->frame);
             e->frame.retries = 0;
             e->stageSums(a, b, c, frames);
         }
         r->lastAsync = nullptr;
-        if (overflow && r->engine2) {
-            // Either engine may render the frames again: both get the larger capacity.
-            const size_t cap = std::max(r->engine->frame.pairCap, r->engine2->frame.pairCap);
+        if (overflow) {
+            // Any engine may render the frames again: all get the largest capacity.
+            size_t cap = r->engine->frame.pairCap;
+            for (int k = 1; k < ans_renderer::kLanes; ++k)
+                if (r->lanes[k]) cap = std::max(cap, r->lanes[k]->frame.pairCap);
             r->engine->frame.reservePairs(cap);
-            r->engine2->frame.reservePairs(cap);
+            for (int k = 1; k < ans_renderer::kLanes; ++k)
+                if (r->lanes[k]) r->lanes[k]->frame.reservePairs(cap);
         }
         if (stats) {
             const Engine& e = r->last ? *r->last : *r->engine;

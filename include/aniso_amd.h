@@ -126,9 +126,12 @@ int ans_render_float(ans_renderer* r, const ans_camera* cam, const ans_options* 
                      float* rgbf_out, size_t pixels_capacity, ans_stats* stats);
 
 /* Asynchronous variant for pipelined callers (the reference's `aniso path` loop renders one
- * view after another, main.cpp:234-244): enqueues the frame on the handle's stream and returns
- * without waiting. dev_rgb (width*height*3 bytes) and dev_rgbf (width*height*3 floats) are
- * DEVICE pointers; either may be NULL. Call ans_render_wait before reading them. */
+ * view after another, main.cpp:234-244): enqueues the frame and returns without waiting.
+ * Consecutive frames rotate over three internal streams with their own frame buffers, so a
+ * frame's preprocess and sorts overlap the previous frames' blends; blends run in enqueue order,
+ * so frames finish (and write their outputs) in the order they were enqueued. dev_rgb
+ * (width*height*3 bytes) and dev_rgbf (width*height*3 floats) are DEVICE pointers; either may
+ * be NULL. Call ans_render_wait before reading them. */
 int ans_render_async(ans_renderer* r, const ans_camera* cam, const ans_options* options, uint8_t* dev_rgb,
                      float* dev_rgbf);
 
