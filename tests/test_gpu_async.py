@@ -84,3 +84,24 @@ def test_sync_and_async_frames_interleaved(gpu):
             _lib.check(L.ans_render_wait(r._h, None))
             torch.cuda.synchronize()
             assert np.array_equal(out.cpu().numpy().reshape(H, W, 3), want[k]), f"async frame {k}"
+
+
+def test_async_full_size_clustered_1080p(gpu):
+    """2.2M clustered Gaussians at 1920x1080 (11.5M pairs per frame): three poses pipelined over
+    the three streams equal their synchronous renders."""
+    w, h = 1920, 1080
+    raw = scenes.synthetic_scene(2_200_000, w, h, clustered=True)
+    cams = [scenes.pose_camera([0.04 * i, -0.02 * i, 0.05 * i], [0.01 * i, 0.0, 1.0], [0, -1, 0], w, h, 60.0)
+            for i in range(3)]
+    L = _lib.load()
+    with GpuRenderer(raw) as r:
+        want = [r.render(c) for c in cams]
+        outs = [torch.empty(w * h * 3, dtype=torch.uint8, device="cuda") for _ in cams]
+        opt = RenderOptions().c()
+        for _ in range(2):
+            for c, o in zip(cams, outs):
+                _lib.check(L.ans_render_async(r._h, camera_c(c), opt, o.data_ptr(), None))
+        _lib.check(L.ans_render_wait(r._h, None))
+        torch.cuda.synchronize()
+        for i, (o, img) in enumerate(zip(outs, want)):
+            assert np.array_equal(o.cpu().numpy().reshape(h, w, 3), img), f"frame {i}"
