@@ -24,7 +24,7 @@ namespace ans {
 constexpr int kSortBlock = 256;
 constexpr int kRadix = 256;
 constexpr int kSortWaves = kSortBlock / kWave;
-constexpr int kDepthItems = 4;  // 1024 keys per sort block (per-Gaussian sorts)
+constexpr int kDepthItems = 8;  // 2048 keys per sort block (per-Gaussian sorts)
 constexpr int kPairItems = 16;  // 4096 keys per sort block (pair sorts)
 
 // Inclusive scan of one value per lane across a 64-lane wavefront.
