@@ -136,7 +136,7 @@ void Engine::reserveScratch(size_t n, size_t pairCap) {
     ttByRank.reserve(m); rectByRank.reserve(m);
     const size_t blocksDepth = (n + kSortBlock * kDepthItems - 1) / (kSortBlock * kDepthItems);
     const size_t blocksDup = (n + kDupTile - 1) / kDupTile;
-    const size_t blocksPair = (pairCap + kSortBlock * kPairItems - 1) / (kSortBlock * kPairItems);
+    const size_t blocksPair = (pairCap + kSortBlock * kTileItems - 1) / (kSortBlock * kTileItems);
     hist.reserve(kRadix * std::max({blocksDepth, blocksPair, size_t(1)}));
     rowTotal.reserve(kRadix);
     blockPairs.reserve(std::max<size_t>(blocksDup, 1));
@@ -234,18 +234,18 @@ void Engine::enqueueRest(FrameState& f, uint8_t* out8, float* outf) {
                                                            blockPairs.p, f.tileA.p, f.gaussA.p, ctl, 0, 0, 0,
                                                            nullptr);
         const uint32_t blocksPair =
-            static_cast<uint32_t>((f.pairCap + kSortBlock * kPairItems - 1) / (kSortBlock * kPairItems));
+            static_cast<uint32_t>((f.pairCap + kSortBlock * kTileItems - 1) / (kSortBlock * kTileItems));
         uint32_t* tk[2] = {f.tileA.p, f.tileB.p};
         uint32_t* tv[2] = {f.gaussA.p, f.gaussB.p};
         for (int pass = 0; pass < passes; ++pass) {
             const int src = pass & 1, dst = src ^ 1;
             const int shift = pass * digitBits;
             const int bits = std::min(digitBits, tileBits - shift);
-            k_radixUpsweep<kPairItems><<<blocksPair, kSortBlock, 0, stream>>>(tk[src], true, 0, cap, shift, bits,
+            k_radixUpsweep<kTileItems><<<blocksPair, kSortBlock, 0, stream>>>(tk[src], true, 0, cap, shift, bits,
                                                                               hist.p, blocksPair, ctl, 0, 0);
             k_scanRows<<<1u << bits, kSortBlock, 0, stream>>>(hist.p, blocksPair, blocksPair, rowTotal.p, ctl, true,
-                                                              cap, kSortBlock * kPairItems, 0);
-            k_radixDownsweep<kPairItems><<<blocksPair, kSortBlock, 0, stream>>>(
+                                                              cap, kSortBlock * kTileItems, 0);
+            k_radixDownsweep<kTileItems><<<blocksPair, kSortBlock, 0, stream>>>(
                 tk[src], tv[src], tk[dst], tv[dst], true, 0, cap, shift, bits, hist.p, rowTotal.p, blocksPair, ctl,
                 0, 0);
         }

@@ -25,7 +25,8 @@ constexpr int kSortBlock = 256;
 constexpr int kRadix = 256;
 constexpr int kSortWaves = kSortBlock / kWave;
 constexpr int kDepthItems = 8;  // 2048 keys per sort block (per-Gaussian sorts)
-constexpr int kPairItems = 16;  // 4096 keys per sort block (pair sorts)
+constexpr int kPairItems = 16;  // 4096 keys per sort block (the batched views' sorts)
+constexpr int kTileItems = 8;   // 2048 keys per sort block (a single frame's tile sort)
 
 // Inclusive scan of one value per lane across a 64-lane wavefront.
 __device__ inline uint32_t waveInclusiveScan(uint32_t v) {
