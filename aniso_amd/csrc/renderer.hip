@@ -123,6 +123,9 @@ Engine::Engine() {
 }
 
 Engine::~Engine() {
+    // Kernels still in flight write the pinned count mirror and stamp ring: let them finish first
+    // (a caller's stream, used by the differentiable path, is drained by ans_diff_destroy).
+    if (ownStream) (void)hipStreamSynchronize(ownStream);
     if (ctlHost) (void)hipHostFree(ctlHost);
     if (mirror) (void)hipHostFree(mirror);
     if (stampsHost) (void)hipHostFree(stampsHost);

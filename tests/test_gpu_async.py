@@ -105,3 +105,24 @@ def test_async_full_size_clustered_1080p(gpu):
         torch.cuda.synchronize()
         for i, (o, img) in enumerate(zip(outs, want)):
             assert np.array_equal(o.cpu().numpy().reshape(h, w, 3), img), f"frame {i}"
+
+
+def test_destroy_with_frames_in_flight(gpu):
+    """Destroying a handle right after enqueueing frames (no wait) drains them first; a new
+    handle then renders correctly."""
+    raw = scenes.synthetic_scene(200_000, W, H, seed=8)
+    cam = cameras(1)[0]
+    L = _lib.load()
+    out = torch.empty(W * H * 3, dtype=torch.uint8, device="cuda")
+    opt = RenderOptions().c()
+    for _ in range(3):
+        r = GpuRenderer(raw)
+        for _ in range(5):
+            _lib.check(L.ans_render_async(r._h, camera_c(cam), opt, out.data_ptr(), None))
+        r.close()
+    with GpuRenderer(raw) as r:
+        want = r.render(cam)
+        _lib.check(L.ans_render_async(r._h, camera_c(cam), opt, out.data_ptr(), None))
+        _lib.check(L.ans_render_wait(r._h, None))
+        torch.cuda.synchronize()
+        assert np.array_equal(out.cpu().numpy().reshape(H, W, 3), want)
