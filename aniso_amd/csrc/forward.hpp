@@ -595,7 +595,12 @@ static __global__ __launch_bounds__(256) void k_tileRanges(const uint32_t* __res
 __device__ inline float quadMinOverBox(float A, float B, float C, float rB_C, float rB_A, float dx0, float dx1,
                                        float dy0, float dy1) {
     if (dx0 <= 0.0f && dx1 >= 0.0f && dy0 <= 0.0f && dy1 >= 0.0f) return 0.0f;
-    auto q = [&](float dx, float dy) { return A * dx * dx + 2.0f * B * dx * dy + C * dy * dy; };
+    // dx (A dx + 2B dy) + C dy^2 with fused multiply-adds: a culling bound, compared against a
+    // threshold widened far beyond any rounding, so the operation order is free here.
+    const float B2 = B + B;
+    auto q = [&](float dx, float dy) {
+        return __builtin_fmaf(dx, __builtin_fmaf(A, dx, B2 * dy), dy * (C * dy));
+    };
     const float a = q(dx0, fminf(fmaxf(rB_C * dx0, dy0), dy1));
     const float b = q(dx1, fminf(fmaxf(rB_C * dx1, dy0), dy1));
     const float c = q(fminf(fmaxf(rB_A * dy0, dx0), dx1), dy0);
