@@ -410,7 +410,7 @@ __host__ __device__ inline void gaussAdjoint(const GaussProj& g, const CamParams
 }
 
 // Pair slots in Gaussian index order: per block of 256 Gaussians, the sum of their tile counts.
-static __global__ __launch_bounds__(256) void k_slotBlockSums(uint32_t n, const uint32_t* __restrict__ tilesTouched,
+[[maybe_unused]] static __global__ __launch_bounds__(256) void k_slotBlockSums(uint32_t n, const uint32_t* __restrict__ tilesTouched,
                                                               uint32_t* __restrict__ blockSum) {
     __shared__ uint32_t sTmp[256 / kWave];
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
@@ -420,7 +420,7 @@ static __global__ __launch_bounds__(256) void k_slotBlockSums(uint32_t n, const 
 }
 
 // One workgroup: exclusive scan of the per-block sums in place.
-static __global__ __launch_bounds__(256) void k_scanBlockSums(uint32_t* __restrict__ blockSum, uint32_t blocks) {
+[[maybe_unused]] static __global__ __launch_bounds__(256) void k_scanBlockSums(uint32_t* __restrict__ blockSum, uint32_t blocks) {
     __shared__ uint32_t sTmp[256 / kWave];
     constexpr int kPer = 8;
     uint32_t carry = 0;
@@ -444,7 +444,7 @@ static __global__ __launch_bounds__(256) void k_scanBlockSums(uint32_t* __restri
 }
 
 // slotStart[i]: the first pair slot of Gaussian i (exclusive scan of tile counts in index order).
-static __global__ __launch_bounds__(256) void k_slotStart(uint32_t n, const uint32_t* __restrict__ tilesTouched,
+[[maybe_unused]] static __global__ __launch_bounds__(256) void k_slotStart(uint32_t n, const uint32_t* __restrict__ tilesTouched,
                                                           const uint32_t* __restrict__ blockBase,
                                                           uint32_t* __restrict__ slotStart) {
     __shared__ uint32_t sTmp[256 / kWave];
@@ -471,7 +471,7 @@ __device__ inline int rowOf(int e, float rcpK3) {
 // way. gaussAdjoint then gives the gradients of means, log-scales, quaternions (w, x, y, z),
 // opacity logits and SH [coefficient][channel]; the summed splat-record gradient goes to
 // splatGrad. Culled Gaussians get zero gradients.
-static __global__ __launch_bounds__(kWave) void k_gaussBackward(
+[[maybe_unused]] static __global__ __launch_bounds__(kWave) void k_gaussBackward(
     int n, const float* __restrict__ pos, const float* __restrict__ logScale, const float4* __restrict__ quat,
     const float* __restrict__ logit, const float* __restrict__ sh, CamParams cam,
     const uint32_t* __restrict__ tilesTouched, const uint32_t* __restrict__ slotStart,

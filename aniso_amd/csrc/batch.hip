@@ -152,19 +152,30 @@ void reserveArena(ViewBatch& B, const Layout& L) {
     moved |= grow(B.table, TableLayout(L.views).bytes);
     moved |= grow(B.ctl, V + 1);
     moved |= grow(B.rec, vn);
-    moved |= grow(B.depthKey, vn) | grow(B.order, vn);
-    moved |= grow(B.kA, vn) | grow(B.vA, vn) | grow(B.kB, vn) | grow(B.vB, vn);
-    moved |= grow(B.ttByRank, vn) | grow(B.rectByRank, vn);
+    moved |= grow(B.depthKey, vn);
+    moved |= grow(B.order, vn);
+    moved |= grow(B.kA, vn);
+    moved |= grow(B.vA, vn);
+    moved |= grow(B.kB, vn);
+    moved |= grow(B.vB, vn);
+    moved |= grow(B.ttByRank, vn);
+    moved |= grow(B.rectByRank, vn);
     const size_t histD = V * kRadix * size_t(L.blocksD), histP = V * kRadix * size_t(L.blocksP),
                  histI = size_t(kRadix) * L.blocksI;
     moved |= grow(B.hist, std::max({histD, histP, histI}));
     moved |= grow(B.rowTotal, (V + 1) * kRadix);
-    moved |= grow(B.blockPairs, V * L.blocksDup) | grow(B.blockVisible, V * L.blocksDup);
+    moved |= grow(B.blockPairs, V * L.blocksDup);
+    moved |= grow(B.blockVisible, V * L.blocksDup);
     const size_t vp = V * L.pairCap;
-    moved |= grow(B.tileA, vp) | grow(B.gaussA, vp) | grow(B.tileB, vp) | grow(B.gaussB, vp);
+    moved |= grow(B.tileA, vp);
+    moved |= grow(B.gaussA, vp);
+    moved |= grow(B.tileB, vp);
+    moved |= grow(B.gaussB, vp);
     moved |= grow(B.ranges, size_t(L.items));
-    moved |= grow(B.itemKey, L.items) | grow(B.itemK2, L.items) | grow(B.itemV2, L.items) |
-             grow(B.itemOrder, L.items);
+    moved |= grow(B.itemKey, L.items);
+    moved |= grow(B.itemK2, L.items);
+    moved |= grow(B.itemV2, L.items);
+    moved |= grow(B.itemOrder, L.items);
     if (moved) {
         B.dropGraphs();
         ++B.generation;

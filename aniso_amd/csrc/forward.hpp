@@ -333,7 +333,7 @@ constexpr uint32_t kBigPairs = 32;
 // Pairs and visible Gaussians per block of 256 depth ranks. Also gathers each rank's tile count
 // and tile rect (one line of its splat record) into depth order (ttByRank, rectByRank; rect left
 // unwritten for culled ranks), so that the expansion reads them with coalesced loads.
-static __global__ __launch_bounds__(kDupBlock) void k_pairTotals(const uint32_t* __restrict__ sortedIdx,
+[[maybe_unused]] static __global__ __launch_bounds__(kDupBlock) void k_pairTotals(const uint32_t* __restrict__ sortedIdx,
                                                                  const SplatRec* __restrict__ rec, uint32_t n,
                                                                  uint32_t* __restrict__ ttByRank,
                                                                  uint2* __restrict__ rectByRank,
@@ -370,7 +370,7 @@ static __global__ __launch_bounds__(kDupBlock) void k_pairTotals(const uint32_t*
 // One workgroup: exclusive scan of the per-block pair counts; publishes the pair total (the sum
 // gpu_render.cu:324-329 reads back to the host) and the visible count, and flags a pair total
 // above the pair buffers' capacity; with mirror, also to pinned host memory under number seq.
-static __global__ __launch_bounds__(kDupBlock) void k_scanPairs(uint32_t* __restrict__ blockPairs,
+[[maybe_unused]] static __global__ __launch_bounds__(kDupBlock) void k_scanPairs(uint32_t* __restrict__ blockPairs,
                                                                 uint32_t* __restrict__ blockVisible,
                                                                 uint32_t blocks, uint32_t cap,
                                                                 FrameCtl* __restrict__ ctl, uint32_t blockStride,
@@ -466,7 +466,7 @@ __device__ inline void writeRectPairs(uint32_t first, uint32_t g, uint32_t rectL
 }
 
 // tilesXView (batched views, else null) holds each view's tile-grid width.
-static __global__ __launch_bounds__(kDupBlock) void k_expandPairs(
+[[maybe_unused]] static __global__ __launch_bounds__(kDupBlock) void k_expandPairs(
     const uint32_t* __restrict__ sortedIdx, const uint32_t* __restrict__ ttByRank,
     const uint2* __restrict__ rectByRank, uint32_t n, int tilesX, const uint32_t* __restrict__ blockBase,
     uint32_t* __restrict__ tileOut, uint32_t* __restrict__ gaussOut, const FrameCtl* __restrict__ ctl,
@@ -560,7 +560,7 @@ static __global__ __launch_bounds__(kDupBlock) void k_expandPairs(
 
 // Start and end of each tile's run in the sorted pairs (gpu_render.cu:164-177); ranges were
 // cleared to [0, 0) by the preprocess. Grid-stride over groups of four pairs.
-static __global__ __launch_bounds__(256) void k_tileRanges(const uint32_t* __restrict__ sortedTile,
+[[maybe_unused]] static __global__ __launch_bounds__(256) void k_tileRanges(const uint32_t* __restrict__ sortedTile,
                                                            const FrameCtl* __restrict__ ctl,
                                                            uint2* __restrict__ ranges, uint32_t pairStride,
                                                            uint32_t tileStride) {

@@ -130,7 +130,7 @@ __global__ __launch_bounds__(kSortBlock) void k_radixUpsweep(const uint32_t* __r
 
 // Exclusive scan of one row hist[bin * stride, + blocks holding items) in place by one workgroup
 // (blockIdx.x = bin); the row total goes to rowTotal[bin]. tileItems: items per sort block.
-static __global__ __launch_bounds__(kSortBlock) void k_scanRows(uint32_t* __restrict__ hist, uint32_t stride,
+[[maybe_unused]] static __global__ __launch_bounds__(kSortBlock) void k_scanRows(uint32_t* __restrict__ hist, uint32_t stride,
                                                                 uint32_t len, uint32_t* __restrict__ rowTotal,
                                                                 const FrameCtl* __restrict__ ctl, bool countFromCtl,
                                                                 uint32_t cap, uint32_t tileItems,

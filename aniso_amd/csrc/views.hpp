@@ -9,7 +9,7 @@
 namespace ans {
 
 // Clears the first ctlCount control blocks (all but sticky) and rangeCount tile ranges.
-static __global__ __launch_bounds__(256) void k_initViews(FrameCtl* __restrict__ ctl, uint32_t ctlCount,
+[[maybe_unused]] static __global__ __launch_bounds__(256) void k_initViews(FrameCtl* __restrict__ ctl, uint32_t ctlCount,
                                                           uint2* __restrict__ ranges, uint32_t rangeCount) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i < ctlCount) {
@@ -27,7 +27,7 @@ static __global__ __launch_bounds__(256) void k_initViews(FrameCtl* __restrict__
 // into each view in turn (cameraTransform,
 // finishProjection): the arithmetic of the single-view k_preprocess. Outputs for view v at
 // v * n + i.
-static __global__ __launch_bounds__(256) void k_preprocessViews(
+[[maybe_unused]] static __global__ __launch_bounds__(256) void k_preprocessViews(
     int n, int views, const float* __restrict__ pos, const float* __restrict__ scale, const float4* __restrict__ rot,
     const float* __restrict__ opacity, const float* __restrict__ sh, int shCoeffs, const CamParams* __restrict__ cams,
     SplatRec* __restrict__ rec, uint32_t* __restrict__ tilesTouched, uint32_t* __restrict__ depthKey) {
@@ -60,7 +60,7 @@ static __global__ __launch_bounds__(256) void k_preprocessViews(
 
 // Sort key of item v * maxTiles + t for longest-first order: 0x7fff - list length (lengths above
 // 0x7fff count as 0x7fff); items past view v's tile count get 0xffff and come last.
-static __global__ __launch_bounds__(256) void k_itemKeys(const uint2* __restrict__ ranges,
+[[maybe_unused]] static __global__ __launch_bounds__(256) void k_itemKeys(const uint2* __restrict__ ranges,
                                                          const int* __restrict__ tilesView, uint32_t maxTiles,
                                                          uint32_t items, uint32_t* __restrict__ keys) {
     const uint32_t it = blockIdx.x * 256 + threadIdx.x;
