@@ -68,6 +68,10 @@ __device__ inline void pairSumStore(float (&d1)[9], float (&d2)[9], float* rowBa
     for (int k = 0; k < 5; ++k) u[k] = dppAdd<0x141>(u[k]);
 #pragma unroll
     for (int k = 0; k < 5; ++k) u[k] = dppAdd<0x140>(u[k]);
+    // An empty use of each sum outside the branch below: otherwise the compiler moves the last
+    // DPP add into the branch and splits it into a register clear, a DPP move and an add.
+#pragma unroll
+    for (int k = 0; k < 5; ++k) asm volatile("" : : "v"(u[k]));
     if ((__lane_id() & 15) == 15) {
 #pragma unroll
         for (int k = 0; k < 5; ++k) rowBase[k] = u[k];
