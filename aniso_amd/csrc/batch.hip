@@ -583,8 +583,12 @@ void renderPath(ans_renderer* r, int64_t nFrames, const ans_camera* cams, const 
         }
     } catch (...) {
         drain();
+        B.views = 0;
         throw;
     }
+    // A path leaves no batch to inspect: an arena growth while a chunk was in flight would leave
+    // that chunk's lists in freed buffers, so the per-view counts and dumps refuse after a path.
+    B.views = 0;
     st.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (stats) *stats = st;
 }

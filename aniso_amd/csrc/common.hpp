@@ -56,7 +56,11 @@ struct CountMirror {
     uint32_t pairs;
     uint32_t visible;
     uint32_t overflow;
-    uint32_t sticky;  // set with overflow and left set: the host clears it when it has read it
+    // Frames on this stream whose pairs exceeded the buffers and that the host has not handled
+    // yet, and the largest pair count among them (the buffers must grow to that, not to the
+    // count of whichever frame ran last). The scan increments them; the host clears them.
+    uint32_t sticky;
+    uint32_t stickyPairs;
 };
 
 // Stage timestamps of one frame, read from the GPU's constant-rate wall clock (s_memrealtime) by

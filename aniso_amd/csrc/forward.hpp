@@ -416,7 +416,11 @@ constexpr uint32_t kBigPairs = 32;
             mirror->pairs = pairs;
             mirror->visible = visible;
             mirror->overflow = carry > cap ? 1u : 0u;
-            if (carry > cap) mirror->sticky = 1u;
+            // Frames of one stream run in order, so a plain read-modify-write is enough.
+            if (carry > cap) {
+                mirror->sticky += 1u;
+                mirror->stickyPairs = max(mirror->stickyPairs, pairs);
+            }
             __hip_atomic_store(&mirror->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
         }
     }

@@ -302,8 +302,11 @@ bool Engine::readBack(FrameState& f) {
     f.visible = f.n ? mirror->visible : 0;
     f.pairs = f.n ? mirror->pairs : 0;
     if (mirror->sticky) {
+        // Grow to the largest count of the frames that overflowed, which need not be the last
+        // frame's.
+        const uint32_t pairs = mirror->stickyPairs;
         mirror->sticky = 0;
-        const uint32_t pairs = mirror->pairs;
+        mirror->stickyPairs = 0;
         if (pairs >= (1u << 30) - 1) throw std::runtime_error("pair count exceeds 2^30");
         f.reservePairs(static_cast<size_t>(pairs) + pairs / 4 + 1024);
         return true;
@@ -356,7 +359,10 @@ int Engine::renderCounted(FrameState& f, const SceneDev& s, const CamParams& cp,
     if (m.overflow) {
         if (m.pairs >= (1u << 30) - 1) throw std::runtime_error("pair count exceeds 2^30");
         hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");  // the skipped kernels have ended
-        mirror->sticky = 0;  // handled here, not by a later wait
+        // This frame's overflow is handled here; overflows of earlier asynchronous frames on the
+        // stream stay for their wait to report.
+        if (mirror->sticky) mirror->sticky -= 1;
+        if (!mirror->sticky) mirror->stickyPairs = 0;
         f.reservePairs(static_cast<size_t>(m.pairs) + m.pairs / 4 + 1024);
         // The flags only stopped the kernels of this frame; with room now, clear them (and the
         // blend's workgroup counter, which the skipped blend advanced).

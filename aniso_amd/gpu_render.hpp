@@ -52,7 +52,7 @@ public:
     GpuRenderer(const GpuRenderer&) = delete;
     GpuRenderer& operator=(const GpuRenderer&) = delete;
 
-    // Timings in stats are GPU time per stage, measured with HIP events.
+    // Timings in stats are GPU time per stage, from clock readings the frame's own kernels take.
     Image render(const Camera& camera, RenderStats* stats = nullptr, const RenderOptions& options = {}) {
         const ans_camera c = toC(camera);
         ans_options o{};

@@ -65,7 +65,7 @@ def main():
     c3 = oracle.activate(scenes.synthetic_scene(2_200_000, 1280, 720, clustered=True))
     for w, hh in ((1280, 720), (1920, 1080)):
         H[f"image:clustered-2.2M:{w}x{hh}:sh3"] = h(oracle.ref_render(c3, scenes.pinhole_camera(w, hh))[0].tobytes())
-    # The reference GPU test's scene and cameras (test_gpu_render.cpp:173-179, :221).
+    # The reference GPU test's scene and cameras (test_gpu_render.cpp:41-58 for the scene, :21-27 and :69-70 for the cameras).
     mt = oracle.mt19937_scene(20000, 7)
     for w, hh in ((320, 240), (333, 217)):
         c = scenes.Camera(np.eye(3), np.zeros(3), 0.8 * w, 0.8 * w, w / 2.0, hh / 2.0, w, hh)

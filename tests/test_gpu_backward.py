@@ -117,6 +117,48 @@ def test_backward_is_deterministic(gpu):
     assert np.array_equal(a, b)
 
 
+def high_opacity_case(seed=13):
+    """Opacity logits shifted by +4 (opacity about 0.98 on average, so most alphas near a splat's
+    centre reach the 0.99 clamp) plus 40 splats made about ten times larger, so that many pixels
+    stop on T < 1e-4 after a few clamped splats: the regime of trained scenes, where the
+    transmittance the adjoint recovers by division through (1 - alpha) is most sensitive."""
+    raw, cam = l1_case(20000, 320, 180, seed=seed, scale_boost=1.5)
+    raw.opacity_logits += np.float32(4.0)
+    rng = np.random.default_rng(seed)
+    big = rng.choice(raw.n, 40, replace=False)
+    raw.log_scales[big] += np.float32(2.3)
+    return raw, cam
+
+
+def test_high_opacity_gradients_match_fp64(gpu):
+    raw, cam = high_opacity_case()
+    params = tensors(raw)
+    start, values, _ = frame_lists(*[p.detach() for p in params], cam)
+    img0, _, _ = oracle.grad_run(raw, cam, np.zeros((cam.height, cam.width, 3)), lists=(start, values))
+    rng = np.random.default_rng(4)
+    dLdC = np.sign(img0 - (img0 + rng.normal(0, 0.05, img0.shape)))
+    img, g = gpu_grads(params, cam, dLdC)
+    ref_img, ref, _ = oracle.grad_run(raw, cam, dLdC, lists=(start, values))
+    op = 1.0 / (1.0 + np.exp(-raw.opacity_logits.astype(np.float64)))
+    print(f"case high-opacity: mean opacity {op.mean():.3f}, image |GPU - FP64| max "
+          f"{np.abs(img - ref_img).max():.2e}", flush=True)
+    assert np.abs(img - ref_img).max() < 1.0 / 255.0
+    assert compare("vs FP64 reference", g, ref) < TOL
+    idx = rng.choice(np.flatnonzero(np.abs(ref).sum(1) > 0), 150, replace=False)
+    fd, flags = oracle.grad_fd(raw, cam, dLdC, idx)
+    ok = flags == 0
+    print(f"  FD samples kept {ok.mean():.3f}", flush=True)
+    worst = 0.0
+    for name, sl in oracle.PARAM_GROUPS.items():
+        a, b = g[idx][:, sl][ok[:, sl]], fd[:, sl][ok[:, sl]]
+        if b.size == 0:
+            continue
+        err = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        print(f"  vs FP64 central differences {name:15s} rel {err:.2e}", flush=True)
+        worst = max(worst, err)
+    assert worst < TOL
+
+
 @pytest.fixture(scope="module")
 def c4_case():
     """741,883 Gaussians (SH3) at 1280x720; L1 against a target rendered after perturbing the

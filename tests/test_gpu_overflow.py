@@ -75,6 +75,38 @@ def test_async_render_reports_overflow_then_renders(gpu):
         assert np.array_equal(dev.cpu().numpy().reshape(H, W, 3), want)
 
 
+def test_async_overflow_grows_to_the_largest_frame(gpu):
+    """Frames [big, empty, empty, empty] on the three asynchronous streams: the fourth frame runs
+    on the first frame's stream after it, so the buffers must grow to the overflowing frame's
+    pair count, not to the count of the frame that ran last on that stream. The wait reports
+    ANS_ERR_CAPACITY once; the same four frames rendered again all succeed and equal their
+    synchronous renders."""
+    raw = wide_scene()
+    big = scenes.pinhole_camera(W, H)
+    behind = scenes.pinhole_camera(W, H)
+    behind.R = np.diag([-1.0, 1.0, -1.0])
+    cams = [big, behind, behind, behind]
+    L = _lib.load()
+    with GpuRenderer(raw) as ref_r:
+        want = [ref_r.render(c) for c in cams]
+    assert want[0].any() and not any(w.any() for w in want[1:])
+    outs = [torch.full((W * H * 3,), 7, dtype=torch.uint8, device="cuda") for _ in cams]
+    c_opt = RenderOptions().c()
+    with GpuRenderer(raw) as r:
+        for c, o in zip(cams, outs):
+            _lib.check(L.ans_render_async(r._h, camera_c(c), c_opt, o.data_ptr(), None))
+        rc = L.ans_render_wait(r._h, None)
+        assert rc == 3, rc  # ANS_ERR_CAPACITY
+        for c, o in zip(cams, outs):
+            _lib.check(L.ans_render_async(r._h, camera_c(c), c_opt, o.data_ptr(), None))
+        st = _lib.Stats_()
+        _lib.check(L.ans_render_wait(r._h, ctypes.byref(st)))
+        torch.cuda.synchronize()
+        assert st.pair_capacity > (1 << 20)
+        for k, (o, w) in enumerate(zip(outs, want)):
+            assert np.array_equal(o.cpu().numpy().reshape(H, W, 3), w), f"frame {k}"
+
+
 def test_differentiable_forward_retries(gpu):
     raw = wide_scene()
     cam = scenes.pinhole_camera(W, H)

@@ -68,6 +68,16 @@ def test_reference_mode_matches_c2():
         assert (f.visible, f.pairs) == (741_883, 8_664_728)  # counts the compiled reference prints
 
 
+def test_reference_mode_matches_c3_both_sizes():
+    """2.2M clustered Gaussians at 1280x720 and 1920x1080: the compiled reference's image hashes
+    and the visible and 8x8 pair counts it prints."""
+    act = oracle.activate(scenes.synthetic_scene(2_200_000, 1280, 720, clustered=True))
+    for (w, hh), counts in (((1280, 720), (1_756_407, 18_204_303)), ((1920, 1080), (1_755_622, 32_844_327))):
+        with oracle.Frame(act, scenes.pinhole_camera(w, hh), oracle.REFERENCE, tile=(8, 8)) as f:
+            assert h(f.image().tobytes()) == GOLDEN[f"image:clustered-2.2M:{w}x{hh}:sh3"]
+            assert (f.visible, f.pairs) == counts
+
+
 def test_reference_mode_matches_mt19937_scene(mt):
     for w, hh in ((320, 240), (333, 217)):
         for deg in (3, 0):
