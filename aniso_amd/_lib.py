@@ -38,7 +38,7 @@ class Stats_(ctypes.Structure):
 class Params_(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int64), ("sh_degree", ctypes.c_int32), ("means", ctypes.c_void_p),
                 ("log_scales", ctypes.c_void_p), ("quats", ctypes.c_void_p), ("opacity_logits", ctypes.c_void_p),
-                ("sh", ctypes.c_void_p)]
+                ("sh", ctypes.c_void_p), ("activated", ctypes.c_int32)]
 
 
 class Grads_(ctypes.Structure):
@@ -97,6 +97,15 @@ SIGNATURES = {
     "ans_diff_screen_grads": (ctypes.c_int, [P, ctypes.c_int64, P, P]),
     "ans_diff_frame_counts": (ctypes.c_int, [P, ctypes.c_int64, P, P, P]),
     "ans_diff_frame_dump": (ctypes.c_int, [P, ctypes.c_int64, P, P]),
+    "ans_diff_views_forward": (ctypes.c_int, [P, ctypes.POINTER(Params_), ctypes.c_int32, ctypes.POINTER(Camera_),
+                                              ctypes.POINTER(Options_), P, P, ctypes.POINTER(ctypes.c_int64),
+                                              ctypes.POINTER(Stats_)]),
+    "ans_diff_views_backward": (ctypes.c_int, [P, ctypes.c_int64, ctypes.POINTER(Params_), P, ctypes.POINTER(Grads_),
+                                               P, P]),
+    "ans_diff_views_alpha": (ctypes.c_int, [P, ctypes.c_int64, P, P]),
+    "ans_diff_views_radii": (ctypes.c_int, [P, ctypes.c_int64, P, P]),
+    "ans_diff_views_counts": (ctypes.c_int, [P, ctypes.c_int64, ctypes.c_int32, P, P, P]),
+    "ans_diff_views_dump": (ctypes.c_int, [P, ctypes.c_int64, ctypes.c_int32, P, P]),
     "ans_ply_write": (ctypes.c_int, [ctypes.POINTER(Params_), P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), P]),
 }
 

@@ -39,11 +39,11 @@ def _handle(device_index: int):
         return h
 
 
-def _params(means, log_scales, quats, opacity_logits, sh) -> _lib.Params_:
+def _params(means, log_scales, quats, opacity_logits, sh, activated: bool = False) -> _lib.Params_:
     n = means.shape[0]
     degree = int(round(sh.shape[1] ** 0.5)) - 1
     return _lib.Params_(n, degree, means.data_ptr(), log_scales.data_ptr(), quats.data_ptr(),
-                        opacity_logits.data_ptr(), sh.data_ptr())
+                        opacity_logits.data_ptr(), sh.data_ptr(), 1 if activated else 0)
 
 
 def _check(t, name, shape):
@@ -236,3 +236,148 @@ def save_ply(means, log_scales, quats, opacity_logits, sh, path=None) -> bytes:
         with open(path, "wb") as f:
             f.write(data)
     return data
+
+
+# ---- Batched views: many cameras, one parameter set ------------------------------------------
+
+def _device_index(t):
+    return t.device.index if t.device.index is not None else torch.cuda.current_device()
+
+
+class RasterizeViewsFunction(torch.autograd.Function):
+    """Inputs: the five parameter tensors (raw, or with activated=True standard deviations and
+    opacities in place of log-scales and logits), an optional `means2d` leaf ([V, N, 2], receives
+    dL/d(u, v) per view), the cameras, options, activated, optional stats list. Outputs: all views'
+    images concatenated (flat float32, view v's [H_v, W_v, 3] at its offset), the radii [V, N]
+    and the alpha images concatenated (flat, 1 - final transmittance; neither differentiable)."""
+
+    @staticmethod
+    def forward(ctx, means, log_scales, quats, opacity_logits, sh, means2d, cameras, options: RenderOptions,
+                activated: bool, stats):
+        n = means.shape[0]
+        k = sh.shape[1]
+        V = len(cameras)
+        for t, name, shape in ((means, "means", (n, 3)), (log_scales, "log_scales", (n, 3)), (quats, "quats", (n, 4)),
+                               (opacity_logits, "opacity_logits", (n,)), (sh, "sh", (n, k, 3))):
+            _check(t, name, shape)
+        if means2d is not None:
+            _check(means2d, "means2d", (V, n, 2))
+        means, log_scales, quats, opacity_logits, sh = (t.contiguous() for t in
+                                                        (means, log_scales, quats, opacity_logits, sh))
+        handle = _handle(_device_index(means))
+        L = _lib.load()
+        pixels = [c.width * c.height for c in cameras]
+        offs = np.concatenate([[0], np.cumsum(pixels)]).astype(np.int64)
+        flat = torch.empty(int(offs[-1]) * 3, dtype=torch.float32, device=means.device)
+        alpha = torch.empty(int(offs[-1]), dtype=torch.float32, device=means.device)
+        cams = (_lib.Camera_ * V)(*[camera_c(c) for c in cameras])
+        imgs = (ctypes.c_void_p * V)(*[flat.data_ptr() + 12 * int(o) for o in offs[:-1]])
+        alps = (ctypes.c_void_p * V)(*[alpha.data_ptr() + 4 * int(o) for o in offs[:-1]])
+        st = (_lib.Stats_ * V)()
+        fid = ctypes.c_int64()
+        params = _params(means, log_scales, quats, opacity_logits, sh, activated)
+        _lib.check(L.ans_diff_views_forward(handle, ctypes.byref(params), V, cams, ctypes.byref(options.c()), imgs,
+                                            _stream(), ctypes.byref(fid), st))
+        frame = _Frame(handle, fid.value)
+        _lib.check(L.ans_diff_views_alpha(handle, fid.value, alps, _stream()))
+        radii = torch.empty((V, n), dtype=torch.int32, device=means.device)
+        if means2d is not None:
+            _lib.check(L.ans_diff_views_radii(handle, fid.value, radii.data_ptr(), _stream()))
+        if stats is not None:
+            stats.clear()
+            for s_ in st:
+                stats.append(RenderStats(s_.visible, s_.tile_pairs, s_.project_ms, s_.sort_ms, s_.blend_ms,
+                                         s_.pair_capacity, s_.retries))
+        if any(ctx.needs_input_grad[:6]):
+            ctx.frame = frame
+            ctx.offs = offs
+            ctx.activated = activated
+            ctx.save_for_backward(means, log_scales, quats, opacity_logits, sh)
+        else:
+            frame.release()
+        ctx.mark_non_differentiable(radii, alpha)
+        return flat, radii, alpha
+
+    @staticmethod
+    def backward(ctx, grad_flat, grad_radii, grad_alpha):
+        means, log_scales, quats, opacity_logits, sh = ctx.saved_tensors
+        grads = [torch.empty_like(t) for t in (means, log_scales, quats, opacity_logits, sh)]
+        g = grad_flat.contiguous().float()
+        V = len(ctx.offs) - 1
+        gptrs = (ctypes.c_void_p * V)(*[g.data_ptr() + 12 * int(o) for o in ctx.offs[:-1]])
+        grad_uv = None
+        if ctx.needs_input_grad[5]:
+            grad_uv = torch.empty((V, means.shape[0], 2), dtype=torch.float32, device=means.device)
+        frame = ctx.frame
+        L = _lib.load()
+        _lib.check(L.ans_diff_views_backward(
+            frame.handle, frame.fid, ctypes.byref(_params(means, log_scales, quats, opacity_logits, sh, ctx.activated)),
+            gptrs, ctypes.byref(_lib.Grads_(*[t.data_ptr() for t in grads])),
+            grad_uv.data_ptr() if grad_uv is not None else None, _stream()))
+        frame.release()
+        return (*grads, grad_uv, None, None, None, None)
+
+
+def _split_views(flat, cameras, channels):
+    out, o = [], 0
+    for c in cameras:
+        k = c.width * c.height * channels
+        out.append(flat[o:o + k].view(c.height, c.width, channels) if channels > 1 else
+                   flat[o:o + k].view(c.height, c.width))
+        o += k
+    return out
+
+
+def rasterize_views(means, log_scales, quats, opacity_logits, sh, cameras, options: RenderOptions | None = None,
+                    stats: list | None = None, info: bool = False, activated: bool = False):
+    """Differentiable render of many cameras (uneven sizes allowed) in one batched pass: a list of
+    unclamped colour images [H_v, W_v, 3]. Gradients w.r.t. the parameters are summed over the
+    views. With info=True returns (images, info) with "radii" (int32 [V, N]), "means2d" (a zero
+    [V, N, 2] leaf whose .grad receives dL/d(u, v) per view), "alphas" (list of [H_v, W_v]),
+    "n_cameras"."""
+    opts = options or RenderOptions()
+    means2d = None
+    if info:
+        means2d = torch.zeros((len(cameras), means.shape[0], 2), dtype=torch.float32, device=means.device,
+                              requires_grad=True)
+    flat, radii, alpha = RasterizeViewsFunction.apply(means, log_scales, quats, opacity_logits, sh, means2d,
+                                                      list(cameras), opts, activated, stats)
+    images = _split_views(flat, cameras, 3)
+    if not info:
+        return images
+    return images, {"radii": radii, "means2d": means2d, "alphas": _split_views(alpha, cameras, 1),
+                    "n_cameras": len(cameras)}
+
+
+def views_frame_lists(means, log_scales, quats, opacity_logits, sh, cameras, options: RenderOptions | None = None,
+                      activated: bool = False):
+    """Per-view (start [T+1], concatenated lists [P]) of a batched differentiable forward, for
+    evaluating reference gradients on exactly the GPU's lists."""
+    handle = _handle(_device_index(means))
+    L = _lib.load()
+    V = len(cameras)
+    bufs = [torch.empty((c.height, c.width, 3), dtype=torch.float32, device=means.device) for c in cameras]
+    cams = (_lib.Camera_ * V)(*[camera_c(c) for c in cameras])
+    imgs = (ctypes.c_void_p * V)(*[b.data_ptr() for b in bufs])
+    fid = ctypes.c_int64()
+    _lib.check(L.ans_diff_views_forward(handle, ctypes.byref(_params(means, log_scales, quats, opacity_logits, sh,
+                                                                      activated)),
+                                        V, cams, ctypes.byref((options or RenderOptions()).c()), imgs, _stream(),
+                                        ctypes.byref(fid), None))
+    out = []
+    try:
+        for v in range(V):
+            vis, pairs, tiles = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+            _lib.check(L.ans_diff_views_counts(handle, fid.value, v, ctypes.byref(vis), ctypes.byref(pairs),
+                                               ctypes.byref(tiles)))
+            values = np.empty(pairs.value, np.uint32)
+            ranges = np.empty((tiles.value, 2), np.uint32)
+            _lib.check(L.ans_diff_views_dump(handle, fid.value, v, values.ctypes.data_as(ctypes.c_void_p),
+                                             ranges.ctypes.data_as(ctypes.c_void_p)))
+            lengths = (ranges[:, 1] - ranges[:, 0]).astype(np.int64)
+            start = np.zeros(len(ranges) + 1, np.uint32)
+            start[1:] = np.cumsum(lengths)
+            out.append((start, values))
+    finally:
+        L.ans_diff_release(handle, fid.value)
+    return out, bufs

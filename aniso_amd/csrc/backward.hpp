@@ -94,10 +94,10 @@ constexpr int kPartialStride = 12;  // floats per pair record: u v A B C opacity
 // wavefront, and the wavefronts' sums are added in fixed order into one partial record per pair:
 // partial[slot * 12 + {u, v, A, B, C, opacity, r, g, b}].
 template <int TW, int TH, int BATCH>
-__global__ __launch_bounds__(TW* TH) void k_blendBackward(
-    const uint2* __restrict__ ranges, const uint32_t* __restrict__ sortedGauss, const SplatRec* __restrict__ rec,
-    const uint32_t* __restrict__ slotStart, const uint2* __restrict__ pixState, int width, int height, int tilesX,
-    const float* __restrict__ gradImage, float* __restrict__ partial, const FrameCtl* __restrict__ ctl) {
+__device__ inline void blendBackwardTile(int tileX, int tileY, uint2 range, const uint32_t* __restrict__ sortedGauss,
+                                         const SplatRec* __restrict__ rec, const uint32_t* __restrict__ slotStart,
+                                         const uint2* __restrict__ pixState, int width, int height,
+                                         const float* __restrict__ gradImage, float* __restrict__ partial) {
     constexpr int kThreads = TW * TH;
     constexpr int kWaves = kThreads / kWave;
     constexpr int kWavesX = TW / 8;
@@ -108,9 +108,6 @@ __global__ __launch_bounds__(TW* TH) void k_blendBackward(
     __shared__ float sGrad[kWaves][BATCH + 1][10];  // row BATCH: the partner of an unpaired splat
     __shared__ uint32_t sMaxLast;
 
-    if (ctl->overflow) return;
-    const int tile = blockIdx.x;
-    const int tileX = tile % tilesX, tileY = tile / tilesX;
     const int tid = threadIdx.x;
     const int wave = tid / kWave, lane = __lane_id();
     const int bx = tileX * TW + (wave % kWavesX) * 8, by = tileY * TH + (wave / kWavesX) * 8;
@@ -119,7 +116,6 @@ __global__ __launch_bounds__(TW* TH) void k_blendBackward(
     const float px = x + 0.5f, py = y + 0.5f;
     const uint64_t below = lanesBelow();
 
-    const uint2 range = ranges[tile];
     const uint32_t len = range.y - range.x;
 
     // Stages batch b: records, each pair's partial slot and the per-wavefront culling mask; then
@@ -289,6 +285,17 @@ __global__ __launch_bounds__(TW* TH) void k_blendBackward(
     }
 }
 
+template <int TW, int TH, int BATCH>
+__global__ __launch_bounds__(TW* TH) void k_blendBackward(
+    const uint2* __restrict__ ranges, const uint32_t* __restrict__ sortedGauss, const SplatRec* __restrict__ rec,
+    const uint32_t* __restrict__ slotStart, const uint2* __restrict__ pixState, int width, int height, int tilesX,
+    const float* __restrict__ gradImage, float* __restrict__ partial, const FrameCtl* __restrict__ ctl) {
+    if (ctl->overflow) return;
+    const int tile = blockIdx.x;
+    blendBackwardTile<TW, TH, BATCH>(tile % tilesX, tile / tilesX, ranges[tile], sortedGauss, rec, slotStart, pixState,
+                                     width, height, gradImage, partial);
+}
+
 // Row k of the SH basis Jacobian: d basis_k / d (x, y, z) for the real SH of sh.cpp:18-58
 // (rows past the degree the caller evaluates are never asked for). With k a constant in an
 // unrolled loop the switch folds away, so the 16 x 3 Jacobian is never held at once.
@@ -324,7 +331,11 @@ __host__ __device__ inline void shBasisGradRow(int k, float x, float y, float z,
 // Adjoint of projectGaussian (raw parameters): from gs = dL/d{u, v, A, B, C, opacity, r, g, b}
 // to the gradients of the mean (gm), log-scales (gls), raw quaternion (gq), opacity logit and the
 // SH coefficients (gsh, [coefficient][channel], zero past the evaluated degree). shi holds the
-// Gaussian's SH, [coefficient][channel]; gsh may be the same array.
+// Gaussian's SH, [coefficient][channel]; gsh may be the same array unless ACC. With ACC the SH
+// gradients are added to gsh (which then holds a running sum over views) instead of stored.
+// With g.act the scales and the opacity are the activated ones: gls is then the gradient w.r.t.
+// the standard deviations and *glogit the one w.r.t. the opacity.
+template <bool ACC = false>
 __host__ __device__ inline void gaussAdjoint(const GaussProj& g, const CamParams& cam, const float gs[9],
                                              const float* shi, float gm[3], float gls[3], float gq[4], float* glogit,
                                              float* gsh) {
@@ -371,7 +382,8 @@ __host__ __device__ inline void gaussAdjoint(const GaussProj& g, const CamParams
             gRq[r][c] = gM * g.s[c];
             gsc[c] += gM * g.Rq[r][c];
         }
-    for (int j = 0; j < 3; ++j) gls[j] = gsc[j] * g.s[j];
+    // s = exp(log-scale) * splatScale, or (act) scale * splatScale.
+    for (int j = 0; j < 3; ++j) gls[j] = gsc[j] * (g.act ? cam.splatScale : g.s[j]);
     const float w = g.q[0], x = g.q[1], y = g.q[2], z = g.q[3];
     float gn[4];
     gn[0] = 2.0f * (-z * gRq[0][1] + y * gRq[0][2] + z * gRq[1][0] - x * gRq[1][2] - y * gRq[2][0] + x * gRq[2][1]);
@@ -388,8 +400,8 @@ __host__ __device__ inline void gaussAdjoint(const GaussProj& g, const CamParams
     } else {
         for (int j = 0; j < 4; ++j) gq[j] = 0.0f;
     }
-    // opacity = sigmoid(logit).
-    *glogit = go * g.opacity * (1.0f - g.opacity);
+    // opacity = sigmoid(logit), or (act) the opacity itself.
+    *glogit = g.act ? go : go * g.opacity * (1.0f - g.opacity);
     // SH colour, clamped at zero: d sh_k = basis_k d rgb; d dir through the basis.
     const float grgb[3] = {g.rgbRaw.x > 0.0f ? gs[6] : 0.0f, g.rgbRaw.y > 0.0f ? gs[7] : 0.0f,
                            g.rgbRaw.z > 0.0f ? gs[8] : 0.0f};
@@ -400,32 +412,41 @@ __host__ __device__ inline void gaussAdjoint(const GaussProj& g, const CamParams
             float gbk = 0.0f;
             for (int c = 0; c < 3; ++c) {
                 gbk += grgb[c] * shi[k * 3 + c];  // read before the write below: gsh may alias shi
-                gsh[k * 3 + c] = grgb[c] * g.basis[k];
+                if constexpr (ACC) gsh[k * 3 + c] += grgb[c] * g.basis[k];
+                else gsh[k * 3 + c] = grgb[c] * g.basis[k];
             }
             float row[3];
             shBasisGradRow(k, g.dir[0], g.dir[1], g.dir[2], row);
             for (int j = 0; j < 3; ++j) gdir[j] += gbk * row[j];
         }
     }
-    for (int j = g.shCount * 3; j < cam.shCoeffs * 3; ++j) gsh[j] = 0.0f;
+    if constexpr (!ACC)
+        for (int j = g.shCount * 3; j < cam.shCoeffs * 3; ++j) gsh[j] = 0.0f;
     // dir = (m - eye) / |m - eye|.
     const float dd = gdir[0] * g.dir[0] + gdir[1] * g.dir[1] + gdir[2] * g.dir[2];
     for (int j = 0; j < 3; ++j) gm[j] += (gdir[j] - dd * g.dir[j]) / g.dlen;
 }
 
 // Pair slots in Gaussian index order: per block of 256 Gaussians, the sum of their tile counts.
+// Batched views: blockIdx.y is the view; tile counts at view * gaussStride, block sums at
+// view * blockStride (a single frame: gridDim.y = 1).
 [[maybe_unused]] static __global__ __launch_bounds__(256) void k_slotBlockSums(uint32_t n, const uint32_t* __restrict__ tilesTouched,
-                                                              uint32_t* __restrict__ blockSum) {
+                                                              uint32_t* __restrict__ blockSum, uint32_t gaussStride,
+                                                              uint32_t blockStride) {
     __shared__ uint32_t sTmp[256 / kWave];
+    tilesTouched += static_cast<size_t>(blockIdx.y) * gaussStride;
+    blockSum += static_cast<size_t>(blockIdx.y) * blockStride;
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     uint32_t total;
     blockExclusiveScan<256>(i < n ? tilesTouched[i] : 0u, sTmp, &total);
     if (threadIdx.x == 0) blockSum[blockIdx.x] = total;
 }
 
-// One workgroup: exclusive scan of the per-block sums in place.
-[[maybe_unused]] static __global__ __launch_bounds__(256) void k_scanBlockSums(uint32_t* __restrict__ blockSum, uint32_t blocks) {
+// One workgroup per view: exclusive scan of the per-block sums in place.
+[[maybe_unused]] static __global__ __launch_bounds__(256) void k_scanBlockSums(uint32_t* __restrict__ blockSum, uint32_t blocks,
+                                                              uint32_t blockStride) {
     __shared__ uint32_t sTmp[256 / kWave];
+    blockSum += static_cast<size_t>(blockIdx.y) * blockStride;
     constexpr int kPer = 8;
     uint32_t carry = 0;
     for (uint32_t base = 0; base < blocks; base += 256 * kPer) {
@@ -450,8 +471,12 @@ __host__ __device__ inline void gaussAdjoint(const GaussProj& g, const CamParams
 // slotStart[i]: the first pair slot of Gaussian i (exclusive scan of tile counts in index order).
 [[maybe_unused]] static __global__ __launch_bounds__(256) void k_slotStart(uint32_t n, const uint32_t* __restrict__ tilesTouched,
                                                           const uint32_t* __restrict__ blockBase,
-                                                          uint32_t* __restrict__ slotStart) {
+                                                          uint32_t* __restrict__ slotStart, uint32_t gaussStride,
+                                                          uint32_t blockStride) {
     __shared__ uint32_t sTmp[256 / kWave];
+    tilesTouched += static_cast<size_t>(blockIdx.y) * gaussStride;
+    slotStart += static_cast<size_t>(blockIdx.y) * gaussStride;
+    blockBase += static_cast<size_t>(blockIdx.y) * blockStride;
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     uint32_t total;
     const uint32_t excl = blockExclusiveScan<256>(i < n ? tilesTouched[i] : 0u, sTmp, &total);
@@ -481,7 +506,7 @@ __device__ inline int rowOf(int e, float rcpK3) {
     const uint32_t* __restrict__ tilesTouched, const uint32_t* __restrict__ slotStart,
     const float* __restrict__ partial, float* __restrict__ splatGrad, float* __restrict__ gMeans,
     float* __restrict__ gScales, float* __restrict__ gQuats, float* __restrict__ gLogit, float* __restrict__ gSh,
-    const FrameCtl* __restrict__ ctl) {
+    const FrameCtl* __restrict__ ctl, bool act) {
     __shared__ float4 sRec[kWave][3];
     __shared__ float sSh[kWave * kShRow];
     if (ctl->overflow) return;
@@ -549,7 +574,7 @@ __device__ inline int rowOf(int e, float rcpK3) {
     if (valid) {
         for (int c = 0; c < 9; ++c) splatGrad[static_cast<size_t>(i) * 9 + c] = gs[c];
         GaussProj g;
-        if (tt > 0 && projectGaussian<false, true>(i, pos, logScale, quat, logit, row, 1, cam, g)) {
+        if (tt > 0 && projectGaussian<false, true>(i, pos, logScale, quat, logit, row, 1, cam, g, act)) {
             float gm[3], gls[3], gq[4], gl;
             gaussAdjoint(g, cam, gs, row, gm, gls, gq, &gl, row);
             for (int j = 0; j < 3; ++j) gMeans[3 * i + j] = gm[j], gScales[3 * i + j] = gls[j];

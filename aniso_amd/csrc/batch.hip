@@ -8,8 +8,7 @@
 // view's segment of a shared arena, so each view's splat records, lists, ranges and image equal
 // a single-frame render of that camera with the same options.
 #include "aniso_amd.h"
-#include "engine.hpp"
-#include "views.hpp"
+#include "batch.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -23,121 +22,46 @@
 
 namespace ans {
 
-namespace {
-
-constexpr int kMaxViews = 256;
-
-size_t alignUp(size_t v, size_t a) { return (v + a - 1) / a * a; }
-
-// Byte offsets of the per-view table: CamParams[V], uint8_t*[V] outputs, int tilesX[V],
-// int tiles[V].
-struct TableLayout {
-    size_t cams, outs, tilesX, tiles, bytes;
-    explicit TableLayout(int v) {
-        cams = 0;
-        outs = alignUp(sizeof(CamParams) * v, 16);
-        tilesX = alignUp(outs + sizeof(uint8_t*) * v, 16);
-        tiles = alignUp(tilesX + sizeof(int) * v, 16);
-        bytes = alignUp(tiles + sizeof(int) * v, 16);
+ViewBatch::ViewBatch() {
+    hipCheck(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+    for (auto& slot : ev)
+        for (auto& e : slot) hipCheck(hipEventCreate(&e), "hipEventCreate");
+    for (int k = 0; k < 2; ++k) {
+        hipCheck(hipHostMalloc(reinterpret_cast<void**>(&tableHost[k]), TableLayout(kMaxViews).bytes,
+                               hipHostMallocDefault),
+                 "hipHostMalloc");
+        hipCheck(hipHostMalloc(reinterpret_cast<void**>(&ctlHost[k]), sizeof(FrameCtl) * (kMaxViews + 1),
+                               hipHostMallocDefault),
+                 "hipHostMalloc");
     }
-};
+}
 
-}  // namespace
-
-struct ViewBatch {
-    // Grow-only arena. Cached graphs bake its addresses, so growth of any buffer drops them.
-    DevBuf<uint8_t> table;
-    DevBuf<FrameCtl> ctl;  // one per view, plus one for the item-order sort
-    DevBuf<SplatRec> rec;
-    DevBuf<uint32_t> depthKey, order, kA, vA, kB, vB, ttByRank;
-    DevBuf<uint2> rectByRank;
-    DevBuf<uint32_t> hist, rowTotal, blockPairs, blockVisible;
-    DevBuf<uint32_t> tileA, gaussA, tileB, gaussB;
-    DevBuf<uint2> ranges;
-    DevBuf<uint32_t> itemKey, itemK2, itemV2, itemOrder;
-
-    hipStream_t stream = nullptr;  // graphs are captured here and launched on the caller's stream
-    // Per submission slot (two batches may be in flight along a camera path): start and end of
-    // the launches, control-block copy done; pinned staging of the per-view table; pinned copy of
-    // the control blocks; the buffer the batch's sorted lists ended in.
-    hipEvent_t ev[2][3] = {};
-    uint8_t* tableHost[2] = {};
-    FrameCtl* ctlHost[2] = {};
-    const uint32_t* sortedSlot[2] = {};
-    size_t pairCap = 0;       // pairs per view
-    uint64_t generation = 0;  // bumped when the arena moves
-
-    using GraphKey = std::tuple<int, int64_t, int, int, int, bool, uint32_t, uint32_t, size_t, uint64_t>;
-    std::map<GraphKey, hipGraphExec_t> graphs;
-
-    // The last finished batch, for stats and dumps.
-    int views = 0;
-    uint32_t maxTiles = 0;
-    std::vector<int> tilesPerView;
-    std::vector<FrameCtl> lastCtl;
-    const uint32_t* sortedGauss = nullptr;
-    float lastMs = 0.0f;
-    int lastRetries = 0;
-
-    // Camera-path streaming: its render and copy streams, per-slot device frames and pinned host
-    // frames, copy-done events. Created on first use.
-    hipStream_t pathRender = nullptr, pathCopy = nullptr;
-    hipEvent_t copied[2] = {};
-    DevBuf<uint8_t> pathDev[2];
-    uint8_t* pathHost[2] = {};
-    size_t pathHostCap[2] = {};
-
-    ViewBatch() {
-        hipCheck(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
-        for (auto& slot : ev)
-            for (auto& e : slot) hipCheck(hipEventCreate(&e), "hipEventCreate");
-        for (int k = 0; k < 2; ++k) {
-            hipCheck(hipHostMalloc(reinterpret_cast<void**>(&tableHost[k]), TableLayout(kMaxViews).bytes,
-                                   hipHostMallocDefault),
-                     "hipHostMalloc");
-            hipCheck(hipHostMalloc(reinterpret_cast<void**>(&ctlHost[k]), sizeof(FrameCtl) * (kMaxViews + 1),
-                                   hipHostMallocDefault),
-                     "hipHostMalloc");
-        }
+ViewBatch::~ViewBatch() {
+    if (pathRender) (void)hipStreamSynchronize(pathRender);
+    if (pathCopy) (void)hipStreamSynchronize(pathCopy);
+    dropGraphs();
+    for (int k = 0; k < 2; ++k) {
+        if (tableHost[k]) (void)hipHostFree(tableHost[k]);
+        if (ctlHost[k]) (void)hipHostFree(ctlHost[k]);
+        if (pathHost[k]) (void)hipHostFree(pathHost[k]);
+        if (copied[k]) (void)hipEventDestroy(copied[k]);
     }
-    ~ViewBatch() {
-        if (pathRender) (void)hipStreamSynchronize(pathRender);
-        if (pathCopy) (void)hipStreamSynchronize(pathCopy);
-        dropGraphs();
-        for (int k = 0; k < 2; ++k) {
-            if (tableHost[k]) (void)hipHostFree(tableHost[k]);
-            if (ctlHost[k]) (void)hipHostFree(ctlHost[k]);
-            if (pathHost[k]) (void)hipHostFree(pathHost[k]);
-            if (copied[k]) (void)hipEventDestroy(copied[k]);
-        }
-        for (auto& slot : ev)
-            for (auto& e : slot)
-                if (e) (void)hipEventDestroy(e);
-        if (pathRender) (void)hipStreamDestroy(pathRender);
-        if (pathCopy) (void)hipStreamDestroy(pathCopy);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-    void dropGraphs() {
-        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
-        graphs.clear();
-    }
-};
+    for (auto& slot : ev)
+        for (auto& e : slot)
+            if (e) (void)hipEventDestroy(e);
+    if (pathRender) (void)hipStreamDestroy(pathRender);
+    if (pathCopy) (void)hipStreamDestroy(pathCopy);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+void ViewBatch::dropGraphs() {
+    for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    graphs.clear();
+}
 
 void ViewBatchDeleter::operator()(ViewBatch* b) const { delete b; }
 
 namespace {
-
-// Shapes of one batched call.
-struct Layout {
-    int views;
-    uint32_t n;
-    ans_options opts;
-    bool cull;
-    uint32_t maxTiles, totalTiles, items;
-    int tileBits, passes, digitBits;
-    uint32_t blocksD, blocksDup, blocksP, blocksI;
-    size_t pairCap;
-};
 
 template <class T>
 bool grow(DevBuf<T>& b, size_t n) {
@@ -145,6 +69,8 @@ bool grow(DevBuf<T>& b, size_t n) {
     b.reserve(n);
     return true;
 }
+
+}  // namespace
 
 void reserveArena(ViewBatch& B, const Layout& L) {
     const size_t V = L.views, n = std::max<uint32_t>(L.n, 1), vn = V * n;
@@ -176,6 +102,10 @@ void reserveArena(ViewBatch& B, const Layout& L) {
     moved |= grow(B.itemK2, L.items);
     moved |= grow(B.itemV2, L.items);
     moved |= grow(B.itemOrder, L.items);
+    if (L.diff) {
+        moved |= grow(B.tilesTouched, vn);
+        moved |= grow(B.pixState, std::max<uint64_t>(L.pixels, 1));
+    }
     if (moved) {
         B.dropGraphs();
         ++B.generation;
@@ -184,19 +114,28 @@ void reserveArena(ViewBatch& B, const Layout& L) {
     }
 }
 
+namespace {
+
 template <int TW, int TH, int BATCH>
 void launchBlendViews(ViewBatch& B, const Layout& L, const TableLayout& T, const uint32_t* sortedGauss,
                       hipStream_t st) {
     const CamParams* cams = reinterpret_cast<const CamParams*>(B.table.p + T.cams);
     uint8_t* const* outs = reinterpret_cast<uint8_t* const*>(B.table.p + T.outs);
-    if (L.cull)
-        k_blendViews<TW, TH, BATCH, true><<<L.totalTiles, TW * TH, 0, st>>>(
-            B.itemOrder.p, L.maxTiles, B.ranges.p, sortedGauss, static_cast<uint32_t>(L.pairCap), B.rec.p, L.n, cams,
-            outs, B.ctl.p);
+    float* const* outf = reinterpret_cast<float* const*>(B.table.p + T.outf);
+    const uint64_t* pixOff = reinterpret_cast<const uint64_t*>(B.table.p + T.pixOff);
+    const uint32_t pairStride = static_cast<uint32_t>(L.pairCap);
+    if (L.diff)  // frames for the backward pass always cull (culling changes no pixel)
+        k_blendViews<TW, TH, BATCH, true, true><<<L.totalTiles, TW * TH, 0, st>>>(
+            B.itemOrder.p, L.maxTiles, B.ranges.p, sortedGauss, pairStride, B.rec.p, L.n, cams, outs, outf,
+            B.pixState.p, pixOff, B.ctl.p);
+    else if (L.cull)
+        k_blendViews<TW, TH, BATCH, true, false><<<L.totalTiles, TW * TH, 0, st>>>(
+            B.itemOrder.p, L.maxTiles, B.ranges.p, sortedGauss, pairStride, B.rec.p, L.n, cams, outs, outf, nullptr,
+            pixOff, B.ctl.p);
     else
-        k_blendViews<TW, TH, BATCH, false><<<L.totalTiles, TW * TH, 0, st>>>(
-            B.itemOrder.p, L.maxTiles, B.ranges.p, sortedGauss, static_cast<uint32_t>(L.pairCap), B.rec.p, L.n, cams,
-            outs, B.ctl.p);
+        k_blendViews<TW, TH, BATCH, false, false><<<L.totalTiles, TW * TH, 0, st>>>(
+            B.itemOrder.p, L.maxTiles, B.ranges.p, sortedGauss, pairStride, B.rec.p, L.n, cams, outs, outf, nullptr,
+            pixOff, B.ctl.p);
 }
 
 template <int TW, int TH>
@@ -207,8 +146,8 @@ void launchBlendViewsT(ViewBatch& B, const Layout& L, const TableLayout& T, cons
     else launchBlendViews<TW, TH, 256>(B, L, T, sortedGauss, st);
 }
 
-// Enqueues the whole batch on st (kernel launches only, so it can be captured). Returns the
-// buffer holding the per-view sorted Gaussian lists.
+}  // namespace
+
 const uint32_t* enqueueBatch(ViewBatch& B, const SceneDev& s, const Layout& L, hipStream_t st) {
     const TableLayout T(L.views);
     const CamParams* cams = reinterpret_cast<const CamParams*>(B.table.p + T.cams);
@@ -220,9 +159,14 @@ const uint32_t* enqueueBatch(ViewBatch& B, const SceneDev& s, const Layout& L, h
 
     k_initViews<<<(std::max(V + 1, L.items) + 255) / 256, 256, 0, st>>>(ctl, V + 1, B.ranges.p, L.items);
     if (n > 0) {
-        k_preprocessViews<<<(n + 255) / 256, 256, 0, st>>>(static_cast<int>(n), L.views, s.pos, s.scale, s.rot,
-                                                           s.opacity, s.sh, (s.shDegree + 1) * (s.shDegree + 1),
-                                                           cams, B.rec.p, nullptr, B.depthKey.p);
+        if (L.diff)
+            k_preprocessViewsRaw<<<(n + 255) / 256, 256, 0, st>>>(
+                static_cast<int>(n), L.views, reinterpret_cast<const DiffParamsDev*>(B.table.p + T.params), cams,
+                B.rec.p, B.tilesTouched.p, B.depthKey.p);
+        else
+            k_preprocessViews<<<(n + 255) / 256, 256, 0, st>>>(static_cast<int>(n), L.views, s.pos, s.scale, s.rot,
+                                                               s.opacity, s.sh, (s.shDegree + 1) * (s.shDegree + 1),
+                                                               cams, B.rec.p, nullptr, B.depthKey.p);
         // Depth sort of each view's Gaussians: four stable 8-bit passes.
         const uint32_t histD = kRadix * L.blocksD;  // depth sort: kPairItems keys per thread
         const uint32_t* kin = B.depthKey.p;
@@ -303,18 +247,20 @@ const uint32_t* enqueueBatch(ViewBatch& B, const SceneDev& s, const Layout& L, h
 }
 
 Layout makeLayout(const ViewBatch& B, const SceneDev& s, int views, const std::vector<CamParams>& cps,
-                  const ans_options& o) {
+                  const ans_options& o, bool diff) {
     Layout L{};
     L.views = views;
     L.n = static_cast<uint32_t>(s.n);
     L.opts = o;
     L.cull = !(o.flags & ANS_FLAG_NO_CULL);
+    L.diff = diff;
     uint64_t total = 0;
     uint32_t maxT = 1;
     for (const CamParams& cp : cps) {
         const uint32_t t = static_cast<uint32_t>(cp.tilesX) * static_cast<uint32_t>(cp.tilesY);
         maxT = std::max(maxT, t);
         total += t;
+        L.pixels += static_cast<uint64_t>(cp.width) * static_cast<uint64_t>(cp.height);
     }
     if (uint64_t(maxT) * views >= (uint64_t(1) << 31)) throw std::invalid_argument("too many tiles in the batch");
     L.maxTiles = maxT;
@@ -331,27 +277,32 @@ Layout makeLayout(const ViewBatch& B, const SceneDev& s, int views, const std::v
     return L;
 }
 
-// Uploads the per-view table (through submission slot `slot`'s pinned staging) and enqueues the
-// batch on st: graph replay unless ANS_FLAG_NO_GRAPH (graphs are captured on the batch's own
-// stream and launched on st), then the control blocks' copy to the slot's pinned buffer. Returns
-// without waiting.
+// Graphs are captured on the batch's own stream and launched on st.
 void submitBatch(ViewBatch& B, const SceneDev& s, const Layout& L, const std::vector<CamParams>& cps,
-                 uint8_t* const* outs, hipStream_t st, int slot) {
+                 const BatchIO& io, hipStream_t st, int slot) {
     const TableLayout T(L.views);
     uint8_t* th = B.tableHost[slot];
+    uint64_t pix = 0;
     for (int v = 0; v < L.views; ++v) {
         reinterpret_cast<CamParams*>(th + T.cams)[v] = cps[v];
-        reinterpret_cast<uint8_t**>(th + T.outs)[v] = outs[v];
+        reinterpret_cast<uint8_t**>(th + T.outs)[v] = io.outs ? io.outs[v] : nullptr;
+        reinterpret_cast<float**>(th + T.outf)[v] = io.outf ? io.outf[v] : nullptr;
+        reinterpret_cast<uint64_t*>(th + T.pixOff)[v] = pix;
+        pix += static_cast<uint64_t>(cps[v].width) * static_cast<uint64_t>(cps[v].height);
         reinterpret_cast<int*>(th + T.tilesX)[v] = cps[v].tilesX;
         reinterpret_cast<int*>(th + T.tiles)[v] = cps[v].tilesX * cps[v].tilesY;
+        reinterpret_cast<const float**>(th + T.grads)[v] = nullptr;
+        reinterpret_cast<uint64_t*>(th + T.pairBase)[v] = 0;
     }
+    *reinterpret_cast<DiffParamsDev*>(th + T.params) = io.params ? *io.params : DiffParamsDev{};
     hipCheck(hipMemcpyAsync(B.table.p, th, T.bytes, hipMemcpyHostToDevice, st), "hipMemcpyAsync table");
     hipCheck(hipEventRecord(B.ev[slot][0], st), "hipEventRecord");
     if (L.opts.flags & ANS_FLAG_NO_GRAPH) {
         B.sortedSlot[slot] = enqueueBatch(B, s, L, st);
     } else {
-        const ViewBatch::GraphKey key{L.views, s.n, L.opts.tile_w, L.opts.tile_h, L.opts.batch_depth, L.cull,
-                                      L.maxTiles, L.totalTiles, L.pairCap, B.generation};
+        const ViewBatch::GraphKey key{L.views,    s.n,          L.opts.tile_w, L.opts.tile_h,
+                                      L.opts.batch_depth, L.cull, L.maxTiles, L.totalTiles,
+                                      L.pairCap,  B.generation, L.diff,      L.pixels};
         auto it = B.graphs.find(key);
         if (it == B.graphs.end()) {
             hipGraph_t g = nullptr;
@@ -379,9 +330,6 @@ void submitBatch(ViewBatch& B, const SceneDev& s, const Layout& L, const std::ve
     hipCheck(hipEventRecord(B.ev[slot][2], st), "hipEventRecord");
 }
 
-// Waits for the batch submitted through `slot`, makes it the last finished batch (stats, dumps)
-// and returns true if any view's pairs overflowed the arena; the per-view capacity has then been
-// raised for the next layout.
 bool finishBatch(ViewBatch& B, const Layout& L, const std::vector<CamParams>& cps, int slot) {
     hipCheck(hipEventSynchronize(B.ev[slot][2]), "hipEventSynchronize");
     hipCheck(hipEventElapsedTime(&B.lastMs, B.ev[slot][0], B.ev[slot][1]), "hipEventElapsedTime");
@@ -403,6 +351,8 @@ bool finishBatch(ViewBatch& B, const Layout& L, const std::vector<CamParams>& cp
     }
     return over;
 }
+
+namespace {
 
 ViewBatch& batchOf(ans_renderer* r) {
     if (!r->batch) r->batch.reset(new ViewBatch());
@@ -428,7 +378,9 @@ void renderViews(ans_renderer* r, int32_t n_views, const ans_camera* cams, const
         const Layout L = makeLayout(B, r->scene, n_views, cps, o);
         reserveArena(B, L);
         B.pairCap = L.pairCap;
-        submitBatch(B, r->scene, L, cps, dev_rgb, static_cast<hipStream_t>(stream), 0);
+        BatchIO io;
+        io.outs = dev_rgb;
+        submitBatch(B, r->scene, L, cps, io, static_cast<hipStream_t>(stream), 0);
         if (!finishBatch(B, L, cps, 0)) break;
         ++retries;
     }
@@ -540,7 +492,9 @@ void renderPath(ans_renderer* r, int64_t nFrames, const ans_camera* cams, const 
         B.pairCap = L.pairCap;
         std::vector<uint8_t*> outs(k.count);
         for (int v = 0; v < k.count; ++v) outs[v] = B.pathDev[slot].p + k.offset[v];
-        submitBatch(B, r->scene, L, k.cps, outs.data(), B.pathRender, slot);
+        BatchIO io;
+        io.outs = outs.data();
+        submitBatch(B, r->scene, L, k.cps, io, B.pathRender, slot);
         ++st.chunks;
         return L;
     };

@@ -2,8 +2,7 @@
 // sequence (blend adjoint, then per-Gaussian reduction and projection adjoint). C ABI in
 // include/aniso_amd.h.
 #include "aniso_amd.h"
-#include "backward.hpp"
-#include "engine.hpp"
+#include "diff.hpp"
 
 #include <map>
 #include <memory>
@@ -49,15 +48,15 @@ void Engine::enqueueBackward(FrameState& f, const SceneDev& s, const float* grad
     const uint32_t grid = (n + 255) / 256;
     blockPairs.reserve(grid);
     // Pair slots in Gaussian index order.
-    k_slotBlockSums<<<grid, 256, 0, stream>>>(n, f.tilesTouched.p, blockPairs.p);
-    k_scanBlockSums<<<1, 256, 0, stream>>>(blockPairs.p, grid);
-    k_slotStart<<<grid, 256, 0, stream>>>(n, f.tilesTouched.p, blockPairs.p, slotStart.p);
+    k_slotBlockSums<<<grid, 256, 0, stream>>>(n, f.tilesTouched.p, blockPairs.p, 0, 0);
+    k_scanBlockSums<<<1, 256, 0, stream>>>(blockPairs.p, grid, 0);
+    k_slotStart<<<grid, 256, 0, stream>>>(n, f.tilesTouched.p, blockPairs.p, slotStart.p, 0, 0);
     if (f.opts.tile_w == 8) launchBlendBackward<8, 8>(*this, f, gradImage);
     else if (f.opts.tile_w == 32) launchBlendBackward<32, 16>(*this, f, gradImage);
     else launchBlendBackward<16, 16>(*this, f, gradImage);
     k_gaussBackward<<<(n + kWave - 1) / kWave, kWave, 0, stream>>>(static_cast<int>(n), s.pos, s.scale, s.rot, s.opacity, s.sh, f.cp,
                                                    f.tilesTouched.p, slotStart.p, partial.p, splatGrad.p, gMeans,
-                                                   gScales, gQuats, gOpacity, gSh, f.ctl.p);
+                                                   gScales, gQuats, gOpacity, gSh, f.ctl.p, s.act);
     hipCheck(hipGetLastError(), "kernel launch");
 }
 
@@ -65,12 +64,6 @@ void Engine::enqueueBackward(FrameState& f, const SceneDev& s, const float* grad
 
 using namespace ans;
 
-struct ans_diff {
-    std::unique_ptr<Engine> engine;
-    std::map<int64_t, std::unique_ptr<FrameState>> frames;
-    std::vector<std::unique_ptr<FrameState>> pool;
-    int64_t next = 1;
-};
 
 namespace {
 
@@ -91,7 +84,11 @@ int guardedDiff(Fn&& fn) {
     }
 }
 
-SceneDev sceneOf(const ans_params* p) {
+}  // namespace
+
+namespace ans {
+
+SceneDev sceneOfParams(const ans_params* p) {
     if (!p) throw std::invalid_argument("params is null");
     if (p->n < 0 || p->n > (int64_t(1) << 30)) throw std::invalid_argument("Gaussian count must be in 0..2^30");
     if (p->sh_degree < 0 || p->sh_degree > 3) throw std::invalid_argument("sh_degree must be in 0..3");
@@ -109,8 +106,16 @@ SceneDev sceneOf(const ans_params* p) {
     s.sh = p->sh;
     s.shPlanar = false;
     s.raw = true;
+    if (p->activated != 0 && p->activated != 1) throw std::invalid_argument("activated must be 0 or 1");
+    s.act = p->activated != 0;
     return s;
 }
+
+}  // namespace ans
+
+namespace {
+
+SceneDev sceneOf(const ans_params* p) { return sceneOfParams(p); }
 
 FrameState& frameOf(ans_diff* d, int64_t id) {
     if (!d) throw std::invalid_argument("handle is null");
@@ -192,9 +197,15 @@ int ans_diff_release(ans_diff* d, int64_t frame) {
     return guardedDiff([&] {
         if (!d) throw std::invalid_argument("handle is null");
         auto it = d->frames.find(frame);
-        if (it == d->frames.end()) throw std::invalid_argument("unknown or released frame");
-        d->pool.push_back(std::move(it->second));
-        d->frames.erase(it);
+        if (it != d->frames.end()) {
+            d->pool.push_back(std::move(it->second));
+            d->frames.erase(it);
+            return ANS_OK;
+        }
+        auto vit = d->viewFrames.find(frame);
+        if (vit == d->viewFrames.end()) throw std::invalid_argument("unknown or released frame");
+        d->viewPool.push_back(std::move(vit->second));
+        d->viewFrames.erase(vit);
         return ANS_OK;
     });
 }

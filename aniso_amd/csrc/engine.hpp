@@ -53,6 +53,7 @@ struct SceneDev {
     const float* sh = nullptr;
     bool shPlanar = false;  // [coefficient * 3 + channel][n] instead of [n][coefficient][channel]
     bool raw = false;
+    bool act = false;  // raw with activated scales and opacities (ans_params.activated)
 };
 
 CamParams makeCamParams(const ans_camera& c, const ans_options& o, int sceneShDegree);

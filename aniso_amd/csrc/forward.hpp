@@ -101,6 +101,7 @@ struct GaussProj {
     float px, py, pz, tx, ty;
     bool clampX, clampY;
     float s[3], q[4], qlen;
+    bool act;  // RAW with activated scales and opacities (standard deviations, opacities in [0, 1])
     float Rq[3][3], M[3][3], Sigma[3][3], T[2][3];
     float a, b, c, det;
     float dir[3], dlen;
@@ -127,8 +128,13 @@ __host__ __device__ inline bool cameraTransform(const F3 m, const CamParams& cam
 }
 
 // The view-independent part: rotation, scale (times splatScale), M = R(q) diag(s), Sigma = M M^T.
+// RAW with act: the quaternion is normalized but the scales and the opacity are taken as given
+// (the convention of rasterizers that receive exp(log-scale) and sigmoid(logit) from the caller,
+// pipeline/train_gsplat.py:98-101).
 template <bool RAW>
-__host__ __device__ inline void prepareGaussian(const float4 q4, const F3 sc, float splatScale, GaussProj& g) {
+__host__ __device__ inline void prepareGaussian(const float4 q4, const F3 sc, float splatScale, GaussProj& g,
+                                                bool act = false) {
+    g.act = RAW && act;
     if constexpr (RAW) {
         g.qlen = sqrtf(q4.x * q4.x + q4.y * q4.y + q4.z * q4.z + q4.w * q4.w);
         if (g.qlen > 0.0f) {
@@ -144,7 +150,7 @@ __host__ __device__ inline void prepareGaussian(const float4 q4, const F3 sc, fl
     g.Rq[0][0] = 1 - 2 * (y * y + z * z); g.Rq[0][1] = 2 * (x * y - w * z);     g.Rq[0][2] = 2 * (x * z + w * y);
     g.Rq[1][0] = 2 * (x * y + w * z);     g.Rq[1][1] = 1 - 2 * (x * x + z * z); g.Rq[1][2] = 2 * (y * z - w * x);
     g.Rq[2][0] = 2 * (x * z - w * y);     g.Rq[2][1] = 2 * (y * z + w * x);     g.Rq[2][2] = 1 - 2 * (x * x + y * y);
-    if constexpr (RAW) {
+    if (RAW && !g.act) {
         g.s[0] = expf(sc.x) * splatScale; g.s[1] = expf(sc.y) * splatScale; g.s[2] = expf(sc.z) * splatScale;
     } else {
         g.s[0] = sc.x * splatScale; g.s[1] = sc.y * splatScale; g.s[2] = sc.z * splatScale;
@@ -170,10 +176,10 @@ template <bool PLANAR, bool RAW>
 __host__ __device__ inline bool projectGaussian(int i, const float* __restrict__ pos, const float* __restrict__ scale,
                                                 const float4* __restrict__ rot, const float* __restrict__ opacity,
                                                 const float* __restrict__ shBase, size_t shStride, const CamParams& cam,
-                                                GaussProj& g) {
+                                                GaussProj& g, bool act = false) {
     const F3 m = reinterpret_cast<const F3*>(pos)[i];
     if (!cameraTransform(m, cam, g)) return false;
-    prepareGaussian<RAW>(rot[i], reinterpret_cast<const F3*>(scale)[i], cam.splatScale, g);
+    prepareGaussian<RAW>(rot[i], reinterpret_cast<const F3*>(scale)[i], cam.splatScale, g, act);
     return finishProjection<PLANAR, RAW>(m, opacity[i], shBase, shStride, cam, g);
 }
 
@@ -239,7 +245,7 @@ __host__ __device__ inline bool finishProjection(const F3 m, float opacityIn, co
     g.rgb.x = (0.0f < g.rgbRaw.x) ? g.rgbRaw.x : 0.0f;
     g.rgb.y = (0.0f < g.rgbRaw.y) ? g.rgbRaw.y : 0.0f;
     g.rgb.z = (0.0f < g.rgbRaw.z) ? g.rgbRaw.z : 0.0f;
-    if constexpr (RAW) g.opacity = 1.0f / (1.0f + expf(-opacityIn));
+    if (RAW && !g.act) g.opacity = 1.0f / (1.0f + expf(-opacityIn));
     else g.opacity = opacityIn;
     g.A = g.c / g.det;
     g.B = -g.b / g.det;
@@ -295,7 +301,7 @@ __global__ __launch_bounds__(256) void k_preprocess(int n, const float* __restri
                                                     uint32_t* __restrict__ tilesTouched,
                                                     uint32_t* __restrict__ depthKey, FrameCtl* __restrict__ ctl,
                                                     uint2* __restrict__ ranges, uint32_t tiles,
-                                                    FrameStamps* __restrict__ stamps) {
+                                                    FrameStamps* __restrict__ stamps, bool act) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     // The frame starts here: its control block (all but sticky) and tile ranges are cleared by
     // the first max(n, tiles) lanes; no later kernel of the frame runs before this one ends.
@@ -313,7 +319,7 @@ __global__ __launch_bounds__(256) void k_preprocess(int n, const float* __restri
     GaussProj g;
     const float* shBase = PLANAR ? sh + i : sh + static_cast<size_t>(i) * cam.shCoeffs * 3;
     if (!projectGaussian<PLANAR, RAW>(i, pos, scale, rot, opacity, shBase, PLANAR ? static_cast<size_t>(n) : 1, cam,
-                                      g)) {
+                                      g, act)) {
         storeCulled(i, rec, tilesTouched, depthKey);
         return;
     }

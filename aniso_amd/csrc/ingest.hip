@@ -378,6 +378,7 @@ int ans_ply_write(const ans_params* params, void* out, size_t capacity, size_t* 
         if (params->n < 0 || params->n > (int64_t(1) << 30))
             throw std::invalid_argument("Gaussian count must be in 0..2^30");
         if (params->sh_degree < 0 || params->sh_degree > 3) throw std::invalid_argument("sh_degree must be in 0..3");
+        if (params->activated) throw std::invalid_argument("the PLY layout stores raw parameters (activated must be 0)");
         const int coeffs = (params->sh_degree + 1) * (params->sh_degree + 1);
         const std::string header = plyHeader(params->n, coeffs);
         const int stride = 17 + 3 * (coeffs - 1);

@@ -179,15 +179,15 @@ void Engine::enqueueCount(FrameState& f, const SceneDev& s, const CamParams& cp,
         if (s.raw)
             k_preprocess<false, true><<<grid, 256, 0, stream>>>(static_cast<int>(n), s.pos, s.scale, s.rot, s.opacity,
                                                                 s.sh, cp, f.rec.p, f.tilesTouched.p, depthKey.p, ctl,
-                                                                f.ranges.p, tiles, S);
+                                                                f.ranges.p, tiles, S, s.act);
         else if (s.shPlanar)
             k_preprocess<true, false><<<grid, 256, 0, stream>>>(static_cast<int>(n), s.pos, s.scale, s.rot, s.opacity,
                                                                 s.sh, cp, f.rec.p, f.tilesTouched.p, depthKey.p, ctl,
-                                                                f.ranges.p, tiles, S);
+                                                                f.ranges.p, tiles, S, s.act);
         else
             k_preprocess<false, false><<<grid, 256, 0, stream>>>(static_cast<int>(n), s.pos, s.scale, s.rot,
                                                                  s.opacity, s.sh, cp, f.rec.p, f.tilesTouched.p,
-                                                                 depthKey.p, ctl, f.ranges.p, tiles, S);
+                                                                 depthKey.p, ctl, f.ranges.p, tiles, S, s.act);
     }
     if (n > 0) {
         // Depth sort: four stable 8-bit passes over all Gaussians (culled ones carry the largest

@@ -206,7 +206,9 @@ int ans_render_path(ans_renderer* r, int64_t n_frames, const ans_camera* cams, c
 
 /* Raw parameters as an optimizer holds them: means [n][3], log_scales [n][3], quats [n][4]
  * (w, x, y, z; normalized inside), opacity_logits [n], sh [n][(sh_degree+1)^2][3]; quats and
- * sh 16-byte aligned. */
+ * sh 16-byte aligned. activated = 1: log_scales holds standard deviations and opacity_logits
+ * opacities in [0, 1], the arguments the reference's training script passes to its rasterizer
+ * after exp and sigmoid (pipeline/train_gsplat.py:98-101); the gradients are then w.r.t. those. */
 typedef struct ans_params {
     int64_t n;
     int32_t sh_degree;
@@ -215,6 +217,7 @@ typedef struct ans_params {
     const float* quats;
     const float* opacity_logits;
     const float* sh;
+    int32_t activated;
 } ans_params;
 
 /* Gradient outputs, same shapes as ans_params; overwritten (not accumulated). quats must be
@@ -246,6 +249,7 @@ int ans_diff_forward(ans_diff* d, const ans_params* params, const ans_camera* ca
 int ans_diff_backward(ans_diff* d, int64_t frame, const ans_params* params, const float* grad_image,
                       const ans_grads* grads, void* stream);
 
+/* Releases a frame of ans_diff_forward or ans_diff_views_forward. */
 int ans_diff_release(ans_diff* d, int64_t frame);
 
 /* After ans_diff_backward on a frame: per-Gaussian gradients w.r.t. its splat record, copied to
@@ -265,6 +269,46 @@ int ans_diff_screen_grads(ans_diff* d, int64_t frame, float* grad_uv, void* stre
  * CPU reference evaluated on the same lists: values [pairs], ranges [tiles][2]. */
 int ans_diff_frame_counts(ans_diff* d, int64_t frame, int64_t* visible, int64_t* pairs, int64_t* tiles);
 int ans_diff_frame_dump(ans_diff* d, int64_t frame, uint32_t* values, uint32_t* ranges);
+
+/* ---- Batched differentiable frames (no counterpart in the reference, whose training loop
+ * renders and differentiates one view per step through a third-party rasterizer,
+ * pipeline/train_gsplat.py:170-190 with the render at :96-102). n_views (1..256) cameras of
+ * uneven sizes render one parameter set in one sequence of batched launches replayed as a
+ * captured HIP graph (the batched render's sequence, ans_render_views, on raw parameters); the
+ * backward is one more graph: pair slots per view, one blend-adjoint launch over every view's
+ * tiles, and one per-Gaussian launch that adds the views' gradients in view order (no atomics).
+ * Each view's image equals ans_diff_forward of that camera; the gradients equal the sum over
+ * views, in view order, of the single-view ans_diff_backward gradients. */
+
+/* Forward: images[v] is a DEVICE float [height][width][3] for cams[v]. Runs on `stream` after
+ * the caller's earlier work there and waits for the batch (its pair arena grows and the batch is
+ * rendered again if a view's pairs exceeded it). stats: NULL or n_views entries (visible,
+ * tile_pairs, pair_capacity per view; blend_ms = the batch's GPU time). The frame (*frame) stays
+ * valid until ans_diff_release. */
+int ans_diff_views_forward(ans_diff* d, const ans_params* params, int32_t n_views, const ans_camera* cams,
+                           const ans_options* options, float* const* images, void* stream, int64_t* frame,
+                           ans_stats* stats);
+
+/* Backward of a batched frame: grad_images[v] = dL/dimage of view v (DEVICE, as the images);
+ * gradients of every raw parameter (overwritten) summed over the views; screen_grads: NULL or a
+ * DEVICE float [n_views][n][2] (8-byte aligned) receiving dL/d(u, v) per view, what a
+ * densification strategy reads (pipeline/train_gsplat.py:177, :185). Enqueued on `stream`, not
+ * waited for. Backward passes of one handle must not overlap (they share scratch memory). */
+int ans_diff_views_backward(ans_diff* d, int64_t frame, const ans_params* params, const float* const* grad_images,
+                            const ans_grads* grads, float* screen_grads, void* stream);
+
+/* Per view, 1 - final transmittance of every pixel (the alpha image, DEVICE float [height][width]);
+ * it carries no gradient. */
+int ans_diff_views_alpha(ans_diff* d, int64_t frame, float* const* alphas, void* stream);
+
+/* Per (view, Gaussian) splat radius in pixels, 0 if culled in that view: DEVICE int32 [n_views][n]. */
+int ans_diff_views_radii(ans_diff* d, int64_t frame, int32_t* radii, void* stream);
+
+/* Per-view lists of a batched frame, for parity checks: counts, then values [pairs] and ranges
+ * [tiles][2]. */
+int ans_diff_views_counts(ans_diff* d, int64_t frame, int32_t view, int64_t* visible, int64_t* pairs,
+                          int64_t* tiles);
+int ans_diff_views_dump(ans_diff* d, int64_t frame, int32_t view, uint32_t* values, uint32_t* ranges);
 
 /* The raw parameters (device arrays, as for ans_diff_forward) as a 3DGS PLY file image in host
  * memory, the training pipeline's writer (pipeline/train_gsplat.py:109-128): header, then per

@@ -1,0 +1,209 @@
+"""Batched differentiable frames on the GPU (ans_diff_views_*, through the autograd Function),
+against single-view differentiable frames and the FP64 CPU reference gradient.
+
+Contract: each view's image and per-tile lists equal the single-view differentiable forward's
+bit for bit, and the gradients equal the sum over views, in view order, of the single-view
+backward gradients (the same per-view arithmetic, added in the same order: bit-identical; the
+test allows 1e-6 relative). Against the FP64 reference gradient (the sum over views of the
+oracle's gradient on the GPU's own lists): 1e-3 relative per parameter group. The activated
+parameter convention (standard deviations and opacities instead of log-scales and logits) is
+checked against the raw convention through the chain rule.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from aniso_amd import scenes
+from aniso_amd.autograd import frame_lists, rasterize, rasterize_views, views_frame_lists
+from aniso_amd.renderer import RenderOptions, RenderStats
+
+from test_gpu_backward import TOL, compare, tensors
+
+pytestmark = pytest.mark.gpu
+
+
+def small_views(seed=1):
+    rng = np.random.default_rng(seed)
+    sizes = [(320, 180), (333, 217), (160, 90), (256, 256)]
+    cams = []
+    for w, h in sizes:
+        pos = rng.uniform(-0.3, 0.3, 3)
+        yaw = np.radians(rng.uniform(-15, 15))
+        cams.append(scenes.pose_camera(pos, (np.sin(yaw), 0.0, np.cos(yaw)), (0.0, -1.0, 0.0), w, h, 60.0))
+    return cams
+
+
+def small_scene(seed=3):
+    raw = scenes.synthetic_scene(30000, 320, 180, seed=seed)
+    raw.log_scales += np.float32(1.5)
+    return raw
+
+
+def grad_rows(params):
+    n = params[0].shape[0]
+    return np.concatenate([params[0].grad.cpu().numpy(), params[1].grad.cpu().numpy(), params[2].grad.cpu().numpy(),
+                           params[3].grad.cpu().numpy()[:, None], params[4].grad.cpu().numpy().reshape(n, -1)], 1)
+
+
+def weights(cams, seed=7):
+    rng = np.random.default_rng(seed)
+    return [rng.normal(size=(c.height, c.width, 3)).astype(np.float32) for c in cams]
+
+
+def batched(params, cams, W, opts=None, activated=False, stats=None):
+    for p in params:
+        p.grad = None
+    imgs = rasterize_views(*params, cams, opts, stats=stats, activated=activated)
+    loss = sum((img * torch.tensor(w, device="cuda")).sum() for img, w in zip(imgs, W))
+    loss.backward()
+    return [i.detach().cpu().numpy() for i in imgs], grad_rows(params)
+
+
+def single_sum(params, cams, W, opts=None):
+    """Per-view images and the float32 sum, in view order, of the single-view gradients."""
+    imgs, acc = [], None
+    for cam, w in zip(cams, W):
+        for p in params:
+            p.grad = None
+        img = rasterize(*params, cam, opts)
+        (img * torch.tensor(w, device="cuda")).sum().backward()
+        g = grad_rows(params)
+        acc = g.copy() if acc is None else acc + g
+        imgs.append(img.detach().cpu().numpy())
+    return imgs, acc
+
+
+def rel(a, b):
+    return float(np.abs(a.astype(np.float64) - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+@pytest.mark.parametrize("tile", [(16, 16), (8, 8), (32, 16)], ids=["16x16", "8x8", "32x16"])
+def test_views_equal_single_views_and_gradients_sum(gpu, tile):
+    raw = small_scene()
+    cams = small_views()
+    params = tensors(raw)
+    opts = RenderOptions(tile=tile)
+    print(f"case small-30k 4 views tile={tile}", flush=True)
+    W = weights(cams)
+    imgs_b, g_b = batched(params, cams, W, opts)
+    imgs_s, g_s = single_sum(params, cams, W, opts)
+    for v, (a, b) in enumerate(zip(imgs_b, imgs_s)):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), f"view {v} image differs"
+    lists_b, _ = views_frame_lists(*[p.detach() for p in params], cams, opts)
+    for v, cam in enumerate(cams):
+        start, values, _ = frame_lists(*[p.detach() for p in params], cam, opts)
+        assert np.array_equal(lists_b[v][0], start) and np.array_equal(lists_b[v][1], values), f"view {v} lists"
+    r = rel(g_b, g_s)
+    print(f"  batched vs sum of single-view gradients: max rel {r:.2e}, identical {np.array_equal(g_b, g_s)}",
+          flush=True)
+    assert r <= 1e-6
+
+
+def test_views_gradients_match_fp64_reference(gpu):
+    raw = small_scene(seed=5)
+    cams = small_views(seed=2)
+    params = tensors(raw)
+    W = weights(cams, seed=3)
+    lists, _ = views_frame_lists(*[p.detach() for p in params], cams)
+    imgs, g = batched(params, cams, W)
+    ref = np.zeros_like(g, dtype=np.float64)
+    for cam, w, lst, img in zip(cams, W, lists, imgs):
+        ref_img, r, _ = oracle.grad_run(raw, cam, w.astype(np.float64), lists=lst)
+        ref += r
+        d = np.abs(img - ref_img)
+        assert d.max() < 1.0 / 255.0 and d.mean() < 1e-5
+    assert compare("batched vs FP64 reference", g.astype(np.float64), ref) < TOL
+
+
+def test_views_graph_replays_and_direct_launches(gpu):
+    """The forward and backward graphs replay for new poses and new image gradients, with the
+    program's own eager work between replays; every replay equals direct launches."""
+    raw = small_scene(seed=8)
+    params = tensors(raw)
+    for k in range(3):
+        cams = small_views(seed=10 + k)
+        W = weights(cams, seed=20 + k)
+        torch.cuda.synchronize()
+        junk = torch.randn(1 << 20, device="cuda").sum()  # eager work between replays
+        imgs_g, g_g = batched(params, cams, W, RenderOptions())
+        imgs_d, g_d = batched(params, cams, W, RenderOptions(graph=False))
+        for a, b in zip(imgs_g, imgs_d):
+            assert np.array_equal(a, b)
+        assert np.array_equal(g_g, g_d), f"replay {k}"
+        assert torch.isfinite(junk)
+
+
+def test_views_alpha_and_info(gpu):
+    """alpha = 1 - final transmittance equals the image of the same scene with every colour 1
+    (the composite of alpha_i T_i) to rounding, and the FP64 reference's; radii are the single-view
+    ones; means2d.grad per view equals the single-view screen gradient."""
+    raw = small_scene(seed=9)
+    white = scenes.RawScene(raw.means, raw.log_scales, raw.quats, raw.opacity_logits, np.zeros_like(raw.sh))
+    white.sh[:, 0, :] = 0.5 / scenes.SH_C0
+    cams = small_views(seed=4)
+    params = tensors(raw)
+    imgs, info = rasterize_views(*params, cams, info=True)
+    W = weights(cams, seed=5)
+    sum((img * torch.tensor(w, device="cuda")).sum() for img, w in zip(imgs, W)).backward()
+    with torch.no_grad():
+        white_imgs = rasterize_views(*tensors(white), cams)
+    for v, cam in enumerate(cams):
+        a = info["alphas"][v].cpu().numpy()
+        wimg = white_imgs[v].cpu().numpy()
+        assert np.abs(a[..., None] - wimg).max() < 1e-5
+        ref_img, _, _ = oracle.grad_run(white, cam, np.zeros((cam.height, cam.width, 3)))
+        d = np.abs(a - ref_img[..., 0])
+        print(f"  view {v}: alpha vs FP64 max {d.max():.2e} mean {d.mean():.2e}", flush=True)
+        assert d.max() < 1.0 / 255.0 and d.mean() < 1e-5
+        p1 = tensors(raw)
+        img1, info1 = rasterize(*p1, cam, info=True)
+        (img1 * torch.tensor(W[v], device="cuda")).sum().backward()
+        assert torch.equal(info["radii"][v], info1["radii"])
+        assert torch.equal(info["means2d"].grad[v], info1["means2d"].grad)
+
+
+def test_activated_parameters_follow_the_chain_rule(gpu):
+    """Standard deviations and opacities in place of log-scales and logits: images agree with the
+    raw convention to rounding (activation on the host here, on the device there), and the
+    gradients agree through d scale / d log-scale = scale and d opacity / d logit = o (1 - o)."""
+    raw = small_scene(seed=12)
+    cams = small_views(seed=6)
+    W = weights(cams, seed=8)
+    p_raw = tensors(raw)
+    imgs_r, g_r = batched(p_raw, cams, W)
+    scale = np.exp(raw.log_scales.astype(np.float64))
+    op = 1.0 / (1.0 + np.exp(-raw.opacity_logits.astype(np.float64)))
+    act = scenes.RawScene(raw.means, scale.astype(np.float32), raw.quats, op.astype(np.float32), raw.sh)
+    p_act = tensors(act)
+    imgs_a, g_a = batched(p_act, cams, W, activated=True)
+    for a, b in zip(imgs_a, imgs_r):
+        assert np.abs(a - b).max() < 1e-3
+    g_a = g_a.astype(np.float64)
+    g_a[:, 3:6] *= scale
+    g_a[:, 10] *= op * (1.0 - op)
+    assert compare("activated vs raw", g_a, g_r.astype(np.float64)) < TOL
+
+
+def test_1m_gaussians_16_uneven_views_fwd_bwd(gpu):
+    """The batched training step at full size: 1.0M Gaussians, 16 cameras of uneven sizes
+    (640x360, 1280x720, 1920x1080), L1 loss: every image equals its single-view forward and the
+    batched gradients are within 1e-6 relative of the sum of the 16 single-view gradients."""
+    raw = scenes.synthetic_scene(1_000_000, 1280, 720)
+    cams = scenes.view_batch_cameras(16)
+    params = tensors(raw)
+    rng = np.random.default_rng(0)
+    W = [np.sign(rng.normal(size=(c.height, c.width, 3))).astype(np.float32) / (c.height * c.width * 3) for c in cams]
+    stats = []
+    imgs_b, g_b = batched(params, cams, W, stats=stats)
+    print(f"case 1M x 16 views: pairs {sum(s.tile_pairs for s in stats)}", flush=True)
+    imgs_s, g_s = single_sum(params, cams, W)
+    for v, (a, b) in enumerate(zip(imgs_b, imgs_s)):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), f"view {v} image differs"
+    worst = 0.0
+    for name, sl in oracle.PARAM_GROUPS.items():
+        a, b = g_b[:, sl].astype(np.float64), g_s[:, sl].astype(np.float64)
+        e = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        print(f"  {name:15s} rel {e:.2e}", flush=True)
+        worst = max(worst, e)
+    assert worst <= 1e-6
