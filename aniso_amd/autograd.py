@@ -329,20 +329,21 @@ def _split_views(flat, cameras, channels):
 
 
 def rasterize_views(means, log_scales, quats, opacity_logits, sh, cameras, options: RenderOptions | None = None,
-                    stats: list | None = None, info: bool = False, activated: bool = False):
+                    stats: list | None = None, info: bool = False, activated: bool = False, flat: bool = False):
     """Differentiable render of many cameras (uneven sizes allowed) in one batched pass: a list of
-    unclamped colour images [H_v, W_v, 3]. Gradients w.r.t. the parameters are summed over the
-    views. With info=True returns (images, info) with "radii" (int32 [V, N]), "means2d" (a zero
-    [V, N, 2] leaf whose .grad receives dL/d(u, v) per view), "alphas" (list of [H_v, W_v]),
-    "n_cameras"."""
+    unclamped colour images [H_v, W_v, 3] (with flat=True, one tensor holding them all, view after
+    view, so that a loss over all views is a few whole-tensor operations). Gradients w.r.t. the
+    parameters are summed over the views. With info=True returns (images, info) with "radii"
+    (int32 [V, N]), "means2d" (a zero [V, N, 2] leaf whose .grad receives dL/d(u, v) per view),
+    "alphas" (list of [H_v, W_v]), "n_cameras"."""
     opts = options or RenderOptions()
     means2d = None
     if info:
         means2d = torch.zeros((len(cameras), means.shape[0], 2), dtype=torch.float32, device=means.device,
                               requires_grad=True)
-    flat, radii, alpha = RasterizeViewsFunction.apply(means, log_scales, quats, opacity_logits, sh, means2d,
-                                                      list(cameras), opts, activated, stats)
-    images = _split_views(flat, cameras, 3)
+    flat_images, radii, alpha = RasterizeViewsFunction.apply(means, log_scales, quats, opacity_logits, sh, means2d,
+                                                             list(cameras), opts, activated, stats)
+    images = flat_images if flat else _split_views(flat_images, cameras, 3)
     if not info:
         return images
     return images, {"radii": radii, "means2d": means2d, "alphas": _split_views(alpha, cameras, 1),

@@ -328,18 +328,12 @@ __host__ __device__ inline void shBasisGradRow(int k, float x, float y, float z,
     }
 }
 
-// Adjoint of projectGaussian (raw parameters): from gs = dL/d{u, v, A, B, C, opacity, r, g, b}
-// to the gradients of the mean (gm), log-scales (gls), raw quaternion (gq), opacity logit and the
-// SH coefficients (gsh, [coefficient][channel], zero past the evaluated degree). shi holds the
-// Gaussian's SH, [coefficient][channel]; gsh may be the same array unless ACC. With ACC the SH
-// gradients are added to gsh (which then holds a running sum over views) instead of stored.
-// With g.act the scales and the opacity are the activated ones: gls is then the gradient w.r.t.
-// the standard deviations and *glogit the one w.r.t. the opacity.
-template <bool ACC = false>
-__host__ __device__ inline void gaussAdjoint(const GaussProj& g, const CamParams& cam, const float gs[9],
-                                             const float* shi, float gm[3], float gls[3], float gq[4], float* glogit,
-                                             float* gsh) {
-    const float gu = gs[0], gv = gs[1], gA = gs[2], gB = gs[3], gC = gs[4], go = gs[5];
+// The view-dependent part of gaussAdjoint: the mean gradient gm (overwritten), dL/dSigma of the
+// 3D covariance (gSigma, overwritten) and the SH gradients (gsh, stored or with ACC added).
+template <bool ACC>
+__host__ __device__ inline void gaussAdjointView(const GaussProj& g, const CamParams& cam, const float gs[9],
+                                                 const float* shi, float gm[3], float gSigma[3][3], float* gsh) {
+    const float gu = gs[0], gv = gs[1], gA = gs[2], gB = gs[3], gC = gs[4];
     const float px = g.px, py = g.py, pz = g.pz;
 
     float gp0 = gu * cam.fx / pz, gp1 = gv * cam.fy / pz;
@@ -357,7 +351,6 @@ __host__ __device__ inline void gaussAdjoint(const GaussProj& g, const CamParams
     float GT[2][3];  // G T
     for (int r = 0; r < 2; ++r)
         for (int j = 0; j < 3; ++j) GT[r][j] = G[r][0] * g.T[0][j] + G[r][1] * g.T[1][j];
-    float gSigma[3][3];
     for (int i2 = 0; i2 < 3; ++i2)
         for (int j = 0; j < 3; ++j) gSigma[i2][j] = g.T[0][i2] * GT[0][j] + g.T[1][i2] * GT[1][j];
     float gT[2][3];
@@ -374,34 +367,6 @@ __host__ __device__ inline void gaussAdjoint(const GaussProj& g, const CamParams
     if (!g.clampX) gp0 += gtx; else gp2 += gtx * (g.tx / pz);
     if (!g.clampY) gp1 += gty; else gp2 += gty * (g.ty / pz);
     for (int j = 0; j < 3; ++j) gm[j] = cam.R[j] * gp0 + cam.R[3 + j] * gp1 + cam.R[6 + j] * gp2;
-    // Sigma = M M^T, M = Rq diag(s), s = exp(log-scale) * splatScale.
-    float gRq[3][3], gsc[3] = {0, 0, 0};
-    for (int r = 0; r < 3; ++r)
-        for (int c = 0; c < 3; ++c) {
-            const float gM = 2.0f * (gSigma[r][0] * g.M[0][c] + gSigma[r][1] * g.M[1][c] + gSigma[r][2] * g.M[2][c]);
-            gRq[r][c] = gM * g.s[c];
-            gsc[c] += gM * g.Rq[r][c];
-        }
-    // s = exp(log-scale) * splatScale, or (act) scale * splatScale.
-    for (int j = 0; j < 3; ++j) gls[j] = gsc[j] * (g.act ? cam.splatScale : g.s[j]);
-    const float w = g.q[0], x = g.q[1], y = g.q[2], z = g.q[3];
-    float gn[4];
-    gn[0] = 2.0f * (-z * gRq[0][1] + y * gRq[0][2] + z * gRq[1][0] - x * gRq[1][2] - y * gRq[2][0] + x * gRq[2][1]);
-    gn[1] = 2.0f * (y * gRq[0][1] + z * gRq[0][2] + y * gRq[1][0] - 2.0f * x * gRq[1][1] - w * gRq[1][2] +
-                    z * gRq[2][0] + w * gRq[2][1] - 2.0f * x * gRq[2][2]);
-    gn[2] = 2.0f * (-2.0f * y * gRq[0][0] + x * gRq[0][1] + w * gRq[0][2] + x * gRq[1][0] + z * gRq[1][2] -
-                    w * gRq[2][0] + z * gRq[2][1] - 2.0f * y * gRq[2][2]);
-    gn[3] = 2.0f * (-2.0f * z * gRq[0][0] - w * gRq[0][1] + x * gRq[0][2] + w * gRq[1][0] - 2.0f * z * gRq[1][1] +
-                    y * gRq[1][2] + x * gRq[2][0] + y * gRq[2][1]);
-    // q = q_raw / |q_raw|.
-    if (g.qlen > 0.0f) {
-        const float dot = gn[0] * w + gn[1] * x + gn[2] * y + gn[3] * z;
-        for (int j = 0; j < 4; ++j) gq[j] = (gn[j] - dot * g.q[j]) / g.qlen;
-    } else {
-        for (int j = 0; j < 4; ++j) gq[j] = 0.0f;
-    }
-    // opacity = sigmoid(logit), or (act) the opacity itself.
-    *glogit = g.act ? go : go * g.opacity * (1.0f - g.opacity);
     // SH colour, clamped at zero: d sh_k = basis_k d rgb; d dir through the basis.
     const float grgb[3] = {g.rgbRaw.x > 0.0f ? gs[6] : 0.0f, g.rgbRaw.y > 0.0f ? gs[7] : 0.0f,
                            g.rgbRaw.z > 0.0f ? gs[8] : 0.0f};
@@ -425,6 +390,57 @@ __host__ __device__ inline void gaussAdjoint(const GaussProj& g, const CamParams
     // dir = (m - eye) / |m - eye|.
     const float dd = gdir[0] * g.dir[0] + gdir[1] * g.dir[1] + gdir[2] * g.dir[2];
     for (int j = 0; j < 3; ++j) gm[j] += (gdir[j] - dd * g.dir[j]) / g.dlen;
+}
+
+// The view-independent part: from dL/dSigma and dL/dopacity (go) to the gradients of the
+// log-scales (or, with g.act, the standard deviations), the raw quaternion and the opacity logit
+// (or the opacity). Linear in (gSigma, go), so a sum over views may be passed in at once.
+__host__ __device__ inline void gaussAdjointShape(const GaussProj& g, float splatScale, const float gSigma[3][3],
+                                                  float go, float gls[3], float gq[4], float* glogit) {
+    // Sigma = M M^T, M = Rq diag(s), s = exp(log-scale) * splatScale.
+    float gRq[3][3], gsc[3] = {0, 0, 0};
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            const float gM = 2.0f * (gSigma[r][0] * g.M[0][c] + gSigma[r][1] * g.M[1][c] + gSigma[r][2] * g.M[2][c]);
+            gRq[r][c] = gM * g.s[c];
+            gsc[c] += gM * g.Rq[r][c];
+        }
+    // s = exp(log-scale) * splatScale, or (act) scale * splatScale.
+    for (int j = 0; j < 3; ++j) gls[j] = gsc[j] * (g.act ? splatScale : g.s[j]);
+    const float w = g.q[0], x = g.q[1], y = g.q[2], z = g.q[3];
+    float gn[4];
+    gn[0] = 2.0f * (-z * gRq[0][1] + y * gRq[0][2] + z * gRq[1][0] - x * gRq[1][2] - y * gRq[2][0] + x * gRq[2][1]);
+    gn[1] = 2.0f * (y * gRq[0][1] + z * gRq[0][2] + y * gRq[1][0] - 2.0f * x * gRq[1][1] - w * gRq[1][2] +
+                    z * gRq[2][0] + w * gRq[2][1] - 2.0f * x * gRq[2][2]);
+    gn[2] = 2.0f * (-2.0f * y * gRq[0][0] + x * gRq[0][1] + w * gRq[0][2] + x * gRq[1][0] + z * gRq[1][2] -
+                    w * gRq[2][0] + z * gRq[2][1] - 2.0f * y * gRq[2][2]);
+    gn[3] = 2.0f * (-2.0f * z * gRq[0][0] - w * gRq[0][1] + x * gRq[0][2] + w * gRq[1][0] - 2.0f * z * gRq[1][1] +
+                    y * gRq[1][2] + x * gRq[2][0] + y * gRq[2][1]);
+    // q = q_raw / |q_raw|.
+    if (g.qlen > 0.0f) {
+        const float dot = gn[0] * w + gn[1] * x + gn[2] * y + gn[3] * z;
+        for (int j = 0; j < 4; ++j) gq[j] = (gn[j] - dot * g.q[j]) / g.qlen;
+    } else {
+        for (int j = 0; j < 4; ++j) gq[j] = 0.0f;
+    }
+    // opacity = sigmoid(logit), or (act) the opacity itself.
+    *glogit = g.act ? go : go * g.opacity * (1.0f - g.opacity);
+}
+
+// Adjoint of projectGaussian (raw parameters): from gs = dL/d{u, v, A, B, C, opacity, r, g, b}
+// to the gradients of the mean (gm), log-scales (gls), raw quaternion (gq), opacity logit and the
+// SH coefficients (gsh, [coefficient][channel], zero past the evaluated degree). shi holds the
+// Gaussian's SH, [coefficient][channel]; gsh may be the same array unless ACC. With ACC the SH
+// gradients are added to gsh (which then holds a running sum over views) instead of stored.
+// With g.act the scales and the opacity are the activated ones: gls is then the gradient w.r.t.
+// the standard deviations and *glogit the one w.r.t. the opacity.
+template <bool ACC = false>
+__host__ __device__ inline void gaussAdjoint(const GaussProj& g, const CamParams& cam, const float gs[9],
+                                             const float* shi, float gm[3], float gls[3], float gq[4], float* glogit,
+                                             float* gsh) {
+    float gSigma[3][3];
+    gaussAdjointView<ACC>(g, cam, gs, shi, gm, gSigma, gsh);
+    gaussAdjointShape(g, cam.splatScale, gSigma, gs[5], gls, gq, glogit);
 }
 
 // Pair slots in Gaussian index order: per block of 256 Gaussians, the sum of their tile counts.

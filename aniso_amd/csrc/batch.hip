@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -34,12 +35,21 @@ ViewBatch::ViewBatch() {
                                hipHostMallocDefault),
                  "hipHostMalloc");
     }
+    hipCheck(hipHostMalloc(reinterpret_cast<void**>(&mirror), sizeof(CountMirror) * kMaxViews,
+                           hipHostMallocMapped | hipHostMallocCoherent),
+             "hipHostMalloc");
+    std::memset(mirror, 0, sizeof(CountMirror) * kMaxViews);
+    hipCheck(hipHostGetDevicePointer(reinterpret_cast<void**>(&mirrorDev), mirror, 0), "hipHostGetDevicePointer");
 }
 
 ViewBatch::~ViewBatch() {
     if (pathRender) (void)hipStreamSynchronize(pathRender);
     if (pathCopy) (void)hipStreamSynchronize(pathCopy);
+    // Kernels of a submitted batch may still write the pinned count mirror.
+    for (auto& slot : ev)
+        if (slot[2]) (void)hipEventSynchronize(slot[2]);
     dropGraphs();
+    if (mirror) (void)hipHostFree(mirror);
     for (int k = 0; k < 2; ++k) {
         if (tableHost[k]) (void)hipHostFree(tableHost[k]);
         if (ctlHost[k]) (void)hipHostFree(ctlHost[k]);
@@ -190,7 +200,7 @@ const uint32_t* enqueueBatch(ViewBatch& B, const SceneDev& s, const Layout& L, h
         k_pairTotals<<<gDup, kDupBlock, 0, st>>>(B.order.p, B.rec.p, n, B.ttByRank.p, B.rectByRank.p, B.blockPairs.p,
                                                  B.blockVisible.p, n, L.blocksDup, nullptr);
         k_scanPairs<<<dim3(1, V), kDupBlock, 0, st>>>(B.blockPairs.p, B.blockVisible.p, L.blocksDup, cap, ctl,
-                                                      L.blocksDup, nullptr, 0u);
+                                                      L.blocksDup, L.diff ? B.mirrorDev : nullptr, 1u);
         k_expandPairs<<<gDup, kDupBlock, 0, st>>>(B.order.p, B.ttByRank.p, B.rectByRank.p, n, 0, B.blockPairs.p,
                                                   B.tileA.p, B.gaussA.p, ctl, n, L.blocksDup, cap, tilesX);
     }
@@ -295,6 +305,8 @@ void submitBatch(ViewBatch& B, const SceneDev& s, const Layout& L, const std::ve
         reinterpret_cast<uint64_t*>(th + T.pairBase)[v] = 0;
     }
     *reinterpret_cast<DiffParamsDev*>(th + T.params) = io.params ? *io.params : DiffParamsDev{};
+    if (L.diff)  // the scan sets seq to 1 when it publishes the view's count
+        for (int v = 0; v < L.views; ++v) __atomic_store_n(&B.mirror[v].seq, 0u, __ATOMIC_RELEASE);
     hipCheck(hipMemcpyAsync(B.table.p, th, T.bytes, hipMemcpyHostToDevice, st), "hipMemcpyAsync table");
     hipCheck(hipEventRecord(B.ev[slot][0], st), "hipEventRecord");
     if (L.opts.flags & ANS_FLAG_NO_GRAPH) {
@@ -350,6 +362,42 @@ bool finishBatch(ViewBatch& B, const Layout& L, const std::vector<CamParams>& cp
         B.pairCap = std::max(B.pairCap, static_cast<size_t>(most) + most / 4 + 1024);
     }
     return over;
+}
+
+bool waitBatchCounts(ViewBatch& B, const Layout& L, const std::vector<CamParams>& cps, int slot) {
+    if (L.n == 0) return finishBatch(B, L, cps, slot);  // no pair-count scan ran
+    for (uint32_t spin = 1;; ++spin) {
+        int ready = 0;
+        for (int v = 0; v < L.views; ++v) ready += __atomic_load_n(&B.mirror[v].seq, __ATOMIC_ACQUIRE) == 1u;
+        if (ready == L.views) break;
+        if ((spin & 1023) == 0) {
+            // Has the batch ended without the counts arriving? Then its copied control blocks hold
+            // them.
+            const hipError_t q = hipEventQuery(B.ev[slot][2]);
+            if (q == hipSuccess) return finishBatch(B, L, cps, slot);
+            if (q != hipErrorNotReady) hipCheck(q, "hipEventQuery");
+        }
+        __builtin_ia32_pause();
+    }
+    bool over = false;
+    for (int v = 0; v < L.views; ++v) over |= B.mirror[v].overflow != 0;
+    if (over) {
+        // The rest of the batch skips its work; let it end, then grow from its control blocks.
+        hipCheck(hipEventSynchronize(B.ev[slot][2]), "hipEventSynchronize");
+        return finishBatch(B, L, cps, slot);
+    }
+    B.lastMs = 0.0f;
+    B.views = L.views;
+    B.maxTiles = L.maxTiles;
+    B.tilesPerView.resize(L.views);
+    B.lastCtl.assign(L.views, FrameCtl{});
+    for (int v = 0; v < L.views; ++v) {
+        B.tilesPerView[v] = cps[v].tilesX * cps[v].tilesY;
+        B.lastCtl[v].visible = B.mirror[v].visible;
+        B.lastCtl[v].pairs = B.mirror[v].pairs;
+    }
+    B.sortedGauss = B.sortedSlot[slot];
+    return false;
 }
 
 namespace {

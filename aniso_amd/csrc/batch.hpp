@@ -60,6 +60,11 @@ struct ViewBatch {
     // the launches, control-block copy done; pinned staging of the per-view table; pinned copy of
     // the control blocks; the buffer the batch's sorted lists ended in.
     hipEvent_t ev[2][3] = {};
+    // Differentiable batches: each view's pair count, published by the pair-count scan to pinned
+    // host memory (seq 1 once written), so the host learns of an overflow without waiting for
+    // the rest of the batch.
+    CountMirror* mirror = nullptr;
+    CountMirror* mirrorDev = nullptr;
     uint8_t* tableHost[2] = {};
     FrameCtl* ctlHost[2] = {};
     const uint32_t* sortedSlot[2] = {};
@@ -132,5 +137,9 @@ void submitBatch(ViewBatch& B, const SceneDev& s, const Layout& L, const std::ve
 // Waits for the batch submitted through `slot`, makes it the last finished batch and returns true
 // if any view's pairs overflowed the arena; the per-view capacity has then been raised.
 bool finishBatch(ViewBatch& B, const Layout& L, const std::vector<CamParams>& cps, int slot);
+// Differentiable batches: waits only until every view's pair count is published (the rest of
+// the batch keeps running) and makes the batch the last one; on an overflow it waits for the
+// batch and grows as finishBatch does, returning true. lastMs is left at 0.
+bool waitBatchCounts(ViewBatch& B, const Layout& L, const std::vector<CamParams>& cps, int slot);
 
 }  // namespace ans

@@ -166,7 +166,7 @@ int ans_diff_views_forward(ans_diff* d, const ans_params* params, int32_t n_view
             reserveArena(B, L);
             B.pairCap = L.pairCap;
             submitBatch(B, s, L, cps, io, static_cast<hipStream_t>(stream), 0);
-            if (!finishBatch(B, L, cps, 0)) break;
+            if (!waitBatchCounts(B, L, cps, 0)) break;
             ++retries;
         }
         B.lastRetries = retries;
@@ -176,6 +176,8 @@ int ans_diff_views_forward(ans_diff* d, const ans_params* params, int32_t n_view
         f->shDegree = s.shDegree;
         f->act = s.act;
         if (stats) {
+            hipCheck(hipEventSynchronize(B.ev[0][1]), "hipEventSynchronize");
+            hipCheck(hipEventElapsedTime(&B.lastMs, B.ev[0][0], B.ev[0][1]), "hipEventElapsedTime");
             for (int v = 0; v < n_views; ++v) {
                 ans_stats& st = stats[v];
                 st.visible = B.lastCtl[v].visible;

@@ -385,6 +385,7 @@ constexpr uint32_t kBigPairs = 32;
     blockPairs += static_cast<size_t>(blockIdx.y) * blockStride;
     blockVisible += static_cast<size_t>(blockIdx.y) * blockStride;
     ctl += blockIdx.y;
+    if (mirror) mirror += blockIdx.y;
     constexpr int kPer = 8;
     uint64_t carry = 0;
     uint32_t visible = 0;

@@ -169,16 +169,21 @@ __global__ __launch_bounds__(TW* TH) void k_blendBackwardViews(
 // Per-Gaussian adjoint of the differentiable batch, one wavefront per 64 consecutive Gaussians:
 // for each view in turn (v = 0, 1, ...) the wavefront streams its pair records of that view
 // (one contiguous run, as in k_gaussBackward), each lane sums its Gaussian's records in slot
-// order, projects the Gaussian into the view again and applies gaussAdjoint; the per-view
-// gradients are added in view order into running sums (SH sums in LDS), so each parameter's
-// gradient is the sum over views, in view order, of the single-view gradients, without atomics.
-// Culled (view, Gaussian) pairs contribute nothing. With P.screenGrads, dL/d(u, v) per view.
+// order, projects the Gaussian into the view again and applies the view-dependent part of the
+// adjoint (gaussAdjointView); the mean and SH gradients, dL/dSigma and dL/dopacity are added in
+// view order into running sums (SH sums in LDS), and the view-independent part
+// (gaussAdjointShape, linear in dL/dSigma and dL/dopacity) runs once on the sums. Each
+// gradient is the sum over views of the single-view gradients (up to the rounding of that
+// linear map), without atomics. Culled (view, Gaussian) pairs contribute nothing. With
+// P.screenGrads, dL/d(u, v) per view.
 [[maybe_unused]] static __global__ __launch_bounds__(kWave) void k_gaussBackwardViews(
     int n, int views, const DiffParamsDev* __restrict__ prm, const CamParams* __restrict__ cams,
     const uint32_t* __restrict__ tilesTouched, const uint32_t* __restrict__ slotStart,
     const uint64_t* __restrict__ pairBase, const float* __restrict__ partial) {
+    // Only the SH gradient sums live in LDS (12.5 KB per wavefront, so that LDS allows ten
+    // wavefronts per CU); each lane reads its Gaussian's SH from global memory, where its
+    // 48-float row stays in cache across the views.
     __shared__ float4 sRec[kWave][3];
-    __shared__ float sSh[kWave * kShRow];
     __shared__ float sGsh[kWave * kShRow];
     const DiffParamsDev P = *prm;
     const int lane = threadIdx.x;
@@ -187,26 +192,7 @@ __global__ __launch_bounds__(TW* TH) void k_blendBackwardViews(
     const int i = g0 + lane;
     const bool valid = lane < rows;
     const int K3 = P.shCoeffs * 3;  // at most 48
-    const int total = rows * K3;
-    const float* src = P.sh + static_cast<size_t>(g0) * K3;
-    {
-        float vsh[48];
-#pragma unroll
-        for (int j = 0; j < 48; ++j) {
-            const int e = lane + j * kWave;
-            vsh[j] = e < total ? src[e] : 0.0f;
-        }
-        const float rcpK3 = 1.0f / static_cast<float>(K3);
-#pragma unroll
-        for (int j = 0; j < 48; ++j) {
-            const int e = lane + j * kWave;
-            if (e < total) {
-                const int row = rowOf(e, rcpK3);
-                sSh[row * kShRow + (e - row * K3)] = vsh[j];
-            }
-        }
-    }
-    float* shRow = sSh + lane * kShRow;
+    const float* shRow = P.sh + static_cast<size_t>(valid ? i : g0) * K3;
     float* gshRow = sGsh + lane * kShRow;
     for (int j = 0; j < K3; ++j) gshRow[j] = 0.0f;
     F3 m{0.0f, 0.0f, 0.0f};
@@ -217,7 +203,7 @@ __global__ __launch_bounds__(TW* TH) void k_blendBackwardViews(
         logit = P.opacity[i];
         prepareGaussian<true>(P.rot[i], reinterpret_cast<const F3*>(P.scale)[i], cams[0].splatScale, g, P.act != 0);
     }
-    float gm[3] = {0, 0, 0}, gls[3] = {0, 0, 0}, gq[4] = {0, 0, 0, 0}, gl = 0.0f;
+    float gm[3] = {0, 0, 0}, gSig[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, go = 0.0f;
     const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     for (int v = 0; v < views; ++v) {
         const size_t vo = static_cast<size_t>(v) * n;
@@ -257,20 +243,29 @@ __global__ __launch_bounds__(TW* TH) void k_blendBackwardViews(
         }
         if (valid && P.screenGrads) P.screenGrads[vo + i] = make_float2(gs[0], gs[1]);
         if (tt > 0 && cameraTransform(m, cams[v], g) && finishProjection<false, true>(m, logit, shRow, 1, cams[v], g)) {
-            float vm[3], vls[3], vq[4], vl;
-            gaussAdjoint<true>(g, cams[v], gs, shRow, vm, vls, vq, &vl, gshRow);
-            for (int j = 0; j < 3; ++j) gm[j] += vm[j], gls[j] += vls[j];
-            for (int j = 0; j < 4; ++j) gq[j] += vq[j];
-            gl += vl;
+            float vm[3], vSig[3][3];
+            gaussAdjointView<true>(g, cams[v], gs, shRow, vm, vSig, gshRow);
+            for (int j = 0; j < 3; ++j) gm[j] += vm[j];
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) gSig[r][c] += vSig[r][c];
+            go += gs[5];
         }
     }
     if (valid) {
+        // The view-independent part once, on the sums; the rotation and scale matrices are formed
+        // again here so that they need not stay in registers through the view loop.
+        GaussProj h;
+        prepareGaussian<true>(P.rot[i], reinterpret_cast<const F3*>(P.scale)[i], cams[0].splatScale, h, P.act != 0);
+        h.opacity = h.act ? logit : 1.0f / (1.0f + expf(-logit));  // as finishProjection sets it
+        float gls[3], gq[4], gl;
+        gaussAdjointShape(h, cams[0].splatScale, gSig, go, gls, gq, &gl);
         for (int j = 0; j < 3; ++j) P.gMeans[3 * i + j] = gm[j], P.gScales[3 * i + j] = gls[j];
         reinterpret_cast<float4*>(P.gQuats)[i] = make_float4(gq[0], gq[1], gq[2], gq[3]);
         P.gOpacity[i] = gl;
     }
     waveSync();
     float* dst = P.gSh + static_cast<size_t>(g0) * K3;
+    const int total = rows * K3;
     const float rcpK3 = 1.0f / static_cast<float>(K3);
 #pragma unroll
     for (int j = 0; j < 48; ++j) {

@@ -140,6 +140,8 @@ def main():
         extra["fwd_clustered_2.2M_1080p"] = bench_forward("clustered-2.2M-1920x1080", max(10, args.steps // 2),
                                                          args.warmup)
         extra["batched_16_views"] = bench_views("batched-1M-16views", max(5, args.steps // 10), args.warmup)
+        extra["batched_16_views_fwd_bwd"] = bench_views_fwd_bwd("batched-1M-16views", max(3, args.steps // 20),
+                                                                 args.warmup)
         extra["path_stream"] = bench_path(raw, 128)
         extra["ply_ingest"] = bench_ingest(raw)
 
@@ -478,6 +480,89 @@ def bench_fwd_bwd(raw, cam, steps, warmup):
             "includes": "forward + loss + backward to means, log-scales, quats, opacity logits, SH"}
 
 
+def bench_views_fwd_bwd(workload, steps, warmup):
+    """The batched training step (1.0M Gaussians, 16 cameras of uneven sizes): one batched
+    differentiable forward of all 16 views (rasterize_views: one graph-captured launch sequence),
+    the mean L1 loss against targets rendered after perturbing means by N(0, 0.01^2) and SH DC by
+    N(0, 0.05^2), and one batched backward (one graph: blend adjoint over every view's tiles,
+    per-Gaussian adjoint summing the views), against the same step as 16 single-view forward +
+    backward passes (rasterize per view, gradients accumulated by autograd)."""
+    import torch
+
+    from aniso_amd import scenes
+    from aniso_amd.autograd import rasterize, rasterize_views
+
+    wl = scenes.WORKLOADS[workload]
+    raw = scenes.synthetic_scene(wl["n"], wl["width"], wl["height"], clustered=wl["clustered"])
+    cams = scenes.view_batch_cameras(wl["views"])
+    dev = "cuda"
+    t = lambda a, g=False: torch.tensor(np.ascontiguousarray(a), dtype=torch.float32, device=dev, requires_grad=g)
+    params = [t(a, True) for a in (raw.means, raw.log_scales, raw.quats, raw.opacity_logits, raw.sh)]
+    rng = np.random.default_rng(0)
+    tm = raw.means + rng.normal(0, 0.01, raw.means.shape).astype(np.float32)
+    tsh = raw.sh.copy()
+    tsh[:, 0, :] += rng.normal(0, 0.05, tsh[:, 0, :].shape).astype(np.float32)
+    with torch.no_grad():
+        tflat = rasterize_views(t(tm), t(raw.log_scales), t(raw.quats), t(raw.opacity_logits), t(tsh), cams,
+                                flat=True)
+    V = len(cams)
+    targets, o = [], 0
+    for c in cams:
+        targets.append(tflat[o:o + c.height * c.width * 3].view(c.height, c.width, 3))
+        o += c.height * c.width * 3
+    # The batched loss over the flat images: mean over views of each view's L1 mean, as one
+    # weighted sum (weight 1 / (V * 3 H W) for every value of a view).
+    weight = torch.cat([torch.full((c.height * c.width * 3,), 1.0 / (V * 3 * c.height * c.width), device=dev)
+                        for c in cams])
+    stats = []
+
+    def batched():
+        for p in params:
+            p.grad = None
+        flat = rasterize_views(*params, cams, stats=stats if not stats else None, flat=True)
+        loss = ((flat - tflat).abs() * weight).sum()
+        loss.backward()
+
+    def per_view():
+        for p in params:
+            p.grad = None
+        for cam, tg in zip(cams, targets):
+            loss = (rasterize(*params, cam) - tg).abs().mean() / V
+            loss.backward()
+
+    def timed(fn):
+        for _ in range(max(1, warmup)):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / steps
+
+    per_view_ms = timed(per_view)
+    batch_ms = timed(batched)
+    return {"views": V, "gaussians": raw.n, "sizes": [f"{c.width}x{c.height}" for c in cams],
+            "pairs_total": int(sum(s.tile_pairs for s in stats)),
+            "batched_ms_per_step": round(batch_ms, 4), "batched_views_per_s": round(V / (batch_ms * 1e-3), 1),
+            "per_view_ms_per_step": round(per_view_ms, 4),
+            "per_view_views_per_s": round(V / (per_view_ms * 1e-3), 1),
+            "speedup": round(per_view_ms / batch_ms, 3),
+            "speedup_basis": "16 single-view differentiable forward + backward passes (rasterize per view)",
+            "loss": "mean over views of L1 mean vs perturbed-scene targets", "steps": steps,
+            "includes": "forward of all views + loss + backward to means, log-scales, quats, opacity logits, SH"}
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(raw, cam, seconds):
     """The reference CPU renderer, compiled from its own sources (oracle/_ref), on the same
     scene and camera, all host threads (std::thread::hardware_concurrency, render.cpp:108).
@@ -500,7 +585,8 @@ def cpu_baseline(raw, cam, seconds):
         fn()
         times.append((time.perf_counter() - t) * 1e3)
     out = {"value": round(statistics.median(times), 2), "unit": "ms/frame", "cores": os.cpu_count(),
-           "kind": kind, "sample": f"{len(times)} frames of the same workload after 1 warm-up, median"}
+           "cpu": cpu_model(), "kind": kind,
+           "sample": f"{len(times)} frames of the same workload after 1 warm-up, median"}
     if oracle.ref_available():
         # The reference's PLY loader (aniso::loadPly, one thread) on the scene's file, beside the
         # device ingest leg.

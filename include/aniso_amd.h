@@ -281,10 +281,12 @@ int ans_diff_frame_dump(ans_diff* d, int64_t frame, uint32_t* values, uint32_t* 
  * views, in view order, of the single-view ans_diff_backward gradients. */
 
 /* Forward: images[v] is a DEVICE float [height][width][3] for cams[v]. Runs on `stream` after
- * the caller's earlier work there and waits for the batch (its pair arena grows and the batch is
- * rendered again if a view's pairs exceeded it). stats: NULL or n_views entries (visible,
- * tile_pairs, pair_capacity per view; blend_ms = the batch's GPU time). The frame (*frame) stays
- * valid until ans_diff_release. */
+ * the caller's earlier work there; the call waits only until every view's pair count is known
+ * (the pair arena grows and the batch is rendered again if a view's pairs exceeded it), and the
+ * rest of the batch runs on: work the caller enqueues on `stream` afterwards sees the finished
+ * images. stats: NULL or n_views entries (visible, tile_pairs, pair_capacity per view; blend_ms =
+ * the batch's GPU time; a non-NULL stats waits for the batch). The frame (*frame) stays valid
+ * until ans_diff_release. */
 int ans_diff_views_forward(ans_diff* d, const ans_params* params, int32_t n_views, const ans_camera* cams,
                            const ans_options* options, float* const* images, void* stream, int64_t* frame,
                            ans_stats* stats);

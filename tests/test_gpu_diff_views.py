@@ -3,8 +3,10 @@ against single-view differentiable frames and the FP64 CPU reference gradient.
 
 Contract: each view's image and per-tile lists equal the single-view differentiable forward's
 bit for bit, and the gradients equal the sum over views, in view order, of the single-view
-backward gradients (the same per-view arithmetic, added in the same order: bit-identical; the
-test allows 1e-6 relative). Against the FP64 reference gradient (the sum over views of the
+backward gradients within 1e-6 relative per parameter group (the mean and SH gradients are the
+same per-view values added in the same order; the log-scale, quaternion and opacity gradients
+apply their view-independent linear map once to the summed dL/dSigma and dL/dopacity instead of
+once per view, which changes only rounding). Against the FP64 reference gradient (the sum over views of the
 oracle's gradient on the GPU's own lists): 1e-3 relative per parameter group. The activated
 parameter convention (standard deviations and opacities instead of log-scales and logits) is
 checked against the raw convention through the chain rule.
@@ -74,8 +76,15 @@ def single_sum(params, cams, W, opts=None):
     return imgs, acc
 
 
-def rel(a, b):
-    return float(np.abs(a.astype(np.float64) - b).max() / max(np.abs(b).max(), 1e-30))
+def group_rel(a, b, label):
+    """Worst relative L2 difference over the parameter groups (printed per group)."""
+    worst = 0.0
+    for name, sl in oracle.PARAM_GROUPS.items():
+        x, y = a[:, sl].astype(np.float64), b[:, sl].astype(np.float64)
+        e = np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-30)
+        print(f"  {label} {name:15s} rel {e:.2e}", flush=True)
+        worst = max(worst, e)
+    return worst
 
 
 @pytest.mark.parametrize("tile", [(16, 16), (8, 8), (32, 16)], ids=["16x16", "8x8", "32x16"])
@@ -94,10 +103,7 @@ def test_views_equal_single_views_and_gradients_sum(gpu, tile):
     for v, cam in enumerate(cams):
         start, values, _ = frame_lists(*[p.detach() for p in params], cam, opts)
         assert np.array_equal(lists_b[v][0], start) and np.array_equal(lists_b[v][1], values), f"view {v} lists"
-    r = rel(g_b, g_s)
-    print(f"  batched vs sum of single-view gradients: max rel {r:.2e}, identical {np.array_equal(g_b, g_s)}",
-          flush=True)
-    assert r <= 1e-6
+    assert group_rel(g_b, g_s, "batched vs sum of single views") <= 1e-6
 
 
 def test_views_gradients_match_fp64_reference(gpu):
@@ -200,10 +206,4 @@ def test_1m_gaussians_16_uneven_views_fwd_bwd(gpu):
     imgs_s, g_s = single_sum(params, cams, W)
     for v, (a, b) in enumerate(zip(imgs_b, imgs_s)):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), f"view {v} image differs"
-    worst = 0.0
-    for name, sl in oracle.PARAM_GROUPS.items():
-        a, b = g_b[:, sl].astype(np.float64), g_s[:, sl].astype(np.float64)
-        e = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
-        print(f"  {name:15s} rel {e:.2e}", flush=True)
-        worst = max(worst, e)
-    assert worst <= 1e-6
+    assert group_rel(g_b, g_s, "batched vs sum of single views") <= 1e-6
