@@ -13,6 +13,27 @@
 
 namespace ans {
 
+// Parameter and gradient arrays of a differentiable call, read by the kernels from the
+// per-call device table (so that a captured graph replays for any tensors): the layout of
+// ans_params / ans_grads (means [n][3], log-scales [n][3], quaternions [n][4], opacity logits [n],
+// SH [n][coefficient][channel]). A batched call reads it from its per-call device table.
+struct DiffParamsDev {
+    const float* pos;
+    const float* scale;
+    const float4* rot;
+    const float* opacity;
+    const float* sh;
+    float* gMeans;
+    float* gScales;
+    float* gQuats;
+    float* gOpacity;
+    float* gSh;
+    float2* screenGrads;  // [view][n] dL/d(u, v), or null
+    int shCoeffs;
+    int act;  // ans_params.activated
+};
+
+
 template <int CTRL>
 __device__ inline float dppAdd(float v) {
     return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
@@ -370,19 +391,26 @@ __host__ __device__ inline void gaussAdjointView(const GaussProj& g, const CamPa
     // SH colour, clamped at zero: d sh_k = basis_k d rgb; d dir through the basis.
     const float grgb[3] = {g.rgbRaw.x > 0.0f ? gs[6] : 0.0f, g.rgbRaw.y > 0.0f ? gs[7] : 0.0f,
                            g.rgbRaw.z > 0.0f ? gs[8] : 0.0f};
+    // Every coefficient is read before any gradient is written (gsh may alias shi): the reads do
+    // not wait on each other's stores.
+    float gbk[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        gbk[k] = 0.0f;
+        if (k < g.shCount)
+            for (int c = 0; c < 3; ++c) gbk[k] += grgb[c] * shi[k * 3 + c];
+    }
     float gdir[3] = {0, 0, 0};
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
         if (k < g.shCount) {
-            float gbk = 0.0f;
             for (int c = 0; c < 3; ++c) {
-                gbk += grgb[c] * shi[k * 3 + c];  // read before the write below: gsh may alias shi
                 if constexpr (ACC) gsh[k * 3 + c] += grgb[c] * g.basis[k];
                 else gsh[k * 3 + c] = grgb[c] * g.basis[k];
             }
             float row[3];
             shBasisGradRow(k, g.dir[0], g.dir[1], g.dir[2], row);
-            for (int j = 0; j < 3; ++j) gdir[j] += gbk * row[j];
+            for (int j = 0; j < 3; ++j) gdir[j] += gbk[k] * row[j];
         }
     }
     if constexpr (!ACC)
@@ -507,6 +535,151 @@ __device__ inline int rowOf(int e, float rcpK3) {
     return static_cast<int>((static_cast<float>(e) + 0.5f) * rcpK3);
 }
 
+// Per Gaussian over several views (the batched differentiable frames), one wavefront per 64
+// consecutive Gaussians (index order). For each view in turn (v = 0, 1, ...) the wavefront's pair records of
+// that view are one contiguous run [slotStart of its first Gaussian, end of its last) starting at
+// pair pairBase[v] of `partial` (pairBase null: 0); it streams them 64 at a time through LDS (each
+// lane loads one 48-byte record, the next chunk prefetched while the current one is summed) and
+// each lane sums its own Gaussian's records in slot order. The lane then projects its Gaussian
+// into the view again and applies the view-dependent part of the adjoint (gaussAdjointView): the
+// mean and SH gradients, dL/dSigma and dL/dopacity are added in view order into running sums, the
+// SH sums in LDS (12.5 KB per wavefront; each lane reads its SH from global memory, where its row
+// stays in cache across views). View v + 1's first loads are issued while view v is processed.
+// The view-independent part (gaussAdjointShape, linear in dL/dSigma and dL/dopacity) runs once on
+// the sums. The SH gradients go out through LDS so that global stores are coalesced. Gaussians
+// culled in every view get zero gradients. P.screenGrads (if set): dL/d(u, v) per view [view][n].
+__device__ inline void gaussBackwardWave(int n, int views, const DiffParamsDev& P, const CamParams* __restrict__ cams,
+                                         const uint32_t* __restrict__ tilesTouched,
+                                         const uint32_t* __restrict__ slotStart,
+                                         const uint64_t* __restrict__ pairBase, const float* __restrict__ partial) {
+    __shared__ float4 sRec[kWave][3];
+    __shared__ float sGsh[kWave * kShRow];
+    const int lane = threadIdx.x;
+    const int g0 = blockIdx.x * kWave;
+    const int rows = min(kWave, n - g0);
+    const int i = g0 + lane;
+    const bool valid = lane < rows;
+    const int K3 = P.shCoeffs * 3;  // at most 48
+    float* gshRow = sGsh + lane * kShRow;
+    const float* shRow = P.sh + static_cast<size_t>(valid ? i : g0) * K3;
+    for (int j = 0; j < K3; ++j) gshRow[j] = 0.0f;
+    F3 m{0.0f, 0.0f, 0.0f};
+    float logit = 0.0f;
+    GaussProj g;
+    if (valid) {
+        m = reinterpret_cast<const F3*>(P.pos)[i];
+        logit = P.opacity[i];
+        prepareGaussian<true>(P.rot[i], reinterpret_cast<const F3*>(P.scale)[i], cams[0].splatScale, g, P.act != 0);
+    }
+    float gm[3] = {0, 0, 0}, gSig[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, go = 0.0f;
+    bool any = false;
+    const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    // Software pipeline over the views: view v + 1's tile count, first slot and first chunk of
+    // records are loaded while view v is processed, so that no view starts with two dependent
+    // round trips to memory.
+    auto recOf = [&](int v) {
+        return reinterpret_cast<const float4*>(partial + (pairBase ? pairBase[v] : uint64_t(0)) * kPartialStride);
+    };
+    uint32_t ttN = valid ? tilesTouched[i] : 0u;
+    uint32_t s0N = valid ? slotStart[i] : 0u;
+    float4 n0 = zero, n1 = zero, n2 = zero;
+    {
+        const uint32_t r0 = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(s0N)));
+        const uint32_t r1 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(s0N + ttN), rows - 1));
+        if (r0 + lane < r1) {
+            const float4* rec = recOf(0);
+            const size_t r = static_cast<size_t>(r0 + lane) * 3;
+            n0 = rec[r];
+            n1 = rec[r + 1];
+            n2 = rec[r + 2];
+        }
+    }
+    for (int v = 0; v < views; ++v) {
+        const size_t vo = static_cast<size_t>(v) * n;
+        const uint32_t tt = ttN, s0 = s0N;
+        float4 a0 = n0, a1 = n1, a2 = n2;
+        if (v + 1 < views) {
+            ttN = valid ? tilesTouched[vo + n + i] : 0u;
+            s0N = valid ? slotStart[vo + n + i] : 0u;
+        }
+        const uint32_t r0 = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(s0)));
+        const uint32_t r1 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(s0 + tt), rows - 1));
+        const float4* rec = recOf(v);
+        float gs[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (uint32_t c0 = r0; c0 < r1; c0 += kWave) {
+            waveSync();
+            sRec[lane][0] = a0;
+            sRec[lane][1] = a1;
+            sRec[lane][2] = a2;
+            const uint32_t nx = c0 + kWave + lane;
+            if (nx < r1) {
+                const size_t r = static_cast<size_t>(nx) * 3;
+                a0 = rec[r];
+                a1 = rec[r + 1];
+                a2 = rec[r + 2];
+            }
+            waveSync();
+            const uint32_t a = max(s0, c0), b = min(s0 + tt, c0 + kWave);
+            for (uint32_t k = a; k < b; ++k) {
+                const float4 x = sRec[k - c0][0], y = sRec[k - c0][1], z = sRec[k - c0][2];
+                gs[0] += x.x; gs[1] += x.y; gs[2] += x.z; gs[3] += x.w;
+                gs[4] += y.x; gs[5] += y.y; gs[6] += y.z; gs[7] += y.w;
+                gs[8] += z.x;
+            }
+        }
+        if (v + 1 < views) {
+            // The next view's first chunk, in flight during this view's adjoint.
+            n0 = n1 = n2 = zero;
+            const uint32_t q0 = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(s0N)));
+            const uint32_t q1 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(s0N + ttN), rows - 1));
+            if (q0 + lane < q1) {
+                const float4* recN = recOf(v + 1);
+                const size_t r = static_cast<size_t>(q0 + lane) * 3;
+                n0 = recN[r];
+                n1 = recN[r + 1];
+                n2 = recN[r + 2];
+            }
+        }
+        if (valid && P.screenGrads) P.screenGrads[vo + i] = make_float2(gs[0], gs[1]);
+        if (tt > 0 && cameraTransform(m, cams[v], g) && finishProjection<false, true>(m, logit, shRow, 1, cams[v], g)) {
+            float vm[3], vSig[3][3];
+            gaussAdjointView<true>(g, cams[v], gs, shRow, vm, vSig, gshRow);
+            for (int j = 0; j < 3; ++j) gm[j] += vm[j];
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) gSig[r][c] += vSig[r][c];
+            go += gs[5];
+            any = true;
+        }
+    }
+    if (valid) {
+        float gls[3] = {0, 0, 0}, gq[4] = {0, 0, 0, 0}, gl = 0.0f;
+        if (any) {
+            // The rotation and scale matrices are formed again here so that they need not stay in
+            // registers through the view loop.
+            GaussProj h;
+            prepareGaussian<true>(P.rot[i], reinterpret_cast<const F3*>(P.scale)[i], cams[0].splatScale, h,
+                                  P.act != 0);
+            h.opacity = h.act ? logit : 1.0f / (1.0f + expf(-logit));  // as finishProjection sets it
+            gaussAdjointShape(h, cams[0].splatScale, gSig, go, gls, gq, &gl);
+        }
+        for (int j = 0; j < 3; ++j) P.gMeans[3 * i + j] = gm[j], P.gScales[3 * i + j] = gls[j];
+        reinterpret_cast<float4*>(P.gQuats)[i] = make_float4(gq[0], gq[1], gq[2], gq[3]);
+        P.gOpacity[i] = gl;
+    }
+    waveSync();
+    float* dst = P.gSh + static_cast<size_t>(g0) * K3;
+    const int total = rows * K3;
+    const float rcpK3 = 1.0f / static_cast<float>(K3);
+#pragma unroll
+    for (int j = 0; j < 48; ++j) {
+        const int e = lane + j * kWave;
+        if (e < total) {
+            const int r = rowOf(e, rcpK3);
+            dst[e] = sGsh[r * kShRow + (e - r * K3)];
+        }
+    }
+}
+
 // Per Gaussian, one wavefront per 64 consecutive Gaussians (index order). The wavefront's pair
 // records are one contiguous run [slotStart of its first Gaussian, end of its last); it streams
 // them 64 at a time through LDS (each lane loads one 48-byte record, the next chunk prefetched
@@ -514,8 +687,9 @@ __device__ inline int rowOf(int e, float rcpK3) {
 // The SH of the 64 Gaussians is one contiguous block: it is loaded coalesced (every load issued
 // up front), transposed through LDS to one row per lane, and the SH gradients go back the same
 // way. gaussAdjoint then gives the gradients of means, log-scales, quaternions (w, x, y, z),
-// opacity logits and SH [coefficient][channel]; the summed splat-record gradient goes to
-// splatGrad. Culled Gaussians get zero gradients.
+// opacity logits and SH [coefficient][channel] (with act: standard deviations and opacities in
+// place of log-scales and logits); the summed splat-record gradient goes to splatGrad. Culled
+// Gaussians get zero gradients. (Batched frames use gaussBackwardWave, which adds the views.)
 [[maybe_unused]] static __global__ __launch_bounds__(kWave) void k_gaussBackward(
     int n, const float* __restrict__ pos, const float* __restrict__ logScale, const float4* __restrict__ quat,
     const float* __restrict__ logit, const float* __restrict__ sh, CamParams cam,

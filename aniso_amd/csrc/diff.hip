@@ -54,9 +54,10 @@ void Engine::enqueueBackward(FrameState& f, const SceneDev& s, const float* grad
     if (f.opts.tile_w == 8) launchBlendBackward<8, 8>(*this, f, gradImage);
     else if (f.opts.tile_w == 32) launchBlendBackward<32, 16>(*this, f, gradImage);
     else launchBlendBackward<16, 16>(*this, f, gradImage);
-    k_gaussBackward<<<(n + kWave - 1) / kWave, kWave, 0, stream>>>(static_cast<int>(n), s.pos, s.scale, s.rot, s.opacity, s.sh, f.cp,
-                                                   f.tilesTouched.p, slotStart.p, partial.p, splatGrad.p, gMeans,
-                                                   gScales, gQuats, gOpacity, gSh, f.ctl.p, s.act);
+    k_gaussBackward<<<(n + kWave - 1) / kWave, kWave, 0, stream>>>(static_cast<int>(n), s.pos, s.scale, s.rot,
+                                                                   s.opacity, s.sh, f.cp, f.tilesTouched.p,
+                                                                   slotStart.p, partial.p, splatGrad.p, gMeans,
+                                                                   gScales, gQuats, gOpacity, gSh, f.ctl.p, s.act);
     hipCheck(hipGetLastError(), "kernel launch");
 }
 

@@ -56,6 +56,32 @@ struct SceneDev {
     bool act = false;  // raw with activated scales and opacities (ans_params.activated)
 };
 
+// Device buffers of one single-view frame as plain pointers: the frame's own (control block,
+// splat records, tile counts, depth order, pair double buffers of pairCap slots, tile ranges,
+// pixel state if the backward pass will read it) and the sort and scan scratch.
+struct FrameBuffers {
+    FrameCtl* ctl;
+    SplatRec* rec;
+    uint32_t *tilesTouched, *order;
+    uint32_t *tileA, *gaussA, *tileB, *gaussB;
+    size_t pairCap;
+    uint2* ranges;
+    uint2* pixState;
+    uint32_t *depthKey, *sortKA, *sortVA, *sortKB, *sortVB, *ttByRank;
+    uint2* rectByRank;
+    uint32_t *hist, *rowTotal, *blockPairs, *blockVisible;
+};
+
+// A frame's launches: preprocess, depth sort and pair count (published to the control block and,
+// with mirror, to pinned host memory under number seq); then pair expansion, tile sort, tile
+// ranges and the blend (save: with pixel state for the backward pass), after waitBeforeBlend if
+// given. Kernel launches only.
+void enqueueCountKernels(const FrameBuffers& b, const SceneDev& s, const CamParams& cp, hipStream_t stream,
+                         FrameStamps* stamps, CountMirror* mirror, uint32_t seq);
+void enqueueRestKernels(const FrameBuffers& b, uint32_t n, const CamParams& cp, const ans_options& o, bool save,
+                        uint8_t* out8, float* outf, hipStream_t stream, FrameStamps* stamps,
+                        hipEvent_t waitBeforeBlend, const uint32_t** sortedTile, const uint32_t** sortedGauss);
+
 CamParams makeCamParams(const ans_camera& c, const ans_options& o, int sceneShDegree);
 ans_options resolveOptions(const ans_options* o);
 void checkCamera(const ans_camera* cam);
@@ -132,12 +158,7 @@ struct Engine {
                       float* outf);
     void enqueueCount(FrameState& f, const SceneDev& s, const CamParams& cp, const ans_options& o);
     void enqueueRest(FrameState& f, uint8_t* out8, float* outf);
-    // The blend launch; the kernel writes the blend stage's start and end stamps into st.
-    void launchBlend(FrameState& f, uint8_t* out8, float* outf, FrameStamps* st);
-    template <int TW, int TH>
-    void launchBlendT(FrameState& f, bool cull, uint8_t* out8, float* outf, FrameStamps* st);
-    template <int TW, int TH, int BATCH>
-    void launchBlendB(FrameState& f, bool cull, uint8_t* out8, float* outf, FrameStamps* st);
+    FrameBuffers buffers(FrameState& f);
     // Waits for the stream; true if the pair buffers overflowed in any frame since the last call
     // (they have then been grown).
     bool readBack(FrameState& f);

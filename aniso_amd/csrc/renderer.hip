@@ -146,12 +146,159 @@ void Engine::reserveScratch(size_t n, size_t pairCap) {
     blockVisible.reserve(std::max<size_t>(blocksDup, 1));
 }
 
+// ---------------------------------------------------------------------------------------------
+// The launch sequence of one frame over plain device pointers (FrameBuffers), shared by the
+// engine (grow-only buffers) and the entry point that allocates nothing (a fixed scratch array).
+
+void enqueueCountKernels(const FrameBuffers& b, const SceneDev& s, const CamParams& cp, hipStream_t stream,
+                         FrameStamps* S, CountMirror* mirror, uint32_t seq) {
+    const uint32_t n = static_cast<uint32_t>(s.n);
+    const uint32_t tiles = static_cast<uint32_t>(cp.tilesX) * static_cast<uint32_t>(cp.tilesY);
+    {
+        // The preprocess also clears the frame's control block and tile ranges.
+        const uint32_t grid = (std::max<uint32_t>(std::max<uint32_t>(n, tiles), 1) + 255) / 256;
+        if (s.raw)
+            k_preprocess<false, true><<<grid, 256, 0, stream>>>(static_cast<int>(n), s.pos, s.scale, s.rot, s.opacity,
+                                                                s.sh, cp, b.rec, b.tilesTouched, b.depthKey, b.ctl,
+                                                                b.ranges, tiles, S, s.act);
+        else if (s.shPlanar)
+            k_preprocess<true, false><<<grid, 256, 0, stream>>>(static_cast<int>(n), s.pos, s.scale, s.rot, s.opacity,
+                                                                s.sh, cp, b.rec, b.tilesTouched, b.depthKey, b.ctl,
+                                                                b.ranges, tiles, S, s.act);
+        else
+            k_preprocess<false, false><<<grid, 256, 0, stream>>>(static_cast<int>(n), s.pos, s.scale, s.rot,
+                                                                 s.opacity, s.sh, cp, b.rec, b.tilesTouched,
+                                                                 b.depthKey, b.ctl, b.ranges, tiles, S, s.act);
+    }
+    if (n > 0) {
+        // Depth sort: four stable 8-bit passes over all Gaussians (culled ones carry the largest
+        // key); ties keep ascending Gaussian index.
+        const uint32_t blocks = (n + kSortBlock * kDepthItems - 1) / (kSortBlock * kDepthItems);
+        const uint32_t* kin = b.depthKey;
+        const uint32_t* vin = nullptr;
+        uint32_t* kout[4] = {b.sortKA, b.sortKB, b.sortKA, nullptr};
+        uint32_t* vout[4] = {b.sortVA, b.sortVB, b.sortVA, b.order};
+        for (int pass = 0; pass < 4; ++pass) {
+            k_radixUpsweep<kDepthItems><<<blocks, kSortBlock, 0, stream>>>(kin, false, n, n, 8 * pass, 8, b.hist,
+                                                                           blocks, b.ctl, 0, 0);
+            k_scanRows<<<kRadix, kSortBlock, 0, stream>>>(b.hist, blocks, blocks, b.rowTotal, b.ctl, false, n,
+                                                          kSortBlock * kDepthItems, 0);
+            k_radixDownsweep<kDepthItems><<<blocks, kSortBlock, 0, stream>>>(
+                kin, vin, kout[pass], vout[pass], false, n, n, 8 * pass, 8, b.hist, b.rowTotal, blocks, b.ctl, 0, 0);
+            kin = kout[pass];
+            vin = vout[pass];
+        }
+        const uint32_t blocksDup = (n + kDupTile - 1) / kDupTile;
+        k_pairTotals<<<blocksDup, kDupBlock, 0, stream>>>(b.order, b.rec, n, b.ttByRank, b.rectByRank, b.blockPairs,
+                                                          b.blockVisible, 0, 0, S);
+        k_scanPairs<<<1, kDupBlock, 0, stream>>>(b.blockPairs, b.blockVisible, blocksDup,
+                                                 static_cast<uint32_t>(b.pairCap), b.ctl, 0, mirror, seq);
+    }
+}
+
+namespace {
+
+template <int TW, int TH, int BATCH>
+void launchBlendFrame(const FrameBuffers& b, const CamParams& cp, const uint32_t* sortedGauss, bool save, bool cull,
+                      uint8_t* out8, float* outf, FrameStamps* st, hipStream_t stream) {
+    const uint32_t tiles = static_cast<uint32_t>(cp.tilesX) * cp.tilesY;
+    if (save)  // frames for the backward pass always cull (culling changes no pixel)
+        k_blend<TW, TH, BATCH, true, true><<<tiles, TW * TH, 0, stream>>>(
+            b.ranges, sortedGauss, b.rec, cp.width, cp.height, cp.tilesX, out8, outf, b.pixState, b.ctl, st);
+    else if (cull)
+        k_blend<TW, TH, BATCH, true, false><<<tiles, TW * TH, 0, stream>>>(
+            b.ranges, sortedGauss, b.rec, cp.width, cp.height, cp.tilesX, out8, outf, nullptr, b.ctl, st);
+    else
+        k_blend<TW, TH, BATCH, false, false><<<tiles, TW * TH, 0, stream>>>(
+            b.ranges, sortedGauss, b.rec, cp.width, cp.height, cp.tilesX, out8, outf, nullptr, b.ctl, st);
+}
+
+template <int TW, int TH>
+void launchBlendFrameT(const FrameBuffers& b, const CamParams& cp, const ans_options& o, const uint32_t* sortedGauss,
+                       bool save, uint8_t* out8, float* outf, FrameStamps* st, hipStream_t stream) {
+    const bool cull = !(o.flags & ANS_FLAG_NO_CULL);
+    if (o.batch_depth == 64) launchBlendFrame<TW, TH, 64>(b, cp, sortedGauss, save, cull, out8, outf, st, stream);
+    else if (o.batch_depth == 128) launchBlendFrame<TW, TH, 128>(b, cp, sortedGauss, save, cull, out8, outf, st, stream);
+    else launchBlendFrame<TW, TH, 256>(b, cp, sortedGauss, save, cull, out8, outf, st, stream);
+}
+
+}  // namespace
+
+void enqueueRestKernels(const FrameBuffers& b, uint32_t n, const CamParams& cp, const ans_options& o, bool save,
+                        uint8_t* out8, float* outf, hipStream_t stream, FrameStamps* S, hipEvent_t waitBeforeBlend,
+                        const uint32_t** sortedTile, const uint32_t** sortedGauss) {
+    const uint32_t tiles = static_cast<uint32_t>(cp.tilesX) * static_cast<uint32_t>(cp.tilesY);
+    const int tileBits = std::max(1, bitsFor(tiles - 1));
+    const int passes = (tileBits + 7) / 8;
+    const int digitBits = (tileBits + passes - 1) / passes;
+    *sortedTile = b.tileA;
+    *sortedGauss = b.gaussA;
+    if (n > 0) {
+        const uint32_t blocksDup = (n + kDupTile - 1) / kDupTile;
+        const uint32_t cap = static_cast<uint32_t>(b.pairCap);
+        k_expandPairs<<<blocksDup, kDupBlock, 0, stream>>>(b.order, b.ttByRank, b.rectByRank, n, cp.tilesX,
+                                                           b.blockPairs, b.tileA, b.gaussA, b.ctl, 0, 0, 0, nullptr);
+        const uint32_t blocksPair =
+            static_cast<uint32_t>((b.pairCap + kSortBlock * kTileItems - 1) / (kSortBlock * kTileItems));
+        uint32_t* tk[2] = {b.tileA, b.tileB};
+        uint32_t* tv[2] = {b.gaussA, b.gaussB};
+        for (int pass = 0; pass < passes; ++pass) {
+            const int src = pass & 1, dst = src ^ 1;
+            const int shift = pass * digitBits;
+            const int bits = std::min(digitBits, tileBits - shift);
+            k_radixUpsweep<kTileItems><<<blocksPair, kSortBlock, 0, stream>>>(tk[src], true, 0, cap, shift, bits,
+                                                                              b.hist, blocksPair, b.ctl, 0, 0);
+            k_scanRows<<<1u << bits, kSortBlock, 0, stream>>>(b.hist, blocksPair, blocksPair, b.rowTotal, b.ctl, true,
+                                                              cap, kSortBlock * kTileItems, 0);
+            k_radixDownsweep<kTileItems><<<blocksPair, kSortBlock, 0, stream>>>(
+                tk[src], tv[src], tk[dst], tv[dst], true, 0, cap, shift, bits, b.hist, b.rowTotal, blocksPair, b.ctl,
+                0, 0);
+        }
+        *sortedTile = tk[passes & 1];
+        *sortedGauss = tv[passes & 1];
+        k_tileRanges<<<static_cast<uint32_t>((b.pairCap + 1023) / 1024), 256, 0, stream>>>(*sortedTile, b.ctl,
+                                                                                            b.ranges, 0, 0);
+    }
+    if (waitBeforeBlend) hipCheck(hipStreamWaitEvent(stream, waitBeforeBlend, 0), "hipStreamWaitEvent");
+    if (o.tile_w == 8) launchBlendFrameT<8, 8>(b, cp, o, *sortedGauss, save, out8, outf, S, stream);
+    else if (o.tile_w == 32) launchBlendFrameT<32, 16>(b, cp, o, *sortedGauss, save, out8, outf, S, stream);
+    else launchBlendFrameT<16, 16>(b, cp, o, *sortedGauss, save, out8, outf, S, stream);
+    hipCheck(hipGetLastError(), "kernel launch");
+}
+
 // Enqueues one frame on the engine's stream into f. Outputs (either may be null) are device
 // pointers.
 void Engine::enqueueFrame(FrameState& f, const SceneDev& s, const CamParams& cp, const ans_options& o,
                           uint8_t* out8, float* outf) {
     enqueueCount(f, s, cp, o);
     enqueueRest(f, out8, outf);
+}
+
+FrameBuffers Engine::buffers(FrameState& f) {
+    FrameBuffers b{};
+    b.ctl = f.ctl.p;
+    b.rec = f.rec.p;
+    b.tilesTouched = f.tilesTouched.p;
+    b.order = f.order.p;
+    b.tileA = f.tileA.p;
+    b.gaussA = f.gaussA.p;
+    b.tileB = f.tileB.p;
+    b.gaussB = f.gaussB.p;
+    b.pairCap = f.pairCap;
+    b.ranges = f.ranges.p;
+    b.pixState = f.pixState.p;
+    b.depthKey = depthKey.p;
+    b.sortKA = sortKA.p;
+    b.sortVA = sortVA.p;
+    b.sortKB = sortKB.p;
+    b.sortVB = sortVB.p;
+    b.ttByRank = ttByRank.p;
+    b.rectByRank = rectByRank.p;
+    b.hist = hist.p;
+    b.rowTotal = rowTotal.p;
+    b.blockPairs = blockPairs.p;
+    b.blockVisible = blockVisible.p;
+    return b;
 }
 
 // First part of a frame: preprocess, depth sort, pair counts and their scan (the pair total is
@@ -166,133 +313,22 @@ void Engine::enqueueCount(FrameState& f, const SceneDev& s, const CamParams& cp,
     f.n = n;
     f.cp = cp;
     f.opts = o;
-    FrameCtl* ctl = f.ctl.p;
     FrameStamps* S = stampsDev + stampNext;
     curStamps = S;
     stampNoPairs[stampNext] = n == 0;  // no pair-counting kernel: the sort stage is empty
     stampNext = (stampNext + 1) % kStampRing;
     stampPending = std::min(stampPending + 1, kStampRing);
-
-    {
-        // The preprocess also clears the frame's control block and tile ranges.
-        const uint32_t grid = (std::max<uint32_t>(std::max<uint32_t>(n, tiles), 1) + 255) / 256;
-        if (s.raw)
-            k_preprocess<false, true><<<grid, 256, 0, stream>>>(static_cast<int>(n), s.pos, s.scale, s.rot, s.opacity,
-                                                                s.sh, cp, f.rec.p, f.tilesTouched.p, depthKey.p, ctl,
-                                                                f.ranges.p, tiles, S, s.act);
-        else if (s.shPlanar)
-            k_preprocess<true, false><<<grid, 256, 0, stream>>>(static_cast<int>(n), s.pos, s.scale, s.rot, s.opacity,
-                                                                s.sh, cp, f.rec.p, f.tilesTouched.p, depthKey.p, ctl,
-                                                                f.ranges.p, tiles, S, s.act);
-        else
-            k_preprocess<false, false><<<grid, 256, 0, stream>>>(static_cast<int>(n), s.pos, s.scale, s.rot,
-                                                                 s.opacity, s.sh, cp, f.rec.p, f.tilesTouched.p,
-                                                                 depthKey.p, ctl, f.ranges.p, tiles, S, s.act);
-    }
-    if (n > 0) {
-        // Depth sort: four stable 8-bit passes over all Gaussians (culled ones carry the largest
-        // key); ties keep ascending Gaussian index.
-        const uint32_t blocks = (n + kSortBlock * kDepthItems - 1) / (kSortBlock * kDepthItems);
-        const uint32_t* kin = depthKey.p;
-        const uint32_t* vin = nullptr;
-        uint32_t* kout[4] = {sortKA.p, sortKB.p, sortKA.p, nullptr};
-        uint32_t* vout[4] = {sortVA.p, sortVB.p, sortVA.p, f.order.p};
-        for (int pass = 0; pass < 4; ++pass) {
-            k_radixUpsweep<kDepthItems><<<blocks, kSortBlock, 0, stream>>>(kin, false, n, n, 8 * pass, 8, hist.p,
-                                                                           blocks, ctl, 0, 0);
-            k_scanRows<<<kRadix, kSortBlock, 0, stream>>>(hist.p, blocks, blocks, rowTotal.p, ctl, false, n,
-                                                          kSortBlock * kDepthItems, 0);
-            k_radixDownsweep<kDepthItems><<<blocks, kSortBlock, 0, stream>>>(
-                kin, vin, kout[pass], vout[pass], false, n, n, 8 * pass, 8, hist.p, rowTotal.p, blocks, ctl, 0, 0);
-            kin = kout[pass];
-            vin = vout[pass];
-        }
-    }
-    if (n > 0) {
-        const uint32_t blocksDup = (n + kDupTile - 1) / kDupTile;
-        k_pairTotals<<<blocksDup, kDupBlock, 0, stream>>>(f.order.p, f.rec.p, n, ttByRank.p, rectByRank.p,
-                                                          blockPairs.p, blockVisible.p, 0, 0, S);
-        k_scanPairs<<<1, kDupBlock, 0, stream>>>(blockPairs.p, blockVisible.p, blocksDup,
-                                                 static_cast<uint32_t>(f.pairCap), ctl, 0, mirrorDev, ++countSeq);
-    }
+    enqueueCountKernels(buffers(f), s, cp, stream, S, mirrorDev, ++countSeq);
 }
 
 // Second part: pairs in depth order, a stable sort by tile id, tile ranges, blend. The pair
 // buffers may have grown since enqueueCount (their contents are first written here).
 void Engine::enqueueRest(FrameState& f, uint8_t* out8, float* outf) {
-    const uint32_t n = static_cast<uint32_t>(f.n);
-    const uint32_t tiles = static_cast<uint32_t>(f.tiles);
-    const CamParams& cp = f.cp;
-    FrameCtl* ctl = f.ctl.p;
-    reserveScratch(n, f.pairCap);
-    const int tileBits = std::max(1, bitsFor(tiles - 1));
-    const int passes = (tileBits + 7) / 8;
-    const int digitBits = (tileBits + passes - 1) / passes;
-    f.sortedTile = f.tileA.p;
-    f.sortedGauss = f.gaussA.p;
-    if (n > 0) {
-        const uint32_t blocksDup = (n + kDupTile - 1) / kDupTile;
-        const uint32_t cap = static_cast<uint32_t>(f.pairCap);
-        k_expandPairs<<<blocksDup, kDupBlock, 0, stream>>>(f.order.p, ttByRank.p, rectByRank.p, n, cp.tilesX,
-                                                           blockPairs.p, f.tileA.p, f.gaussA.p, ctl, 0, 0, 0,
-                                                           nullptr);
-        const uint32_t blocksPair =
-            static_cast<uint32_t>((f.pairCap + kSortBlock * kTileItems - 1) / (kSortBlock * kTileItems));
-        uint32_t* tk[2] = {f.tileA.p, f.tileB.p};
-        uint32_t* tv[2] = {f.gaussA.p, f.gaussB.p};
-        for (int pass = 0; pass < passes; ++pass) {
-            const int src = pass & 1, dst = src ^ 1;
-            const int shift = pass * digitBits;
-            const int bits = std::min(digitBits, tileBits - shift);
-            k_radixUpsweep<kTileItems><<<blocksPair, kSortBlock, 0, stream>>>(tk[src], true, 0, cap, shift, bits,
-                                                                              hist.p, blocksPair, ctl, 0, 0);
-            k_scanRows<<<1u << bits, kSortBlock, 0, stream>>>(hist.p, blocksPair, blocksPair, rowTotal.p, ctl, true,
-                                                              cap, kSortBlock * kTileItems, 0);
-            k_radixDownsweep<kTileItems><<<blocksPair, kSortBlock, 0, stream>>>(
-                tk[src], tv[src], tk[dst], tv[dst], true, 0, cap, shift, bits, hist.p, rowTotal.p, blocksPair, ctl,
-                0, 0);
-        }
-        f.sortedTile = tk[passes & 1];
-        f.sortedGauss = tv[passes & 1];
-        k_tileRanges<<<static_cast<uint32_t>((f.pairCap + 1023) / 1024), 256, 0, stream>>>(f.sortedTile, ctl,
-                                                                                            f.ranges.p, 0, 0);
-    }
-    if (waitBeforeBlend) hipCheck(hipStreamWaitEvent(stream, waitBeforeBlend, 0), "hipStreamWaitEvent");
-    launchBlend(f, out8, outf, curStamps);
-    hipCheck(hipGetLastError(), "kernel launch");
+    reserveScratch(f.n, f.pairCap);
+    if (f.savePixels) f.pixState.reserve(static_cast<size_t>(f.cp.width) * f.cp.height);
+    enqueueRestKernels(buffers(f), static_cast<uint32_t>(f.n), f.cp, f.opts, f.savePixels, out8, outf, stream,
+                       curStamps, waitBeforeBlend, &f.sortedTile, &f.sortedGauss);
     if (recordBlendDone) hipCheck(hipEventRecord(blendDone, stream), "hipEventRecord");
-}
-
-template <int TW, int TH, int BATCH>
-void Engine::launchBlendB(FrameState& f, bool cull, uint8_t* out8, float* outf, FrameStamps* st) {
-    const CamParams& cp = f.cp;
-    const uint32_t tiles = static_cast<uint32_t>(cp.tilesX) * cp.tilesY;
-    if (f.savePixels) {
-        // Frames for the backward pass always cull (culling changes no pixel).
-        f.pixState.reserve(static_cast<size_t>(cp.width) * cp.height);
-        k_blend<TW, TH, BATCH, true, true><<<tiles, TW * TH, 0, stream>>>(
-            f.ranges.p, f.sortedGauss, f.rec.p, cp.width, cp.height, cp.tilesX, out8, outf, f.pixState.p, f.ctl.p, st);
-    } else if (cull) {
-        k_blend<TW, TH, BATCH, true, false><<<tiles, TW * TH, 0, stream>>>(
-            f.ranges.p, f.sortedGauss, f.rec.p, cp.width, cp.height, cp.tilesX, out8, outf, nullptr, f.ctl.p, st);
-    } else {
-        k_blend<TW, TH, BATCH, false, false><<<tiles, TW * TH, 0, stream>>>(
-            f.ranges.p, f.sortedGauss, f.rec.p, cp.width, cp.height, cp.tilesX, out8, outf, nullptr, f.ctl.p, st);
-    }
-}
-
-template <int TW, int TH>
-void Engine::launchBlendT(FrameState& f, bool cull, uint8_t* out8, float* outf, FrameStamps* st) {
-    if (f.opts.batch_depth == 64) launchBlendB<TW, TH, 64>(f, cull, out8, outf, st);
-    else if (f.opts.batch_depth == 128) launchBlendB<TW, TH, 128>(f, cull, out8, outf, st);
-    else launchBlendB<TW, TH, 256>(f, cull, out8, outf, st);
-}
-
-void Engine::launchBlend(FrameState& f, uint8_t* out8, float* outf, FrameStamps* st) {
-    const bool cull = !(f.opts.flags & ANS_FLAG_NO_CULL);
-    if (f.opts.tile_w == 8) launchBlendT<8, 8>(f, cull, out8, outf, st);
-    else if (f.opts.tile_w == 32) launchBlendT<32, 16>(f, cull, out8, outf, st);
-    else launchBlendT<16, 16>(f, cull, out8, outf, st);
 }
 
 bool Engine::readBack(FrameState& f) {

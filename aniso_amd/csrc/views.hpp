@@ -8,25 +8,6 @@
 
 namespace ans {
 
-// Parameter and gradient arrays of a batched differentiable call, read by the kernels from the
-// per-call device table (so that a captured graph replays for any tensors): the layout of
-// ans_params / ans_grads (means [n][3], log-scales [n][3], quaternions [n][4], opacity logits [n],
-// SH [n][coefficient][channel]).
-struct DiffParamsDev {
-    const float* pos;
-    const float* scale;
-    const float4* rot;
-    const float* opacity;
-    const float* sh;
-    float* gMeans;
-    float* gScales;
-    float* gQuats;
-    float* gOpacity;
-    float* gSh;
-    float2* screenGrads;  // [view][n] dL/d(u, v), or null
-    int shCoeffs;
-    int act;  // ans_params.activated
-};
 
 // Clears the first ctlCount control blocks (all but sticky) and rangeCount tile ranges.
 [[maybe_unused]] static __global__ __launch_bounds__(256) void k_initViews(FrameCtl* __restrict__ ctl, uint32_t ctlCount,
@@ -166,115 +147,13 @@ __global__ __launch_bounds__(TW* TH) void k_blendBackwardViews(
                                      partial + pairBase[v] * kPartialStride);
 }
 
-// Per-Gaussian adjoint of the differentiable batch, one wavefront per 64 consecutive Gaussians:
-// for each view in turn (v = 0, 1, ...) the wavefront streams its pair records of that view
-// (one contiguous run, as in k_gaussBackward), each lane sums its Gaussian's records in slot
-// order, projects the Gaussian into the view again and applies the view-dependent part of the
-// adjoint (gaussAdjointView); the mean and SH gradients, dL/dSigma and dL/dopacity are added in
-// view order into running sums (SH sums in LDS), and the view-independent part
-// (gaussAdjointShape, linear in dL/dSigma and dL/dopacity) runs once on the sums. Each
-// gradient is the sum over views of the single-view gradients (up to the rounding of that
-// linear map), without atomics. Culled (view, Gaussian) pairs contribute nothing. With
-// P.screenGrads, dL/d(u, v) per view.
+// Per-Gaussian adjoint of the differentiable batch: gaussBackwardWave over every view, the
+// parameter and gradient arrays and the views' first pairs read from the call's device table.
 [[maybe_unused]] static __global__ __launch_bounds__(kWave) void k_gaussBackwardViews(
     int n, int views, const DiffParamsDev* __restrict__ prm, const CamParams* __restrict__ cams,
     const uint32_t* __restrict__ tilesTouched, const uint32_t* __restrict__ slotStart,
     const uint64_t* __restrict__ pairBase, const float* __restrict__ partial) {
-    // Only the SH gradient sums live in LDS (12.5 KB per wavefront, so that LDS allows ten
-    // wavefronts per CU); each lane reads its Gaussian's SH from global memory, where its
-    // 48-float row stays in cache across the views.
-    __shared__ float4 sRec[kWave][3];
-    __shared__ float sGsh[kWave * kShRow];
-    const DiffParamsDev P = *prm;
-    const int lane = threadIdx.x;
-    const int g0 = blockIdx.x * kWave;
-    const int rows = min(kWave, n - g0);
-    const int i = g0 + lane;
-    const bool valid = lane < rows;
-    const int K3 = P.shCoeffs * 3;  // at most 48
-    const float* shRow = P.sh + static_cast<size_t>(valid ? i : g0) * K3;
-    float* gshRow = sGsh + lane * kShRow;
-    for (int j = 0; j < K3; ++j) gshRow[j] = 0.0f;
-    F3 m{0.0f, 0.0f, 0.0f};
-    float logit = 0.0f;
-    GaussProj g;
-    if (valid) {
-        m = reinterpret_cast<const F3*>(P.pos)[i];
-        logit = P.opacity[i];
-        prepareGaussian<true>(P.rot[i], reinterpret_cast<const F3*>(P.scale)[i], cams[0].splatScale, g, P.act != 0);
-    }
-    float gm[3] = {0, 0, 0}, gSig[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, go = 0.0f;
-    const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    for (int v = 0; v < views; ++v) {
-        const size_t vo = static_cast<size_t>(v) * n;
-        const uint32_t tt = valid ? tilesTouched[vo + i] : 0u;
-        const uint32_t s0 = valid ? slotStart[vo + i] : 0u;
-        const uint32_t r0 = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(s0)));
-        const uint32_t r1 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(s0 + tt), rows - 1));
-        const float4* rec = reinterpret_cast<const float4*>(partial + pairBase[v] * kPartialStride);
-        float4 a0 = zero, a1 = zero, a2 = zero;
-        if (r0 + lane < r1) {
-            const size_t r = static_cast<size_t>(r0 + lane) * 3;
-            a0 = rec[r];
-            a1 = rec[r + 1];
-            a2 = rec[r + 2];
-        }
-        float gs[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        for (uint32_t c0 = r0; c0 < r1; c0 += kWave) {
-            waveSync();
-            sRec[lane][0] = a0;
-            sRec[lane][1] = a1;
-            sRec[lane][2] = a2;
-            const uint32_t nx = c0 + kWave + lane;
-            if (nx < r1) {
-                const size_t r = static_cast<size_t>(nx) * 3;
-                a0 = rec[r];
-                a1 = rec[r + 1];
-                a2 = rec[r + 2];
-            }
-            waveSync();
-            const uint32_t a = max(s0, c0), b = min(s0 + tt, c0 + kWave);
-            for (uint32_t k = a; k < b; ++k) {
-                const float4 x = sRec[k - c0][0], y = sRec[k - c0][1], z = sRec[k - c0][2];
-                gs[0] += x.x; gs[1] += x.y; gs[2] += x.z; gs[3] += x.w;
-                gs[4] += y.x; gs[5] += y.y; gs[6] += y.z; gs[7] += y.w;
-                gs[8] += z.x;
-            }
-        }
-        if (valid && P.screenGrads) P.screenGrads[vo + i] = make_float2(gs[0], gs[1]);
-        if (tt > 0 && cameraTransform(m, cams[v], g) && finishProjection<false, true>(m, logit, shRow, 1, cams[v], g)) {
-            float vm[3], vSig[3][3];
-            gaussAdjointView<true>(g, cams[v], gs, shRow, vm, vSig, gshRow);
-            for (int j = 0; j < 3; ++j) gm[j] += vm[j];
-            for (int r = 0; r < 3; ++r)
-                for (int c = 0; c < 3; ++c) gSig[r][c] += vSig[r][c];
-            go += gs[5];
-        }
-    }
-    if (valid) {
-        // The view-independent part once, on the sums; the rotation and scale matrices are formed
-        // again here so that they need not stay in registers through the view loop.
-        GaussProj h;
-        prepareGaussian<true>(P.rot[i], reinterpret_cast<const F3*>(P.scale)[i], cams[0].splatScale, h, P.act != 0);
-        h.opacity = h.act ? logit : 1.0f / (1.0f + expf(-logit));  // as finishProjection sets it
-        float gls[3], gq[4], gl;
-        gaussAdjointShape(h, cams[0].splatScale, gSig, go, gls, gq, &gl);
-        for (int j = 0; j < 3; ++j) P.gMeans[3 * i + j] = gm[j], P.gScales[3 * i + j] = gls[j];
-        reinterpret_cast<float4*>(P.gQuats)[i] = make_float4(gq[0], gq[1], gq[2], gq[3]);
-        P.gOpacity[i] = gl;
-    }
-    waveSync();
-    float* dst = P.gSh + static_cast<size_t>(g0) * K3;
-    const int total = rows * K3;
-    const float rcpK3 = 1.0f / static_cast<float>(K3);
-#pragma unroll
-    for (int j = 0; j < 48; ++j) {
-        const int e = lane + j * kWave;
-        if (e < total) {
-            const int r = rowOf(e, rcpK3);
-            dst[e] = sGsh[r * kShRow + (e - r * K3)];
-        }
-    }
+    gaussBackwardWave(n, views, *prm, cams, tilesTouched, slotStart, pairBase, partial);
 }
 
 }  // namespace ans
