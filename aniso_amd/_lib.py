@@ -106,6 +106,8 @@ SIGNATURES = {
     "ans_diff_views_radii": (ctypes.c_int, [P, ctypes.c_int64, P, P]),
     "ans_diff_views_counts": (ctypes.c_int, [P, ctypes.c_int64, ctypes.c_int32, P, P, P]),
     "ans_diff_views_dump": (ctypes.c_int, [P, ctypes.c_int64, ctypes.c_int32, P, P]),
+    "ans_diff_forward_noalloc": (ctypes.c_int, [ctypes.POINTER(Params_), ctypes.POINTER(Camera_),
+                                                ctypes.POINTER(Options_), P, P, ctypes.POINTER(Stats_)]),
     "ans_ply_write": (ctypes.c_int, [ctypes.POINTER(Params_), P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), P]),
 }
 

@@ -249,6 +249,17 @@ int ans_diff_forward(ans_diff* d, const ans_params* params, const ans_camera* ca
 int ans_diff_backward(ans_diff* d, int64_t frame, const ans_params* params, const float* grad_image,
                       const ans_grads* grads, void* stream);
 
+/* ans_diff_forward's image without allocating anything, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops). No counterpart in the reference. Every
+ * per-frame buffer lives in one fixed 64 MiB array in device memory that the library holds on
+ * each device for the life of the process: at most 524,288 Gaussians, and the pairs that fit in
+ * what the Gaussians and tiles leave (stats->pair_capacity); a frame with more pairs returns
+ * ANS_ERR_CAPACITY and leaves `image` unwritten. Runs on `stream` and waits for it. Calls on one
+ * device share that array: calls that overlap in time on one device race. Keeps no frame for a
+ * backward pass. */
+int ans_diff_forward_noalloc(const ans_params* params, const ans_camera* cam, const ans_options* options,
+                             float* image, void* stream, ans_stats* stats);
+
 /* Releases a frame of ans_diff_forward or ans_diff_views_forward. */
 int ans_diff_release(ans_diff* d, int64_t frame);
 
