@@ -37,6 +37,9 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--skip-extra", action="store_true", help="only the headline forward workload")
+    ap.add_argument("--only", default="",
+                    help="comma-separated extra legs to run (fwd_bwd, clustered, views, views_fwd_bwd, path, "
+                         "ingest); default: all")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a plain 1 GiB device copy (the reference kernel for HBM counter scaling)")
     args = ap.parse_args()
@@ -96,25 +99,7 @@ def main():
     n = raw.n
     b_fwd = 248 * n + 64 * V + 84 * P + 8 * T + 3 * W * H
     b_blend = 40 * P + 3 * W * H
-    stage_ms = {"project": proj, "sort": sort_, "blend": blend}
-    dominant = max(stage_ms, key=stage_ms.get)
-    roofline = {
-        "bound": "hbm",
-        "kernel": "k_blend",
-        "achieved": round(b_blend / (blend * 1e-3) / 1e9, 1),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(b_blend / (blend * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-        "traffic": measured_traffic("k_blend", args.workload),
-        "algorithmic_bytes": b_blend,
-        "kernel_ms": round(blend, 4),
-        "valu": valu_bound(measured_traffic("k_blend", args.workload, "valu_instructions_per_launch"), blend),
-        "timing": f"device wall-clock stamps at the blend's start (workgroup 0) and end (last "
-                  f"workgroup to finish), mean of the last {min(args.steps, 64)} timed frames; the "
-                  f"timed frames overlap on the library's three streams, so the blend shares the "
-                  f"chip with the next frames' sorts (isolated: the same kernel in frames rendered "
-                  f"one at a time)",
-    }
+    overlapped_ms = {"project": proj, "sort": sort_, "blend": blend}
 
     # The same frame through the host-output entry (ans_render: synchronous, image copied to host
     # memory over PCIe), as the reference's GpuRenderer::render returns it.
@@ -129,20 +114,32 @@ def main():
     extra = {"host_output": {"ms_per_frame": round(host_ms, 4), "steps": host_steps,
                              "includes": "synchronous ans_render: frame, pair-count readback, RGB8 copy to host"},
              "single_frame": bench_single_frames(r, L, c_cam, c_opt, dev_img)}
-    iso = extra["single_frame"]["warm_blend_median_ms"]
-    roofline["isolated"] = {"kernel_ms": iso, "achieved": round(b_blend / (iso * 1e-3) / 1e9, 1),
-                            "frac": round(b_blend / (iso * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                            "valu": valu_bound(measured_traffic("k_blend", args.workload,
-                                                                "valu_instructions_per_launch"), iso)}
-    if not args.skip_extra:
+    sf = extra["single_frame"]
+    # Stage costs from frames rendered one at a time (the pipelined frames overlap, so their stage
+    # spans add up to more than a frame).
+    stage_ms = sf.pop("stages_ms")
+    dominant = max(stage_ms, key=stage_ms.get)
+    roofline = blend_roofline(args.workload, sf["warm_blend_median_ms"], b_blend)
+    roofline["overlapped"] = {
+        "kernel_ms": round(blend, 4),
+        "note": "the same kernel's stamp-timed mean in the timed frames, which overlap on the library's three "
+                "streams (the blend shares the chip with the next frames' sorts)"}
+    legs = set(x for x in args.only.split(",") if x) if args.only else None
+    want = lambda leg: not args.skip_extra and (legs is None or leg in legs)
+    if want("fwd_bwd"):
         extra["fwd_bwd"] = bench_fwd_bwd(raw, cam, max(5, args.steps // 4), args.warmup)
+    if want("clustered"):
         extra["fwd_clustered_2.2M"] = bench_forward("clustered-2.2M-1280x720", max(10, args.steps // 2), args.warmup)
         extra["fwd_clustered_2.2M_1080p"] = bench_forward("clustered-2.2M-1920x1080", max(10, args.steps // 2),
                                                          args.warmup)
+    if want("views"):
         extra["batched_16_views"] = bench_views("batched-1M-16views", max(5, args.steps // 10), args.warmup)
-        extra["batched_16_views_fwd_bwd"] = bench_views_fwd_bwd("batched-1M-16views", max(3, args.steps // 20),
+    if want("views_fwd_bwd"):
+        extra["batched_16_views_fwd_bwd"] = bench_views_fwd_bwd("batched-1M-16views", max(5, args.steps // 10),
                                                                  args.warmup)
+    if want("path"):
         extra["path_stream"] = bench_path(raw, 128)
+    if want("ingest"):
         extra["ply_ingest"] = bench_ingest(raw)
 
     cpu = None
@@ -163,18 +160,23 @@ def main():
         "ms_per_step": round(ms, 4),
         "higher_is_better": False,
         "scaling": "weak",
-        "vs_baseline": round(ms / 2.3, 4),
+        "vs_baseline": round(sf["warm_median_ms"] / 2.3, 4),
         "dtype": "fp32",
         "data": "synthetic",
         "config": {"workload": args.workload, "gaussians": n, "width": W, "height": H, "sh_degree": 3,
                    "tile": "16x16", "batch_depth": opts.batch_depth, "visible": V, "pairs": P},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "stages_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "stages_ms": stage_ms,
+        "stages_basis": "median stage times of frames rendered one at a time (device clock stamps)",
+        "stages_ms_overlapped": {k: round(v, 4) for k, v in overlapped_ms.items()},
         "dominant_stage": dominant,
         "frame_hbm": {"algorithmic_bytes": b_fwd, "achieved_GBs": round(b_fwd / (ms * 1e-3) / 1e9, 1)},
-        "vs_baseline_note": "value / 2.3 ms, the reference's published RTX 5090 forward time for its "
-                            "741,883-Gaussian SH3 scene at 1280 wide (README.md:26-28); lower is better",
+        "kernels": kernel_table(args.workload),
+        "vs_baseline_note": "single_frame.warm_median_ms (one frame rendered and waited for, the way the "
+                            "reference's render() is timed, main.cpp:161-162) / 2.3 ms, the reference's "
+                            "published RTX 5090 forward time for its 741,883-Gaussian SH3 scene at 1280 wide "
+                            "(README.md:26-28); lower is better",
         **extra,
     }
     print(json.dumps(out), flush=True)
@@ -225,16 +227,21 @@ def bench_single_frames(r, L, c_cam, c_opt, dev_img, warm=25, cold=10):
 
     from aniso_amd import _lib
 
+    stages = []
+
     def one():
         t0 = time.perf_counter()
         _lib.check(L.ans_render_async(r._h, c_cam, c_opt, dev_img.data_ptr(), None))
         st = _lib.Stats_()
         _lib.check(L.ans_render_wait(r._h, ctypes.byref(st)))
         wall = (time.perf_counter() - t0) * 1e3
+        stages.append((st.project_ms, st.sort_ms, st.blend_ms))
         return wall, st.project_ms + st.sort_ms + st.blend_ms, st.blend_ms
 
     one()
+    stages.clear()
     warm_runs = [one() for _ in range(warm)]
+    warm_stages = list(stages)
     scratch = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")
     cold_runs = []
     for k in range(cold):
@@ -250,19 +257,70 @@ def bench_single_frames(r, L, c_cam, c_opt, dev_img, warm=25, cold=10):
             "cold_gpu_median_ms": med([g for _, g, _ in cold_runs]),
             "warm_runs": warm, "cold_runs": cold,
             "cold_flush": "1 GiB device write before each cold frame (evicts the last-level cache)",
-            "includes": "ans_render_async + ans_render_wait per frame; gpu = stamp-timed project + sort + blend"}
+            "includes": "ans_render_async + ans_render_wait per frame; gpu = stamp-timed project + sort + blend",
+            "stages_ms": {k: med([x[i] for x in warm_stages]) for i, k in enumerate(("project", "sort", "blend"))}}
 
 
-def valu_bound(instructions, kernel_ms):
-    """The blend's compute side beside the HBM roofline: wave instructions per launch (the
-    profile's SQ_INSTS_VALU pass) over the chip's VALU issue capacity during the stamp-timed
-    kernel, at the peak clock (a lower bound on the busy fraction if the clock runs lower)."""
-    if not instructions:
+def profile_figure(kernel, workload, field):
+    """A figure of `kernel` on `workload` from the committed counter summary
+    profiles/<kernel>_<workload>.md (its `<field>:` line), or None."""
+    import re
+
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}.md")
+    if not os.path.exists(path):
         return None
-    capacity = SIMDS * kernel_ms * 1e-3 * CLOCK_HZ / VALU_CYCLES_PER_INST
-    return {"instructions_per_launch": instructions, "capacity_per_launch": int(capacity),
-            "busy_frac": round(instructions / capacity, 3),
-            "model": "wave64 FP32 instruction = 4 cycles on a 16-lane SIMD (no packed FP32), 1024 SIMDs, 2.4 GHz"}
+    m = re.search(rf"^{field}:\s*([0-9.eE+-]+)", open(path).read(), re.M)
+    return float(m.group(1)) if m else None
+
+
+def blend_roofline(workload, kernel_ms, b_blend):
+    """The blend's roofline. Its bound is the vector ALU (about 20 instructions per visited splat and
+    8x8 pixel block; its HBM traffic is 12% of peak), so the primary figure is the VALU issue rate:
+    wave instructions per launch (SQ_INSTS_VALU, committed profile) over the isolated kernel time,
+    against 1,024 SIMDs x 2.4 GHz / 4 cycles per wave64 FP32 instruction. HBM is the secondary
+    bound: algorithmic bytes (40 per pair + 3 per pixel) over the same time, against 8 TB/s, with
+    the measured traffic (FETCH_SIZE + WRITE_SIZE, calibrated on a plain copy)."""
+    peak = SIMDS * CLOCK_HZ / VALU_CYCLES_PER_INST / 1e9
+    inst = profile_figure("k_blend", workload, "valu_instructions_per_launch")
+    achieved = inst / (kernel_ms * 1e-3) / 1e9 if inst else None
+    hbm = b_blend / (kernel_ms * 1e-3) / 1e9
+    frags = profile_figure("k_blend", workload, "fragments_per_frame")
+    return {
+        "bound": "valu",
+        "kernel": "k_blend",
+        "achieved": round(achieved, 1) if achieved else None,
+        "peak": round(peak, 1),
+        "unit": "G wave64 VALU instructions/s",
+        "frac": round(achieved / peak, 4) if achieved else None,
+        "traffic": measured_traffic("k_blend", workload),
+        "kernel_ms": round(kernel_ms, 4),
+        "timing": "device clock stamps at the blend's start and end, median of frames rendered one at a time "
+                  "(the rocprof kernel-trace mean of the same launches is in the committed profile)",
+        "valu_instructions_per_launch": int(inst) if inst else None,
+        "occupancy_waves_per_cu": profile_figure("k_blend", workload, "mean_occupancy_waves_per_cu"),
+        "lds_bank_conflict_cycles": profile_figure("k_blend", workload, "lds_bank_conflict_cycles"),
+        "fragments_per_frame": int(frags) if frags else None,
+        "fragments_per_s": round(frags / (kernel_ms * 1e-3), 1) if frags else None,
+        "hbm": {"achieved": round(hbm, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(hbm / HBM_PEAK_GBS, 4), "algorithmic_bytes": b_blend},
+    }
+
+
+def kernel_table(workload):
+    """Per-kernel HBM rates of a frame and a training step from the committed counter profile
+    profiles/kernels_<workload>.md: measured bytes per launch over the rocprof mean duration."""
+    import re
+
+    path = os.path.join(ROOT, "profiles", f"kernels_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    out = {}
+    for m in re.finditer(r"^kernel: (\S+) us=([0-9.]+) fetch=([0-9]+) write=([0-9]+)", open(path).read(), re.M):
+        name, us, fetch, write = m.group(1), float(m.group(2)), int(m.group(3)), int(m.group(4))
+        gbs = (fetch + write) / (us * 1e-6) / 1e9
+        out[name] = {"us": us, "hbm_bytes": fetch + write, "GBs": round(gbs, 1),
+                     "frac_of_8TBs": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
 
 
 def bench_forward(workload, steps, warmup):

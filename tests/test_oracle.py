@@ -78,6 +78,20 @@ def test_reference_mode_matches_c3_both_sizes():
             assert (f.visible, f.pairs) == counts
 
 
+def test_blend_fragment_count_in_profile():
+    """The fragment count the blend's profile (and bench.py's roofline) quotes for the
+    741,883-Gaussian 1280x720 frame: list entries consumed per pixel until T < 1e-4, summed over
+    pixels, at 16x16 tiles."""
+    import re
+
+    text = open(os.path.join(os.path.dirname(os.path.dirname(__file__)), "profiles",
+                             "k_blend_synthetic-741883-1280x720.md")).read()
+    quoted = int(re.search(r"^fragments_per_frame:\s*(\d+)", text, re.M).group(1))
+    act = oracle.activate(scenes.synthetic_scene(741_883, 1280, 720))
+    with oracle.Frame(act, scenes.pinhole_camera(1280, 720), oracle.KEYEXACT, tile=(16, 16)) as f:
+        assert int(f.n_contrib().astype(np.int64).sum()) == quoted
+
+
 def test_reference_mode_matches_mt19937_scene(mt):
     for w, hh in ((320, 240), (333, 217)):
         for deg in (3, 0):
