@@ -95,6 +95,7 @@ void reserveArena(ViewBatch& B, const Layout& L) {
     moved |= grow(B.kB, vn);
     moved |= grow(B.vB, vn);
     moved |= grow(B.ttByRank, vn);
+    moved |= grow(B.rectIdx, vn);
     moved |= grow(B.rectByRank, vn);
     const size_t histD = V * kRadix * size_t(L.blocksD), histP = V * kRadix * size_t(L.blocksP),
                  histI = size_t(kRadix) * L.blocksI;
@@ -172,11 +173,11 @@ const uint32_t* enqueueBatch(ViewBatch& B, const SceneDev& s, const Layout& L, h
         if (L.diff)
             k_preprocessViewsRaw<<<(n + 255) / 256, 256, 0, st>>>(
                 static_cast<int>(n), L.views, reinterpret_cast<const DiffParamsDev*>(B.table.p + T.params), cams,
-                B.rec.p, B.tilesTouched.p, B.depthKey.p);
+                B.rec.p, B.tilesTouched.p, B.depthKey.p, B.rectIdx.p);
         else
             k_preprocessViews<<<(n + 255) / 256, 256, 0, st>>>(static_cast<int>(n), L.views, s.pos, s.scale, s.rot,
                                                                s.opacity, s.sh, (s.shDegree + 1) * (s.shDegree + 1),
-                                                               cams, B.rec.p, nullptr, B.depthKey.p);
+                                                               cams, B.rec.p, nullptr, B.depthKey.p, B.rectIdx.p);
         // Depth sort of each view's Gaussians: four stable 8-bit passes.
         const uint32_t histD = kRadix * L.blocksD;  // depth sort: kPairItems keys per thread
         const uint32_t* kin = B.depthKey.p;
@@ -197,7 +198,7 @@ const uint32_t* enqueueBatch(ViewBatch& B, const SceneDev& s, const Layout& L, h
         }
         // Pairs of each view in depth order, into the view's region of the pair arena.
         const dim3 gDup(L.blocksDup, V);
-        k_pairTotals<<<gDup, kDupBlock, 0, st>>>(B.order.p, B.rec.p, n, B.ttByRank.p, B.rectByRank.p, B.blockPairs.p,
+        k_pairTotals<<<gDup, kDupBlock, 0, st>>>(B.order.p, B.rectIdx.p, n, B.ttByRank.p, B.rectByRank.p, B.blockPairs.p,
                                                  B.blockVisible.p, n, L.blocksDup, nullptr);
         k_scanPairs<<<dim3(1, V), kDupBlock, 0, st>>>(B.blockPairs.p, B.blockVisible.p, L.blocksDup, cap, ctl,
                                                       L.blocksDup, L.diff ? B.mirrorDev : nullptr, 1u);

@@ -45,7 +45,7 @@ struct ViewBatch {
     DevBuf<FrameCtl> ctl;  // one per view, plus one for the item-order sort
     DevBuf<SplatRec> rec;
     DevBuf<uint32_t> depthKey, order, kA, vA, kB, vB, ttByRank;
-    DevBuf<uint2> rectByRank;
+    DevBuf<uint2> rectIdx, rectByRank;
     DevBuf<uint32_t> hist, rowTotal, blockPairs, blockVisible;
     DevBuf<uint32_t> tileA, gaussA, tileB, gaussB;
     DevBuf<uint2> ranges;

@@ -68,7 +68,7 @@ struct FrameBuffers {
     uint2* ranges;
     uint2* pixState;
     uint32_t *depthKey, *sortKA, *sortVA, *sortKB, *sortVB, *ttByRank;
-    uint2* rectByRank;
+    uint2 *rectIdx, *rectByRank;
     uint32_t *hist, *rowTotal, *blockPairs, *blockVisible;
 };
 
@@ -138,6 +138,7 @@ struct Engine {
 
     DevBuf<uint32_t> depthKey, sortKA, sortVA, sortKB, sortVB;
     DevBuf<uint32_t> ttByRank;      // tile counts in depth order
+    DevBuf<uint2> rectIdx;          // packed tile rects by Gaussian index
     DevBuf<uint2> rectByRank;       // tile rects in depth order
     DevBuf<uint32_t> hist;          // per-block digit histograms, scanned in place
     DevBuf<uint32_t> rowTotal;      // per-digit totals of one pass

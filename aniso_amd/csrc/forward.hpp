@@ -254,19 +254,32 @@ __host__ __device__ inline bool finishProjection(const F3 m, float opacityIn, co
     return true;
 }
 
-// Writes a culled Gaussian's tile count (0, also in its record) and depth key (the largest).
+// Tile rects packed as (x0 | y0 << 16, x1 | y1 << 16), inclusive; a culled Gaussian's is the
+// empty rect (1, 0) (x1 < x0). A visible Gaussian's rect is never empty: the screen cull keeps
+// u - r < width and u + r >= 0, so 0 <= x0 <= x1 <= tilesX - 1, and likewise in y.
+constexpr uint2 kEmptyRect = {1u, 0u};
+
+__device__ inline uint32_t rectTileCount(uint2 rp) {
+    const uint32_t x0 = rp.x & 0xffffu, y0 = rp.x >> 16, x1 = rp.y & 0xffffu, y1 = rp.y >> 16;
+    return x1 >= x0 ? (x1 - x0 + 1) * (y1 - y0 + 1) : 0u;
+}
+
+// Writes a culled Gaussian's tile count (0, also in its record), empty rect and depth key (the
+// largest).
 __device__ inline void storeCulled(size_t o, SplatRec* __restrict__ rec, uint32_t* __restrict__ tilesTouched,
-                                   uint32_t* __restrict__ depthKey) {
+                                   uint32_t* __restrict__ depthKey, uint2* __restrict__ rect) {
     rec[o].meta = make_uint4(0u, 0u, 0u, 0u);
     if (tilesTouched) tilesTouched[o] = 0;
+    rect[o] = kEmptyRect;
     depthKey[o] = 0xffffffffu;
 }
 
 // Writes a visible Gaussian's splat record (with its tile rect and count), tile count (unless
-// tilesTouched is null: the batched views read the count from the record) and depth key at
-// index o.
+// tilesTouched is null), packed tile rect (a compact copy of the record's, which the pair count
+// gathers in depth order) and depth key at index o.
 __device__ inline void storeSplat(const GaussProj& g, size_t o, SplatRec* __restrict__ rec,
-                                  uint32_t* __restrict__ tilesTouched, uint32_t* __restrict__ depthKey) {
+                                  uint32_t* __restrict__ tilesTouched, uint32_t* __restrict__ depthKey,
+                                  uint2* __restrict__ rect) {
     // Region where op * exp(-q/2) >= 1/255: q <= 2 ln(255 op); half-extents sqrt(k a), sqrt(k c)
     // of that ellipse, widened by a 5% + 0.1 margin far above the FP32 rounding of the blend's
     // exponent.
@@ -281,9 +294,10 @@ __device__ inline void storeSplat(const GaussProj& g, size_t o, SplatRec* __rest
     r.geo = make_float4(g.u, g.v, g.A, g.B);
     r.geo2 = make_float4(g.C, g.opacity, ex, ey);
     r.col = make_float4(g.rgb.x, g.rgb.y, g.rgb.z, g.depth);
-    r.meta = make_uint4(static_cast<uint32_t>(g.x0) | (static_cast<uint32_t>(g.y0) << 16),
-                        static_cast<uint32_t>(g.x1) | (static_cast<uint32_t>(g.y1) << 16), tt,
-                        static_cast<uint32_t>(g.radius));
+    const uint2 rp = make_uint2(static_cast<uint32_t>(g.x0) | (static_cast<uint32_t>(g.y0) << 16),
+                                static_cast<uint32_t>(g.x1) | (static_cast<uint32_t>(g.y1) << 16));
+    r.meta = make_uint4(rp.x, rp.y, tt, static_cast<uint32_t>(g.radius));
+    rect[o] = rp;
     if (tilesTouched) tilesTouched[o] = tt;
     depthKey[o] = __float_as_uint(g.depth);
 }
@@ -299,7 +313,8 @@ __global__ __launch_bounds__(256) void k_preprocess(int n, const float* __restri
                                                     const float* __restrict__ opacity, const float* __restrict__ sh,
                                                     CamParams cam, SplatRec* __restrict__ rec,
                                                     uint32_t* __restrict__ tilesTouched,
-                                                    uint32_t* __restrict__ depthKey, FrameCtl* __restrict__ ctl,
+                                                    uint32_t* __restrict__ depthKey, uint2* __restrict__ rect,
+                                                    FrameCtl* __restrict__ ctl,
                                                     uint2* __restrict__ ranges, uint32_t tiles,
                                                     FrameStamps* __restrict__ stamps, bool act) {
     const int i = blockIdx.x * 256 + threadIdx.x;
@@ -320,10 +335,10 @@ __global__ __launch_bounds__(256) void k_preprocess(int n, const float* __restri
     const float* shBase = PLANAR ? sh + i : sh + static_cast<size_t>(i) * cam.shCoeffs * 3;
     if (!projectGaussian<PLANAR, RAW>(i, pos, scale, rot, opacity, shBase, PLANAR ? static_cast<size_t>(n) : 1, cam,
                                       g, act)) {
-        storeCulled(i, rec, tilesTouched, depthKey);
+        storeCulled(i, rec, tilesTouched, depthKey, rect);
         return;
     }
-    storeSplat(g, i, rec, tilesTouched, depthKey);
+    storeSplat(g, i, rec, tilesTouched, depthKey, rect);
 }
 
 constexpr int kDupBlock = 256;                       // 4 wavefronts x 64 depth ranks
@@ -336,11 +351,12 @@ constexpr int kDupTile = kDupBlock;                  // depth ranks per block
 // Gaussians with more tiles than this are expanded by their whole wavefront, one at a time.
 constexpr uint32_t kBigPairs = 32;
 
-// Pairs and visible Gaussians per block of 256 depth ranks. Also gathers each rank's tile count
-// and tile rect (one line of its splat record) into depth order (ttByRank, rectByRank; rect left
-// unwritten for culled ranks), so that the expansion reads them with coalesced loads.
+// Pairs and visible Gaussians per block of 256 depth ranks. Also gathers each rank's tile rect
+// (8 bytes of the compact rect array) into depth order with its tile count (ttByRank,
+// rectByRank; rect left unwritten for culled ranks), so that the expansion reads them with
+// coalesced loads.
 [[maybe_unused]] static __global__ __launch_bounds__(kDupBlock) void k_pairTotals(const uint32_t* __restrict__ sortedIdx,
-                                                                 const SplatRec* __restrict__ rec, uint32_t n,
+                                                                 const uint2* __restrict__ rect, uint32_t n,
                                                                  uint32_t* __restrict__ ttByRank,
                                                                  uint2* __restrict__ rectByRank,
                                                                  uint32_t* __restrict__ blockPairs,
@@ -351,7 +367,7 @@ constexpr uint32_t kBigPairs = 32;
     if (stamps && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) stamps->t[1] = wall_clock64();
     const size_t go = static_cast<size_t>(blockIdx.y) * gaussStride;
     sortedIdx += go;
-    rec += go;
+    rect += go;
     ttByRank += go;
     rectByRank += go;
     blockPairs += static_cast<size_t>(blockIdx.y) * blockStride;
@@ -359,10 +375,10 @@ constexpr uint32_t kBigPairs = 32;
     const uint32_t r = blockIdx.x * kDupTile + threadIdx.x;
     uint32_t c = 0;
     if (r < n) {
-        const uint4 m = rec[sortedIdx[r]].meta;
-        c = m.z;
+        const uint2 rp = rect[sortedIdx[r]];
+        c = rectTileCount(rp);
         ttByRank[r] = c;
-        if (c) rectByRank[r] = make_uint2(m.x, m.y);
+        if (c) rectByRank[r] = rp;
     }
     uint32_t total;
     blockExclusiveScan<kDupBlock>(c, sTmp, &total);

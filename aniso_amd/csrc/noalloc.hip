@@ -57,6 +57,7 @@ bool layout(uint8_t* base, size_t n, size_t tiles, FrameBuffers& b) {
     b.sortKB = reinterpret_cast<uint32_t*>(take(4 * m));
     b.sortVB = reinterpret_cast<uint32_t*>(take(4 * m));
     b.ttByRank = reinterpret_cast<uint32_t*>(take(4 * m));
+    b.rectIdx = reinterpret_cast<uint2*>(take(8 * m));
     b.rectByRank = reinterpret_cast<uint2*>(take(8 * m));
     const size_t blocksDup = (m + kDupTile - 1) / kDupTile;
     b.blockPairs = reinterpret_cast<uint32_t*>(take(4 * blocksDup));

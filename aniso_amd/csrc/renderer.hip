@@ -136,7 +136,7 @@ Engine::~Engine() {
 void Engine::reserveScratch(size_t n, size_t pairCap) {
     const size_t m = std::max<size_t>(n, 1);
     depthKey.reserve(m); sortKA.reserve(m); sortVA.reserve(m); sortKB.reserve(m); sortVB.reserve(m);
-    ttByRank.reserve(m); rectByRank.reserve(m);
+    ttByRank.reserve(m); rectIdx.reserve(m); rectByRank.reserve(m);
     const size_t blocksDepth = (n + kSortBlock * kDepthItems - 1) / (kSortBlock * kDepthItems);
     const size_t blocksDup = (n + kDupTile - 1) / kDupTile;
     const size_t blocksPair = (pairCap + kSortBlock * kTileItems - 1) / (kSortBlock * kTileItems);
@@ -159,16 +159,17 @@ void enqueueCountKernels(const FrameBuffers& b, const SceneDev& s, const CamPara
         const uint32_t grid = (std::max<uint32_t>(std::max<uint32_t>(n, tiles), 1) + 255) / 256;
         if (s.raw)
             k_preprocess<false, true><<<grid, 256, 0, stream>>>(static_cast<int>(n), s.pos, s.scale, s.rot, s.opacity,
-                                                                s.sh, cp, b.rec, b.tilesTouched, b.depthKey, b.ctl,
+                                                                s.sh, cp, b.rec, b.tilesTouched, b.depthKey, b.rectIdx, b.ctl,
                                                                 b.ranges, tiles, S, s.act);
         else if (s.shPlanar)
             k_preprocess<true, false><<<grid, 256, 0, stream>>>(static_cast<int>(n), s.pos, s.scale, s.rot, s.opacity,
-                                                                s.sh, cp, b.rec, b.tilesTouched, b.depthKey, b.ctl,
+                                                                s.sh, cp, b.rec, b.tilesTouched, b.depthKey, b.rectIdx, b.ctl,
                                                                 b.ranges, tiles, S, s.act);
         else
             k_preprocess<false, false><<<grid, 256, 0, stream>>>(static_cast<int>(n), s.pos, s.scale, s.rot,
                                                                  s.opacity, s.sh, cp, b.rec, b.tilesTouched,
-                                                                 b.depthKey, b.ctl, b.ranges, tiles, S, s.act);
+                                                                 b.depthKey, b.rectIdx, b.ctl, b.ranges, tiles, S,
+                                                                 s.act);
     }
     if (n > 0) {
         // Depth sort: four stable 8-bit passes over all Gaussians (culled ones carry the largest
@@ -189,7 +190,7 @@ void enqueueCountKernels(const FrameBuffers& b, const SceneDev& s, const CamPara
             vin = vout[pass];
         }
         const uint32_t blocksDup = (n + kDupTile - 1) / kDupTile;
-        k_pairTotals<<<blocksDup, kDupBlock, 0, stream>>>(b.order, b.rec, n, b.ttByRank, b.rectByRank, b.blockPairs,
+        k_pairTotals<<<blocksDup, kDupBlock, 0, stream>>>(b.order, b.rectIdx, n, b.ttByRank, b.rectByRank, b.blockPairs,
                                                           b.blockVisible, 0, 0, S);
         k_scanPairs<<<1, kDupBlock, 0, stream>>>(b.blockPairs, b.blockVisible, blocksDup,
                                                  static_cast<uint32_t>(b.pairCap), b.ctl, 0, mirror, seq);
@@ -293,6 +294,7 @@ FrameBuffers Engine::buffers(FrameState& f) {
     b.sortKB = sortKB.p;
     b.sortVB = sortVB.p;
     b.ttByRank = ttByRank.p;
+    b.rectIdx = rectIdx.p;
     b.rectByRank = rectByRank.p;
     b.hist = hist.p;
     b.rowTotal = rowTotal.p;

@@ -31,7 +31,8 @@ namespace ans {
 [[maybe_unused]] static __global__ __launch_bounds__(256) void k_preprocessViews(
     int n, int views, const float* __restrict__ pos, const float* __restrict__ scale, const float4* __restrict__ rot,
     const float* __restrict__ opacity, const float* __restrict__ sh, int shCoeffs, const CamParams* __restrict__ cams,
-    SplatRec* __restrict__ rec, uint32_t* __restrict__ tilesTouched, uint32_t* __restrict__ depthKey) {
+    SplatRec* __restrict__ rec, uint32_t* __restrict__ tilesTouched, uint32_t* __restrict__ depthKey,
+    uint2* __restrict__ rect) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const F3 m = reinterpret_cast<const F3*>(pos)[i];
@@ -52,10 +53,10 @@ namespace ans {
         }
         const size_t o = static_cast<size_t>(v) * n + i;
         if (!cameraTransform(m, cams[v], g) || !finishProjection<false, false>(m, op, shr, 1, cams[v], g)) {
-            storeCulled(o, rec, tilesTouched, depthKey);
+            storeCulled(o, rec, tilesTouched, depthKey, rect);
             continue;
         }
-        storeSplat(g, o, rec, tilesTouched, depthKey);
+        storeSplat(g, o, rec, tilesTouched, depthKey, rect);
     }
 }
 
@@ -65,7 +66,8 @@ namespace ans {
 // (k_preprocess<false, true>), with the tile counts written for the adjoint's slot numbering.
 [[maybe_unused]] static __global__ __launch_bounds__(256) void k_preprocessViewsRaw(
     int n, int views, const DiffParamsDev* __restrict__ prm, const CamParams* __restrict__ cams,
-    SplatRec* __restrict__ rec, uint32_t* __restrict__ tilesTouched, uint32_t* __restrict__ depthKey) {
+    SplatRec* __restrict__ rec, uint32_t* __restrict__ tilesTouched, uint32_t* __restrict__ depthKey,
+    uint2* __restrict__ rect) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const DiffParamsDev P = *prm;
@@ -81,10 +83,10 @@ namespace ans {
     for (int v = 0; v < views; ++v) {
         const size_t o = static_cast<size_t>(v) * n + i;
         if (!cameraTransform(m, cams[v], g) || !finishProjection<false, true>(m, op, shr, 1, cams[v], g)) {
-            storeCulled(o, rec, tilesTouched, depthKey);
+            storeCulled(o, rec, tilesTouched, depthKey, rect);
             continue;
         }
-        storeSplat(g, o, rec, tilesTouched, depthKey);
+        storeSplat(g, o, rec, tilesTouched, depthKey, rect);
     }
 }
 
