@@ -486,14 +486,15 @@ __host__ __device__ inline void gaussAdjoint(const GaussProj& g, const CamParams
     if (threadIdx.x == 0) blockSum[blockIdx.x] = total;
 }
 
-// One workgroup per view: exclusive scan of the per-block sums in place.
-[[maybe_unused]] static __global__ __launch_bounds__(256) void k_scanBlockSums(uint32_t* __restrict__ blockSum, uint32_t blocks,
+// One workgroup per view (kScanBlock wide, one round of loads up to 4096 blocks): exclusive scan
+// of the per-block sums in place.
+[[maybe_unused]] static __global__ __launch_bounds__(kScanBlock) void k_scanBlockSums(uint32_t* __restrict__ blockSum, uint32_t blocks,
                                                               uint32_t blockStride) {
-    __shared__ uint32_t sTmp[256 / kWave];
+    __shared__ uint32_t sTmp[kScanBlock / kWave];
     blockSum += static_cast<size_t>(blockIdx.y) * blockStride;
-    constexpr int kPer = 8;
+    constexpr int kPer = 4;
     uint32_t carry = 0;
-    for (uint32_t base = 0; base < blocks; base += 256 * kPer) {
+    for (uint32_t base = 0; base < blocks; base += kScanBlock * kPer) {
         const uint32_t first = base + threadIdx.x * kPer;
         uint32_t v[kPer], sum = 0;
 #pragma unroll
@@ -502,7 +503,7 @@ __host__ __device__ inline void gaussAdjoint(const GaussProj& g, const CamParams
             sum += v[k];
         }
         uint32_t total;
-        uint32_t run = carry + blockExclusiveScan<256>(sum, sTmp, &total);
+        uint32_t run = carry + blockExclusiveScan<kScanBlock>(sum, sTmp, &total);
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             if (first + k < blocks) blockSum[first + k] = run;

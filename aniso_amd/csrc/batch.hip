@@ -200,7 +200,7 @@ const uint32_t* enqueueBatch(ViewBatch& B, const SceneDev& s, const Layout& L, h
         const dim3 gDup(L.blocksDup, V);
         k_pairTotals<<<gDup, kDupBlock, 0, st>>>(B.order.p, B.rectIdx.p, n, B.ttByRank.p, B.rectByRank.p, B.blockPairs.p,
                                                  B.blockVisible.p, n, L.blocksDup, nullptr);
-        k_scanPairs<<<dim3(1, V), kDupBlock, 0, st>>>(B.blockPairs.p, B.blockVisible.p, L.blocksDup, cap, ctl,
+        k_scanPairs<<<dim3(1, V), kScanBlock, 0, st>>>(B.blockPairs.p, B.blockVisible.p, L.blocksDup, cap, ctl,
                                                       L.blocksDup, L.diff ? B.mirrorDev : nullptr, 1u);
         k_expandPairs<<<gDup, kDupBlock, 0, st>>>(B.order.p, B.ttByRank.p, B.rectByRank.p, n, 0, B.blockPairs.p,
                                                   B.tileA.p, B.gaussA.p, ctl, n, L.blocksDup, cap, tilesX);

@@ -49,7 +49,7 @@ void Engine::enqueueBackward(FrameState& f, const SceneDev& s, const float* grad
     blockPairs.reserve(grid);
     // Pair slots in Gaussian index order.
     k_slotBlockSums<<<grid, 256, 0, stream>>>(n, f.tilesTouched.p, blockPairs.p, 0, 0);
-    k_scanBlockSums<<<1, 256, 0, stream>>>(blockPairs.p, grid, 0);
+    k_scanBlockSums<<<1, kScanBlock, 0, stream>>>(blockPairs.p, grid, 0);
     k_slotStart<<<grid, 256, 0, stream>>>(n, f.tilesTouched.p, blockPairs.p, slotStart.p, 0, 0);
     if (f.opts.tile_w == 8) launchBlendBackward<8, 8>(*this, f, gradImage);
     else if (f.opts.tile_w == 32) launchBlendBackward<32, 16>(*this, f, gradImage);

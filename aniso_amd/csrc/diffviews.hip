@@ -60,7 +60,7 @@ void enqueueBackwardViews(DiffViewsFrame& F, DiffViewsScratch& S, hipStream_t st
     const uint32_t n = L.n, V = static_cast<uint32_t>(L.views);
     const uint32_t blocks = (n + 255) / 256;
     k_slotBlockSums<<<dim3(blocks, V), 256, 0, st>>>(n, B.tilesTouched.p, S.blockSums.p, n, blocks);
-    k_scanBlockSums<<<dim3(1, V), 256, 0, st>>>(S.blockSums.p, blocks, blocks);
+    k_scanBlockSums<<<dim3(1, V), kScanBlock, 0, st>>>(S.blockSums.p, blocks, blocks);
     k_slotStart<<<dim3(blocks, V), 256, 0, st>>>(n, B.tilesTouched.p, S.blockSums.p, S.slotStart.p, n, blocks);
     if (L.opts.tile_w == 8) launchBlendBackwardViews<8, 8>(F, S, T, st);
     else if (L.opts.tile_w == 32) launchBlendBackwardViews<32, 16>(F, S, T, st);

@@ -389,23 +389,26 @@ constexpr uint32_t kBigPairs = 32;
     }
 }
 
+constexpr int kScanBlock = 1024;  // k_scanPairs: one pass covers 4096 expansion blocks (1M ranks)
+
 // One workgroup: exclusive scan of the per-block pair counts; publishes the pair total (the sum
 // gpu_render.cu:324-329 reads back to the host) and the visible count, and flags a pair total
 // above the pair buffers' capacity; with mirror, also to pinned host memory under number seq.
-[[maybe_unused]] static __global__ __launch_bounds__(kDupBlock) void k_scanPairs(uint32_t* __restrict__ blockPairs,
+// The workgroup is wide so that a frame of up to 1M Gaussians takes one round of loads.
+[[maybe_unused]] static __global__ __launch_bounds__(kScanBlock) void k_scanPairs(uint32_t* __restrict__ blockPairs,
                                                                 uint32_t* __restrict__ blockVisible,
                                                                 uint32_t blocks, uint32_t cap,
                                                                 FrameCtl* __restrict__ ctl, uint32_t blockStride,
                                                                 CountMirror* __restrict__ mirror, uint32_t seq) {
-    __shared__ uint32_t sTmp[kDupBlock / kWave];
+    __shared__ uint32_t sTmp[kScanBlock / kWave];
     blockPairs += static_cast<size_t>(blockIdx.y) * blockStride;
     blockVisible += static_cast<size_t>(blockIdx.y) * blockStride;
     ctl += blockIdx.y;
     if (mirror) mirror += blockIdx.y;
-    constexpr int kPer = 8;
+    constexpr int kPer = 4;
     uint64_t carry = 0;
     uint32_t visible = 0;
-    for (uint32_t base = 0; base < blocks; base += kDupBlock * kPer) {
+    for (uint32_t base = 0; base < blocks; base += kScanBlock * kPer) {
         const uint32_t first = base + threadIdx.x * kPer;
         uint32_t v[kPer];
         uint32_t sum = 0, vis = 0;
@@ -417,8 +420,8 @@ constexpr uint32_t kBigPairs = 32;
             sum += v[k];
         }
         uint32_t total, visTotal;
-        uint32_t run = static_cast<uint32_t>(carry) + blockExclusiveScan<kDupBlock>(sum, sTmp, &total);
-        blockExclusiveScan<kDupBlock>(vis, sTmp, &visTotal);
+        uint32_t run = static_cast<uint32_t>(carry) + blockExclusiveScan<kScanBlock>(sum, sTmp, &total);
+        blockExclusiveScan<kScanBlock>(vis, sTmp, &visTotal);
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             if (first + k < blocks) blockPairs[first + k] = run;
