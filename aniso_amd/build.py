@@ -20,7 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libaniso_amd.so")
-SOURCES = ["renderer.hip", "diff.hip", "batch.hip", "diffviews.hip", "noalloc.hip", "ingest.hip"]
+SOURCES = ["renderer.hip", "diff.hip", "batch.hip", "diffviews.hip", "noalloc.hip", "ingest.hip", "io.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 

@@ -39,7 +39,7 @@ def main():
     ap.add_argument("--skip-extra", action="store_true", help="only the headline forward workload")
     ap.add_argument("--only", default="",
                     help="comma-separated extra legs to run (fwd_bwd, clustered, views, views_fwd_bwd, path, "
-                         "ingest); default: all")
+                         "ingest, png); default: all")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a plain 1 GiB device copy (the reference kernel for HBM counter scaling)")
     args = ap.parse_args()
@@ -141,14 +141,20 @@ def main():
         extra["path_stream"] = bench_path(raw, 128)
     if want("ingest"):
         extra["ply_ingest"] = bench_ingest(raw)
+    png_frame = None
+    if want("png"):
+        extra["png_1080p"], png_frame = bench_png(r)
 
     cpu = None
     if not args.skip_cpu:
-        cpu = cpu_baseline(raw, cam, args.cpu_seconds)
+        cpu = cpu_baseline(raw, cam, args.cpu_seconds, png_frame)
 
     if cpu and "ply_loader" in cpu and "ply_ingest" in extra:
         extra["ply_ingest"]["speedup_vs_reference_loader"] = round(
             cpu["ply_loader"]["value"] / extra["ply_ingest"]["load_ply_ms"], 2)
+    if cpu and "png_writer" in cpu and "png_1080p" in extra:
+        extra["png_1080p"]["speedup_vs_reference_writer"] = round(
+            cpu["png_writer"]["value"] / extra["png_1080p"]["device_to_file_ms"], 2)
 
     out = {
         "metric": METRIC,
@@ -499,6 +505,73 @@ def bench_ingest(raw):
             "includes": "file read (page cache) or bytes in host memory -> renderer ready on the device"}
 
 
+def bench_png(r):
+    """PNG file images of a rendered 1920x1080 frame of the headline scene (16x16 tiles), in the
+    reference writer's layout (image.cpp:41-85): encoded on the device from the frame in HBM
+    (ans_png_encode_device: kernel time from HIP events, best of 20), then with the copy of the file
+    image to pinned host memory, against the host encoder (ans_png_encode) from the frame in host
+    memory. The reference writer's time on the same frame is part of the CPU baseline
+    (cpu_baseline.png_writer). Returns (stats, the frame in host memory)."""
+    import torch
+
+    from aniso_amd import files, scenes
+
+    cam = scenes.pinhole_camera(1920, 1080)
+    frame = torch.empty(1080 * 1920 * 3, dtype=torch.uint8, device="cuda")
+    r.render_views([cam], [frame])
+    torch.cuda.synchronize()
+    img = frame.view(1080, 1920, 3)
+    host_frame = img.cpu().numpy()
+    out = torch.empty(8 << 20, dtype=torch.uint8, device="cuda")
+    work = torch.empty(1 << 20, dtype=torch.uint8, device="cuda")
+    pinned = torch.empty(8 << 20, dtype=torch.uint8, pin_memory=True)
+    enc = files.png_device(img, out, work)
+    size = enc.numel()
+    torch.cuda.synchronize()
+    assert enc.cpu().numpy().tobytes() == files.png_bytes(host_frame)
+    import tempfile
+
+    kernel, full, to_file = [], [], []
+    s = torch.cuda.current_stream()
+    fd, path = tempfile.mkstemp(suffix=".png")
+    os.close(fd)
+    view = pinned[:size].numpy()
+    try:
+        for _ in range(20):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(s)
+            files.png_device(img, out, work)
+            e1.record(s)
+            torch.cuda.synchronize()
+            kernel.append(e0.elapsed_time(e1))
+            t0 = time.perf_counter()
+            files.png_device(img, out, work)
+            pinned[:size].copy_(out[:size], non_blocking=True)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            with open(path, "wb") as f:
+                f.write(view)
+            t2 = time.perf_counter()
+            full.append((t1 - t0) * 1e3)
+            to_file.append((t2 - t0) * 1e3)
+    finally:
+        os.remove(path)
+    host = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        files.png_bytes(host_frame)
+        host.append((time.perf_counter() - t0) * 1e3)
+    k = min(kernel)
+    return ({"width": 1920, "height": 1080, "file_bytes": size, "device_encode_ms": round(k, 4),
+             "device_encode_GBs": round(size / (k * 1e-3) / 1e9, 1),
+             "device_to_host_file_ms": round(min(full), 4), "device_to_file_ms": round(min(to_file), 4),
+             "host_encode_ms": round(min(host), 3),
+             "includes": "device: frame in HBM -> PNG file image in HBM (device_to_host_file_ms: and copied to pinned "
+                         "host memory; device_to_file_ms: and written to a file in the page cache, as the reference "
+                         "writer does); host: frame in host memory -> file image in host memory"},
+            host_frame)
+
+
 def bench_fwd_bwd(raw, cam, steps, warmup):
     """One differentiable training step on the headline scene: forward (unclamped float image), L1
     against a target rendered after perturbing means by N(0, 0.01^2) and SH DC by N(0, 0.05^2),
@@ -621,7 +694,7 @@ def cpu_model():
     return None
 
 
-def cpu_baseline(raw, cam, seconds):
+def cpu_baseline(raw, cam, seconds, png_frame=None):
     """The reference CPU renderer, compiled from its own sources (oracle/_ref), on the same
     scene and camera, all host threads (std::thread::hardware_concurrency, render.cpp:108).
     Falls back to the oracle's restatement in reference mode when the compiled library is
@@ -661,6 +734,17 @@ def cpu_baseline(raw, cam, seconds):
             os.remove(path)
         out["ply_loader"] = {"value": round(ms, 2), "unit": "ms/file", "cores": 1, "kind": "reference",
                              "sample": f"aniso::loadPly of the {raw.n}-Gaussian SH3 PLY, best of 2"}
+        if png_frame is not None:
+            # The reference's PNG writer (aniso::writePng, one thread) on the frame of the png leg,
+            # into a temporary file (page cache).
+            fd, path = tempfile.mkstemp(suffix=".png")
+            os.close(fd)
+            try:
+                ms = min(oracle.ref_write_png(path, png_frame) for _ in range(3))
+            finally:
+                os.remove(path)
+            out["png_writer"] = {"value": round(ms, 2), "unit": "ms/file", "cores": 1, "kind": "reference",
+                                 "sample": "aniso::writePng of the png leg's 1920x1080 frame, best of 3"}
     return out
 
 

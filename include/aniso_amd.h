@@ -331,6 +331,38 @@ int ans_diff_views_dump(ans_diff* d, int64_t frame, int32_t view, uint32_t* valu
  * call waits for them. */
 int ans_ply_write(const ans_params* params, void* out, size_t capacity, size_t* written, void* stream);
 
+/* ---- COLMAP cameras and PNG frames (the reference CLI's files around the renderer) ---------- */
+
+/* COLMAP's binary sparse model, sparse_dir/cameras.bin and sparse_dir/images.bin, as one camera per
+ * registered image sorted by image name: aniso::loadColmapCameras (include/aniso/camera.h:44,
+ * src/camera.cpp:59-140). SIMPLE_PINHOLE and PINHOLE only; other models, truncated files, unknown
+ * models and images naming a missing camera fail with ANS_ERR_ARGUMENT and the reference's
+ * message. *count = the number of images; cameras NULL: nothing else is written. Otherwise
+ * cameras[capacity] receives them (ANS_ERR_CAPACITY if capacity < *count) and, unless names is
+ * NULL, names[i * name_capacity] each NUL-terminated name. */
+int ans_colmap_load(const char* sparse_dir, ans_camera* cameras, char* names, size_t name_capacity, int capacity,
+                    int* count);
+
+/* PNG file image of an RGB8 frame (rows top to bottom) in host memory, byte for byte the file
+ * aniso::writePng writes (include/aniso/image.h:22, src/image.cpp:41-85): stored deflate blocks,
+ * no compression. out NULL: *written = the file size only. */
+int ans_png_encode(const uint8_t* rgb, int width, int height, void* out, size_t capacity, size_t* written);
+
+/* aniso::writePng: the same bytes written to path. */
+int ans_png_write(const char* path, const uint8_t* rgb, int width, int height);
+
+/* Sizes for ans_png_encode_device: the PNG file and the device workspace it needs. */
+int ans_png_device_sizes(int width, int height, size_t* file_bytes, size_t* workspace_bytes);
+
+/* The PNG file image of a DEVICE RGB8 frame into DEVICE memory out[capacity], the same bytes as
+ * ans_png_encode, computed by kernels enqueued on `stream` (scanline copy into the stored blocks,
+ * Adler-32 and CRC-32 as parallel reductions); returns without waiting. workspace: device memory
+ * of the size ans_png_device_sizes gives, used until the kernels end. The first call on a device
+ * uploads a 5 KB constant table (a synchronous copy); later calls only enqueue kernels.
+ * No counterpart in the reference, which encodes on the host after copying the frame back. */
+int ans_png_encode_device(const void* rgb, int width, int height, void* out, size_t capacity, void* workspace,
+                          size_t workspace_bytes, void* stream);
+
 /* Host activation used by ans_renderer_create_raw, exposed for parity tests. */
 void ans_activate(int64_t n, const float* log_scale_xyz, const float* opacity_logit, const float* quat_wxyz,
                   float* scale_xyz, float* opacity, float* rot_wxyz);

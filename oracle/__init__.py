@@ -98,6 +98,12 @@ def ref_lib():
         R.ref_load_ply.argtypes = [ctypes.c_char_p, P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_double)]
         R.ref_scene_arrays.argtypes = [P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int), P, P, P, P, P]
         R.ref_scene_free.argtypes = [P]
+        R.ref_write_png.restype = ctypes.c_int
+        R.ref_write_png.argtypes = [ctypes.c_char_p, P, ctypes.c_int, ctypes.c_int, P, ctypes.c_size_t,
+                                    ctypes.POINTER(ctypes.c_double)]
+        R.ref_load_colmap.restype = ctypes.c_int
+        R.ref_load_colmap.argtypes = [ctypes.c_char_p, ctypes.POINTER(_Camera), P, ctypes.c_size_t, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_int), P, ctypes.c_size_t]
         _ref = R
     return _ref
 
@@ -272,6 +278,43 @@ def ref_load_ply(path):
         return out, ms.value
     finally:
         R.ref_scene_free(h)
+
+
+def ref_write_png(path, rgb: np.ndarray) -> float:
+    """aniso::writePng of rgb [H, W, 3] uint8 to path with the compiled reference writer; returns
+    its wall time in ms. Raises RuntimeError with the writer's message."""
+    rgb = np.ascontiguousarray(rgb, dtype=np.uint8)
+    err = ctypes.create_string_buffer(1024)
+    ms = ctypes.c_double()
+    h, w = rgb.shape[:2]
+    if ref_lib().ref_write_png(os.fsencode(path), _ptr(rgb), w, h, err, len(err), ctypes.byref(ms)) != 0:
+        raise RuntimeError(err.value.decode(errors="replace"))
+    return ms.value
+
+
+def ref_load_colmap(sparse_dir):
+    """aniso::loadColmapCameras with the compiled reference reader: (names, cameras) with cameras
+    as dicts of R [3, 3], t [3], fx, fy, cx, cy, width, height. Raises RefPlyError with the
+    reader's message (the reference reuses PlyError for bad camera files)."""
+    R = ref_lib()
+    err = ctypes.create_string_buffer(1024)
+    count = ctypes.c_int()
+    if R.ref_load_colmap(os.fsencode(sparse_dir), None, None, 0, 0, ctypes.byref(count), err, len(err)) != 0:
+        raise RefPlyError(err.value.decode(errors="replace"))
+    n = count.value
+    cams = (_Camera * max(n, 1))()
+    cap = 4096
+    names = ctypes.create_string_buffer(cap * max(n, 1))
+    if R.ref_load_colmap(os.fsencode(sparse_dir), cams, names, cap, n, ctypes.byref(count), err, len(err)) != 0:
+        raise RefPlyError(err.value.decode(errors="replace"))
+    out_names, out = [], []
+    for i in range(n):
+        raw = names.raw[i * cap:(i + 1) * cap]
+        out_names.append(raw[:raw.index(b"\0")].decode())
+        c = cams[i]
+        out.append(dict(R=np.array(c.R[:], np.float64).reshape(3, 3), t=np.array(c.t[:], np.float64), fx=c.fx,
+                        fy=c.fy, cx=c.cx, cy=c.cy, width=c.width, height=c.height))
+    return out_names, out
 
 
 # --- FP64 gradients (grad.cpp) ------------------------------------------------------------------

@@ -1,8 +1,11 @@
-// Drives the reference CPU renderer and PLY loader, compiled from their own sources
-// (src/render.cpp, src/sh.cpp, src/camera.cpp, src/ply_loader.cpp), through C entry points the
+// Drives the reference CPU renderer, PLY loader, COLMAP reader and PNG writer, compiled from their
+// own sources (src/render.cpp, src/sh.cpp, src/camera.cpp, src/ply_loader.cpp, src/image.cpp),
+// through C entry points the
 // tests can call. TEST INFRASTRUCTURE ONLY: the
 // library built from this file is the compiled reference, used to validate the oracle's
 // restatement and as the CPU baseline; the product path never loads it.
+#include "aniso/camera.h"
+#include "aniso/image.h"
 #include "aniso/ply_loader.h"
 #include "aniso/render.h"
 
@@ -104,5 +107,55 @@ void ref_scene_arrays(const void* scene, int64_t* n, int* sh_degree, float* pos,
 }
 
 void ref_scene_free(void* scene) { delete static_cast<aniso::Scene*>(scene); }
+
+// aniso::writePng (image.h:22) of rgb[height][width][3]; *ms = its wall time. Returns 0, or -1
+// with the message in err[errCap].
+int ref_write_png(const char* path, const uint8_t* rgb, int width, int height, char* err, size_t errCap,
+                  double* ms) {
+    try {
+        aniso::Image img(width, height);
+        std::memcpy(img.rgb.data(), rgb, img.rgb.size());
+        const auto t0 = std::chrono::steady_clock::now();
+        aniso::writePng(path, img);
+        if (ms) *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        return 0;
+    } catch (const std::exception& e) {
+        if (err && errCap) {
+            std::strncpy(err, e.what(), errCap - 1);
+            err[errCap - 1] = 0;
+        }
+        return -1;
+    }
+}
+
+// aniso::loadColmapCameras (camera.h:44): *count cameras; if cams is not NULL (room for
+// capacity), each as ref_camera plus its NUL-terminated name at names[i * nameCap]. Returns 0,
+// or -1 with the message in err[errCap].
+int ref_load_colmap(const char* dir, ref_camera* cams, char* names, size_t nameCap, int capacity, int* count,
+                    char* err, size_t errCap) {
+    try {
+        const auto v = aniso::loadColmapCameras(dir);
+        *count = static_cast<int>(v.size());
+        if (!cams) return 0;
+        for (size_t i = 0; i < v.size() && static_cast<int>(i) < capacity; ++i) {
+            const aniso::Camera& c = v[i];
+            for (int r = 0; r < 3; ++r) {
+                for (int k = 0; k < 3; ++k) cams[i].R[3 * r + k] = c.R[r][k];
+                cams[i].t[r] = c.t[r];
+            }
+            cams[i].fx = c.fx, cams[i].fy = c.fy, cams[i].cx = c.cx, cams[i].cy = c.cy;
+            cams[i].width = c.width, cams[i].height = c.height;
+            std::strncpy(names + i * nameCap, c.imageName.c_str(), nameCap - 1);
+            names[i * nameCap + nameCap - 1] = 0;
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        if (err && errCap) {
+            std::strncpy(err, e.what(), errCap - 1);
+            err[errCap - 1] = 0;
+        }
+        return -1;
+    }
+}
 
 }  // extern "C"
