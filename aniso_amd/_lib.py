@@ -109,6 +109,8 @@ SIGNATURES = {
     "ans_diff_forward_noalloc": (ctypes.c_int, [ctypes.POINTER(Params_), ctypes.POINTER(Camera_),
                                                 ctypes.POINTER(Options_), P, P, ctypes.POINTER(Stats_)]),
     "ans_ply_write": (ctypes.c_int, [ctypes.POINTER(Params_), P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), P]),
+    "ans_render_autotune": (ctypes.c_int, [P, ctypes.POINTER(Camera_), ctypes.POINTER(Options_), P, ctypes.c_int32,
+                                           ctypes.POINTER(Options_), ctypes.POINTER(ctypes.c_double)]),
     "ans_colmap_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(Camera_), P, ctypes.c_size_t, ctypes.c_int,
                                        ctypes.POINTER(ctypes.c_int)]),
     "ans_png_encode": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, P, ctypes.c_size_t,

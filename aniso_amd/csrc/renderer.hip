@@ -709,6 +709,54 @@ int ans_render_wait(ans_renderer* r, ans_stats* stats) {
     });
 }
 
+int ans_render_autotune(ans_renderer* r, const ans_camera* cam, const ans_options* base, uint8_t* dev_rgb,
+                        int32_t repeats, ans_options* best, double* best_ms) {
+    return guarded([&] {
+        if (!r) throw std::invalid_argument("renderer is null");
+        if (!best) throw std::invalid_argument("best is null");
+        if (!dev_rgb) throw std::invalid_argument("dev_rgb is null");
+        checkCamera(cam);
+        const ans_options b = resolveOptions(base);
+        const int reps = std::max(1, repeats);
+        static const int kTiles[3][2] = {{8, 8}, {16, 16}, {32, 16}};
+        static const int kDepths[3] = {64, 128, 256};
+        // One frame rendered and waited for: its device time is the sum of its stamp-timed
+        // stages. A frame whose pairs overflowed is rendered again (the wait grew the buffers).
+        auto frameMs = [&](const ans_options& o) {
+            ans_stats st{};
+            for (int attempt = 0; attempt < 3; ++attempt) {
+                int rc = ans_render_async(r, cam, &o, dev_rgb, nullptr);
+                if (rc != ANS_OK) throw std::runtime_error(g_lastError);
+                rc = ans_render_wait(r, &st);
+                if (rc == ANS_OK) return st.project_ms + st.sort_ms + st.blend_ms;
+                if (rc != ANS_ERR_CAPACITY) throw std::runtime_error(g_lastError);
+            }
+            throw std::runtime_error("pair buffers kept overflowing during autotuning");
+        };
+        double bestMs = 1e30;
+        ans_options bestOpt = b;
+        for (const auto& t : kTiles) {
+            for (int d : kDepths) {
+                ans_options o = b;
+                o.tile_w = t[0];
+                o.tile_h = t[1];
+                o.batch_depth = d;
+                for (int k = 0; k < ans_renderer::kLanes; ++k) (void)frameMs(o);  // sizes every lane
+                double ms = 1e30;
+                for (int k = 0; k < reps; ++k) ms = std::min(ms, frameMs(o));
+                if (ms < bestMs) {
+                    bestMs = ms;
+                    bestOpt = o;
+                }
+            }
+        }
+        g_lastError.clear();
+        *best = bestOpt;
+        if (best_ms) *best_ms = bestMs;
+        return ANS_OK;
+    });
+}
+
 int ans_frame_counts(ans_renderer* r, int64_t* n, int64_t* visible, int64_t* pairs, int64_t* tiles) {
     return guarded([&] {
         if (!r) throw std::invalid_argument("renderer is null");

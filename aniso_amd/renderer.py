@@ -186,6 +186,17 @@ class GpuRenderer:
             for s_ in st:
                 stats.append(RenderStats(**{f.name: getattr(s_, f.name) for f in dataclasses.fields(RenderStats)}))
 
+    def autotune(self, camera: Camera, frame, base: RenderOptions | None = None, repeats: int = 3):
+        """(best RenderOptions, device ms per frame) over tile shapes x LDS batch depths for single
+        frames of `camera`, rendered into the device buffer `frame` (ans_render_autotune)."""
+        best = _lib.Options_()
+        ms = ctypes.c_double()
+        ptr = frame if isinstance(frame, int) else frame.data_ptr()
+        _lib.check(_lib.load().ans_render_autotune(self._h, ctypes.byref(camera_c(camera)),
+                                                   ctypes.byref((base or RenderOptions()).c()), ctypes.c_void_p(ptr),
+                                                   int(repeats), ctypes.byref(best), ctypes.byref(ms)))
+        return RenderOptions.from_c(best), ms.value
+
     def autotune_views(self, cameras, outputs, base: RenderOptions | None = None, repeats: int = 3):
         """(best RenderOptions, batch ms) over tile shapes x LDS batch depths (ans_views_autotune)."""
         k = len(cameras)

@@ -166,6 +166,14 @@ int ans_frame_dump(ans_renderer* r, uint32_t* splat_bits, int32_t* rect, uint64_
 int ans_render_views(ans_renderer* r, int32_t n_views, const ans_camera* cams, const ans_options* options,
                      uint8_t* const* dev_rgb, void* stream, ans_stats* stats);
 
+/* Tries every tile shape (8x8, 16x16, 32x16) and LDS batch depth (64, 128, 256) on single frames
+ * of this camera (sh_degree, splat_scale and flags from base), each rendered into the DEVICE
+ * buffer dev_rgb and waited for, and returns the fastest options in *best and their device time
+ * (stamp-timed stages, the least of `repeats` frames after warm-up frames) in *best_ms. No
+ * counterpart in the reference, whose tile size is fixed (gpu_render.cu:15). */
+int ans_render_autotune(ans_renderer* r, const ans_camera* cam, const ans_options* base, uint8_t* dev_rgb,
+                        int32_t repeats, ans_options* best, double* best_ms);
+
 /* Tries every tile shape (8x8, 16x16, 32x16) and LDS batch depth (64, 128, 256) on this batch
  * (sh_degree, splat_scale and flags from base) and returns the fastest options in *best and
  * their batch time (the least of `repeats` timed replays after a warm-up) in *best_ms. */
