@@ -303,7 +303,7 @@ def blend_roofline(workload, kernel_ms, b_blend):
         "timing": "device clock stamps at the blend's start and end, median of frames rendered one at a time "
                   "(the rocprof kernel-trace mean of the same launches is in the committed profile)",
         "valu_instructions_per_launch": int(inst) if inst else None,
-        "occupancy_waves_per_cu": profile_figure("k_blend", workload, "mean_occupancy_waves_per_cu"),
+        "occupancy_waves_per_simd": profile_figure("k_blend", workload, "mean_occupancy_waves_per_simd"),
         "lds_bank_conflict_cycles": profile_figure("k_blend", workload, "lds_bank_conflict_cycles"),
         "fragments_per_frame": int(frags) if frags else None,
         "fragments_per_s": round(frags / (kernel_ms * 1e-3), 1) if frags else None,
