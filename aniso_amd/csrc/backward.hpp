@@ -536,12 +536,68 @@ __device__ inline int rowOf(int e, float rcpK3) {
     return static_cast<int>((static_cast<float>(e) + 0.5f) * rcpK3);
 }
 
+// Record sums of one wavefront's 64 Gaussians (lane = Gaussian, slots [s0, s0 + tt)) over their
+// records [r0, r1), 64 records per chunk, one record per lane: the owner of each record comes
+// from head flags at the owners' first slots and a wavefront max-scan (as in the pair
+// expansion); a segmented inclusive scan over the lanes sums each owner's records within the
+// chunk, and the lane ending a segment adds it to the owner's accumulator in LDS. Every lane does
+// the same work whatever its Gaussian's record count. The chunk's records arrive in a0..a2 (this
+// lane's record c0 + lane, zero past r1), and the next chunk's are loaded while this one is summed.
+// Returns this lane's sums in gs. The order of additions is fixed, so results are deterministic.
+struct RecordScratch {
+    float acc[kWave * 9];
+    int head[kWave];
+};
+
+__device__ inline void sumRecords(const float4* __restrict__ rec, uint32_t r0, uint32_t r1, uint32_t s0,
+                                  uint32_t tt, float4 a0, float4 a1, float4 a2, RecordScratch& S, float gs[9]) {
+    const int lane = __lane_id();
+#pragma unroll
+    for (int c = 0; c < 9; ++c) S.acc[lane * 9 + c] = 0.0f;
+    int carry = -1;
+    for (uint32_t c0 = r0; c0 < r1; c0 += kWave) {
+        float x[9] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w, a2.x};
+        const uint32_t nx = c0 + kWave + lane;
+        if (nx < r1) {
+            const size_t r = static_cast<size_t>(nx) * 3;
+            a0 = rec[r];
+            a1 = rec[r + 1];
+            a2 = rec[r + 2];
+        }
+        waveSync();
+        S.head[lane] = -1;
+        waveSync();
+        if (tt > 0 && s0 >= c0 && s0 < c0 + kWave) S.head[s0 - c0] = lane;
+        waveSync();
+        const int owner = max(carry, waveInclusiveMax(S.head[lane]));
+        carry = __builtin_amdgcn_readlane(owner, kWave - 1);
+        const bool has = c0 + lane < r1;
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+            const int up = __shfl_up(owner, d, kWave);
+            const bool same = lane >= d && up == owner;
+#pragma unroll
+            for (int c = 0; c < 9; ++c) {
+                const float y = __shfl_up(x[c], d, kWave);
+                x[c] = same ? x[c] + y : x[c];
+            }
+        }
+        const int next = __shfl_down(owner, 1, kWave);
+        const bool end = has && (lane == kWave - 1 || c0 + lane + 1 >= r1 || next != owner);
+        if (end)
+#pragma unroll
+            for (int c = 0; c < 9; ++c) S.acc[owner * 9 + c] += x[c];
+    }
+    waveSync();
+#pragma unroll
+    for (int c = 0; c < 9; ++c) gs[c] = S.acc[lane * 9 + c];
+}
+
 // Per Gaussian over several views (the batched differentiable frames), one wavefront per 64
 // consecutive Gaussians (index order). For each view in turn (v = 0, 1, ...) the wavefront's pair records of
 // that view are one contiguous run [slotStart of its first Gaussian, end of its last) starting at
-// pair pairBase[v] of `partial` (pairBase null: 0); it streams them 64 at a time through LDS (each
-// lane loads one 48-byte record, the next chunk prefetched while the current one is summed) and
-// each lane sums its own Gaussian's records in slot order. The lane then projects its Gaussian
+// pair pairBase[v] of `partial` (pairBase null: 0); sumRecords adds them up per Gaussian, 64
+// records at a time (the next chunk prefetched while the current one is summed). The lane then projects its Gaussian
 // into the view again and applies the view-dependent part of the adjoint (gaussAdjointView): the
 // mean and SH gradients, dL/dSigma and dL/dopacity are added in view order into running sums, the
 // SH sums in LDS (12.5 KB per wavefront; each lane reads its SH from global memory, where its row
@@ -553,7 +609,7 @@ __device__ inline void gaussBackwardWave(int n, int views, const DiffParamsDev& 
                                          const uint32_t* __restrict__ tilesTouched,
                                          const uint32_t* __restrict__ slotStart,
                                          const uint64_t* __restrict__ pairBase, const float* __restrict__ partial) {
-    __shared__ float4 sRec[kWave][3];
+    __shared__ RecordScratch sRecSum;
     __shared__ float sGsh[kWave * kShRow];
     const int lane = threadIdx.x;
     const int g0 = blockIdx.x * kWave;
@@ -606,28 +662,8 @@ __device__ inline void gaussBackwardWave(int n, int views, const DiffParamsDev& 
         const uint32_t r0 = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(s0)));
         const uint32_t r1 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(s0 + tt), rows - 1));
         const float4* rec = recOf(v);
-        float gs[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        for (uint32_t c0 = r0; c0 < r1; c0 += kWave) {
-            waveSync();
-            sRec[lane][0] = a0;
-            sRec[lane][1] = a1;
-            sRec[lane][2] = a2;
-            const uint32_t nx = c0 + kWave + lane;
-            if (nx < r1) {
-                const size_t r = static_cast<size_t>(nx) * 3;
-                a0 = rec[r];
-                a1 = rec[r + 1];
-                a2 = rec[r + 2];
-            }
-            waveSync();
-            const uint32_t a = max(s0, c0), b = min(s0 + tt, c0 + kWave);
-            for (uint32_t k = a; k < b; ++k) {
-                const float4 x = sRec[k - c0][0], y = sRec[k - c0][1], z = sRec[k - c0][2];
-                gs[0] += x.x; gs[1] += x.y; gs[2] += x.z; gs[3] += x.w;
-                gs[4] += y.x; gs[5] += y.y; gs[6] += y.z; gs[7] += y.w;
-                gs[8] += z.x;
-            }
-        }
+        float gs[9];
+        sumRecords(rec, r0, r1, s0, tt, a0, a1, a2, sRecSum, gs);
         if (v + 1 < views) {
             // The next view's first chunk, in flight during this view's adjoint.
             n0 = n1 = n2 = zero;
@@ -682,9 +718,9 @@ __device__ inline void gaussBackwardWave(int n, int views, const DiffParamsDev& 
 }
 
 // Per Gaussian, one wavefront per 64 consecutive Gaussians (index order). The wavefront's pair
-// records are one contiguous run [slotStart of its first Gaussian, end of its last); it streams
-// them 64 at a time through LDS (each lane loads one 48-byte record, the next chunk prefetched
-// while the current one is summed) and each lane sums its own Gaussian's records in slot order.
+// records are one contiguous run [slotStart of its first Gaussian, end of its last); sumRecords
+// adds them up per Gaussian, 64 records at a time (the next chunk prefetched while the current
+// one is summed).
 // The SH of the 64 Gaussians is one contiguous block: it is loaded coalesced (every load issued
 // up front), transposed through LDS to one row per lane, and the SH gradients go back the same
 // way. gaussAdjoint then gives the gradients of means, log-scales, quaternions (w, x, y, z),
@@ -698,7 +734,7 @@ __device__ inline void gaussBackwardWave(int n, int views, const DiffParamsDev& 
     const float* __restrict__ partial, float* __restrict__ splatGrad, float* __restrict__ gMeans,
     float* __restrict__ gScales, float* __restrict__ gQuats, float* __restrict__ gLogit, float* __restrict__ gSh,
     const FrameCtl* __restrict__ ctl, bool act) {
-    __shared__ float4 sRec[kWave][3];
+    __shared__ RecordScratch sRecSum;
     __shared__ float sSh[kWave * kShRow];
     if (ctl->overflow) return;
     const int lane = threadIdx.x;
@@ -738,28 +774,8 @@ __device__ inline void gaussBackwardWave(int n, int views, const DiffParamsDev& 
         }
     }
 
-    float gs[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (uint32_t c0 = r0; c0 < r1; c0 += kWave) {
-        waveSync();
-        sRec[lane][0] = a0;
-        sRec[lane][1] = a1;
-        sRec[lane][2] = a2;
-        const uint32_t nx = c0 + kWave + lane;
-        if (nx < r1) {
-            const size_t r = static_cast<size_t>(nx) * 3;
-            a0 = rec[r];
-            a1 = rec[r + 1];
-            a2 = rec[r + 2];
-        }
-        waveSync();
-        const uint32_t a = max(s0, c0), b = min(s0 + tt, c0 + kWave);
-        for (uint32_t k = a; k < b; ++k) {
-            const float4 x = sRec[k - c0][0], y = sRec[k - c0][1], z = sRec[k - c0][2];
-            gs[0] += x.x; gs[1] += x.y; gs[2] += x.z; gs[3] += x.w;
-            gs[4] += y.x; gs[5] += y.y; gs[6] += y.z; gs[7] += y.w;
-            gs[8] += z.x;
-        }
-    }
+    float gs[9];
+    sumRecords(rec, r0, r1, s0, tt, a0, a1, a2, sRecSum, gs);
     waveSync();
     float* row = sSh + lane * kShRow;
     if (valid) {
