@@ -515,17 +515,22 @@ __device__ inline void writeRectPairs(uint32_t first, uint32_t g, uint32_t rectL
         gaussOut += po;
         if (tilesXView) tilesX = tilesXView[view];
     }
-    if (ctl->overflow) return;
     const int wave = threadIdx.x / kWave, lane = __lane_id();
     const uint32_t r = blockIdx.x * kDupTile + threadIdx.x;
+    // The ranks' counts, rects and indices and the block's first slot are loaded together with
+    // the overflow flag (all in bounds either way), so the flag costs no round trip of its own.
     uint32_t g = 0, cnt = 0;
     uint2 rp = make_uint2(0, 0);
     if (r < n) {
         cnt = ttByRank[r];
-        if (cnt) {
-            rp = rectByRank[r];
-            g = sortedIdx[r];
-        }
+        rp = rectByRank[r];
+        g = sortedIdx[r];
+    }
+    const uint32_t blockFirst = blockBase[blockIdx.x];
+    if (ctl->overflow) return;
+    if (!cnt) {
+        rp = make_uint2(0, 0);
+        g = 0;
     }
     const bool isBig = cnt > kBigPairs;
     const uint32_t cntS = isBig ? 0u : cnt;  // slots the head-flag walk writes
@@ -538,7 +543,7 @@ __device__ inline void writeRectPairs(uint32_t first, uint32_t g, uint32_t rectL
     const uint64_t bigLanes = __ballot(isBig);
     if (lane == 0) sWave[wave] = waveTotal;
     __syncthreads();
-    uint32_t out = blockBase[blockIdx.x];
+    uint32_t out = blockFirst;
     for (int w = 0; w < wave; ++w) out += sWave[w];
 
     const uint32_t x0 = rp.x & 0xffffu, y0 = rp.x >> 16;

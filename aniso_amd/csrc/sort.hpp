@@ -76,8 +76,14 @@ __device__ inline uint64_t matchDigit(uint32_t digit, int bits, bool valid) {
     return peers;
 }
 
-__device__ inline uint32_t itemCount(const FrameCtl* ctl, bool countFromCtl, uint32_t nHost, uint32_t cap) {
-    return countFromCtl ? min(ctl->pairs, cap) : nHost;
+// The number of items to sort, or -1 if this sort is skipped (the pair buffers overflowed). Only
+// a sort of pairs (countFromCtl) reads the control block: a sort with a host-given count runs
+// before anything of its frame can set the overflow flag, and a load it waited for would delay
+// every other load of the workgroup by a round trip.
+__device__ inline int64_t itemCount(const FrameCtl* ctl, bool countFromCtl, uint32_t nHost, uint32_t cap) {
+    if (!countFromCtl) return nHost;
+    if (ctl->overflow) return -1;
+    return min(ctl->pairs, cap);
 }
 
 // Per-block histogram of bits [shift, shift + bits) into hist[bin * stride + block], for the
@@ -97,10 +103,11 @@ __global__ __launch_bounds__(kSortBlock) void k_radixUpsweep(const uint32_t* __r
     __shared__ uint32_t h[kCopies * kRow];
     const uint32_t view = blockIdx.y;
     ctl += view;
-    if (ctl->overflow) return;
+    const int64_t count = itemCount(ctl, countFromCtl, nHost, cap);
+    if (count < 0) return;
+    const uint32_t n = static_cast<uint32_t>(count);
     keys += static_cast<size_t>(view) * itemStride;
     hist += static_cast<size_t>(view) * histStride;
-    const uint32_t n = itemCount(ctl, countFromCtl, nHost, cap);
     const uint32_t base = blockIdx.x * (kSortBlock * ITEMS);
     if (base >= n && blockIdx.x > 0) return;
     const int tid = threadIdx.x;
@@ -138,10 +145,11 @@ __global__ __launch_bounds__(kSortBlock) void k_radixUpsweep(const uint32_t* __r
     __shared__ uint32_t sTmp[kSortWaves];
     const uint32_t view = blockIdx.y;
     ctl += view;
-    if (ctl->overflow) return;
+    const int64_t count = itemCount(ctl, countFromCtl, 0, cap);
+    if (count < 0) return;
     hist += static_cast<size_t>(view) * histStride;
     rowTotal += view * kRadix;
-    if (countFromCtl) len = max(1u, min(len, (itemCount(ctl, true, 0, cap) + tileItems - 1) / tileItems));
+    if (countFromCtl) len = max(1u, min(len, (static_cast<uint32_t>(count) + tileItems - 1) / tileItems));
     constexpr int kPer = 8;
     uint32_t* row = hist + static_cast<size_t>(blockIdx.x) * stride;
     uint32_t carry = 0;
@@ -184,9 +192,20 @@ __global__ __launch_bounds__(kSortBlock) void k_radixDownsweep(
     __shared__ uint32_t sScan[kSortWaves];
 
     const uint32_t view = blockIdx.y;
+    const int tid = threadIdx.x;
+    const uint32_t mask = (1u << bits) - 1u;
+    // This thread's digit's total and row prefix, loaded first (before the count) so that their
+    // latency overlaps the count's and the key loads' instead of following them.
+    const bool live = tid <= static_cast<int>(mask);
+    uint32_t digitTotal = 0, rowPrefix = 0;
+    if (live) {
+        digitTotal = rowTotal[view * kRadix + tid];
+        rowPrefix = hist[static_cast<size_t>(view) * histStride + static_cast<size_t>(tid) * stride + blockIdx.x];
+    }
     ctl += view;
-    if (ctl->overflow) return;
-    const uint32_t n = itemCount(ctl, countFromCtl, nHost, cap);
+    const int64_t count = itemCount(ctl, countFromCtl, nHost, cap);
+    if (count < 0) return;
+    const uint32_t n = static_cast<uint32_t>(count);
     const uint32_t base = blockIdx.x * kTile;
     if (base >= n) return;
     {
@@ -195,15 +214,11 @@ __global__ __launch_bounds__(kSortBlock) void k_radixDownsweep(
         if (valsIn) valsIn += io;
         if (keysOut) keysOut += io;
         valsOut += io;
-        hist += static_cast<size_t>(view) * histStride;
-        rowTotal += view * kRadix;
     }
-    const int tid = threadIdx.x;
     const int lane = __lane_id();
     const int wave = tid / kWave;
     for (int i = tid; i < kSortWaves * kRadix; i += kSortBlock) (&sWaveHist[0][0])[i] = 0;
     __syncthreads();
-    const uint32_t mask = (1u << bits) - 1u;
 
     // Wave-striped order: item j of lane l in wave w is base + (w * ITEMS + j) * 64 + l, so
     // (wave, j, lane) order is input order.
@@ -245,10 +260,9 @@ __global__ __launch_bounds__(kSortBlock) void k_radixDownsweep(
     }
     uint32_t blockTotal, allTotal;
     const uint32_t binStart = blockExclusiveScan<kSortBlock>(run, sScan, &blockTotal);
-    const bool live = b <= static_cast<int>(mask);
-    const uint32_t digitStart = blockExclusiveScan<kSortBlock>(live ? rowTotal[b] : 0u, sScan, &allTotal);
+    const uint32_t digitStart = blockExclusiveScan<kSortBlock>(digitTotal, sScan, &allTotal);
     sBinStart[b] = binStart;
-    if (live) sDst[b] = digitStart + hist[static_cast<size_t>(b) * stride + blockIdx.x] - binStart;
+    if (live) sDst[b] = digitStart + rowPrefix - binStart;
     __syncthreads();
 
 #pragma unroll
