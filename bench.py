@@ -330,7 +330,8 @@ def kernel_table(workload):
 
 
 def bench_forward(workload, steps, warmup):
-    """Forward ms/frame of another synthetic workload, same protocol as the headline line."""
+    """Forward ms/frame of another synthetic workload, same protocol as the headline line, with
+    frames rendered one at a time beside it and the per-kernel HBM rates of its committed profile."""
     import torch
 
     from aniso_amd import _lib, scenes
@@ -352,10 +353,12 @@ def bench_forward(workload, steps, warmup):
         cst = _lib.Stats_()
         _lib.check(L.ans_render_wait(r._h, ctypes.byref(cst)))
         ms = (time.perf_counter() - t0) * 1e3 / steps
+        single = bench_single_frames(r, L, c_cam, c_opt, dev, warm=15, cold=5)
     return {"ms_per_frame": round(ms, 4), "gaussians": raw.n, "visible": cst.visible, "pairs": cst.tile_pairs,
             "width": cam.width, "height": cam.height, "steps": steps,
-            "stages_ms": {"project": round(cst.project_ms, 4), "sort": round(cst.sort_ms, 4),
-                          "blend": round(cst.blend_ms, 4)}}
+            "stages_ms_overlapped": {"project": round(cst.project_ms, 4), "sort": round(cst.sort_ms, 4),
+                                     "blend": round(cst.blend_ms, 4)},
+            "single_frame": single, "kernels": kernel_table(workload)}
 
 
 def bench_views(workload, steps, warmup):
