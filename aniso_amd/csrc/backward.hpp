@@ -599,9 +599,11 @@ __device__ inline void sumRecords(const float4* __restrict__ rec, uint32_t r0, u
 // pair pairBase[v] of `partial` (pairBase null: 0); sumRecords adds them up per Gaussian, 64
 // records at a time (the next chunk prefetched while the current one is summed). The lane then projects its Gaussian
 // into the view again and applies the view-dependent part of the adjoint (gaussAdjointView): the
-// mean and SH gradients, dL/dSigma and dL/dopacity are added in view order into running sums, the
-// SH sums in LDS (12.5 KB per wavefront; each lane reads its SH from global memory, where its row
-// stays in cache across views). View v + 1's first loads are issued while view v is processed.
+// mean and SH gradients, dL/dSigma and dL/dopacity are added in view order into running sums in
+// registers. The wavefront's 64 SH rows are loaded once, coalesced, into LDS (12.5 KB), where every
+// view's projection and adjoint read them (read from global memory per view, they missed the L2
+// and were fetched again for each view). View v + 1's first loads are issued while view v is
+// processed.
 // The view-independent part (gaussAdjointShape, linear in dL/dSigma and dL/dopacity) runs once on
 // the sums. The SH gradients go out through LDS so that global stores are coalesced. Gaussians
 // culled in every view get zero gradients. P.screenGrads (if set): dL/d(u, v) per view [view][n].
@@ -610,16 +612,38 @@ __device__ inline void gaussBackwardWave(int n, int views, const DiffParamsDev& 
                                          const uint32_t* __restrict__ slotStart,
                                          const uint64_t* __restrict__ pairBase, const float* __restrict__ partial) {
     __shared__ RecordScratch sRecSum;
-    __shared__ float sGsh[kWave * kShRow];
+    __shared__ float sSh[kWave * kShRow];
     const int lane = threadIdx.x;
     const int g0 = blockIdx.x * kWave;
     const int rows = min(kWave, n - g0);
     const int i = g0 + lane;
     const bool valid = lane < rows;
     const int K3 = P.shCoeffs * 3;  // at most 48
-    float* gshRow = sGsh + lane * kShRow;
-    const float* shRow = P.sh + static_cast<size_t>(valid ? i : g0) * K3;
-    for (int j = 0; j < K3; ++j) gshRow[j] = 0.0f;
+    const int total = rows * K3;
+    const float rcpK3 = 1.0f / static_cast<float>(K3);
+    {
+        // The 64 rows are one contiguous block: every load issued first, then one row per lane.
+        const float* src = P.sh + static_cast<size_t>(g0) * K3;
+        float v[48];
+#pragma unroll
+        for (int j = 0; j < 48; ++j) {
+            const int e = lane + j * kWave;
+            v[j] = e < total ? src[e] : 0.0f;
+        }
+#pragma unroll
+        for (int j = 0; j < 48; ++j) {
+            const int e = lane + j * kWave;
+            if (e < total) {
+                const int r = rowOf(e, rcpK3);
+                sSh[r * kShRow + (e - r * K3)] = v[j];
+            }
+        }
+    }
+    float* shRow = sSh + lane * kShRow;
+    float gsh[48];
+#pragma unroll
+    for (int j = 0; j < 48; ++j) gsh[j] = 0.0f;
+    waveSync();
     F3 m{0.0f, 0.0f, 0.0f};
     float logit = 0.0f;
     GaussProj g;
@@ -680,7 +704,7 @@ __device__ inline void gaussBackwardWave(int n, int views, const DiffParamsDev& 
         if (valid && P.screenGrads) P.screenGrads[vo + i] = make_float2(gs[0], gs[1]);
         if (tt > 0 && cameraTransform(m, cams[v], g) && finishProjection<false, true>(m, logit, shRow, 1, cams[v], g)) {
             float vm[3], vSig[3][3];
-            gaussAdjointView<true>(g, cams[v], gs, shRow, vm, vSig, gshRow);
+            gaussAdjointView<true>(g, cams[v], gs, shRow, vm, vSig, gsh);
             for (int j = 0; j < 3; ++j) gm[j] += vm[j];
             for (int r = 0; r < 3; ++r)
                 for (int c = 0; c < 3; ++c) gSig[r][c] += vSig[r][c];
@@ -703,16 +727,20 @@ __device__ inline void gaussBackwardWave(int n, int views, const DiffParamsDev& 
         reinterpret_cast<float4*>(P.gQuats)[i] = make_float4(gq[0], gq[1], gq[2], gq[3]);
         P.gOpacity[i] = gl;
     }
+    // The SH gradients go out through LDS (the SH rows are no longer needed) so that the global
+    // stores are coalesced.
+    waveSync();
+#pragma unroll
+    for (int j = 0; j < 48; ++j)
+        if (j < K3) shRow[j] = gsh[j];
     waveSync();
     float* dst = P.gSh + static_cast<size_t>(g0) * K3;
-    const int total = rows * K3;
-    const float rcpK3 = 1.0f / static_cast<float>(K3);
 #pragma unroll
     for (int j = 0; j < 48; ++j) {
         const int e = lane + j * kWave;
         if (e < total) {
             const int r = rowOf(e, rcpK3);
-            dst[e] = sGsh[r * kShRow + (e - r * K3)];
+            dst[e] = sSh[r * kShRow + (e - r * K3)];
         }
     }
 }
