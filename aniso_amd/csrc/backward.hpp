@@ -391,16 +391,20 @@ __host__ __device__ inline void gaussAdjointView(const GaussProj& g, const CamPa
     // SH colour, clamped at zero: d sh_k = basis_k d rgb; d dir through the basis.
     const float grgb[3] = {g.rgbRaw.x > 0.0f ? gs[6] : 0.0f, g.rgbRaw.y > 0.0f ? gs[7] : 0.0f,
                            g.rgbRaw.z > 0.0f ? gs[8] : 0.0f};
-    // Every coefficient is read before any gradient is written (gsh may alias shi): the reads do
-    // not wait on each other's stores.
-    float gbk[16];
+    // The direction gradient through the basis, coefficient by coefficient (each coefficient's
+    // three values are used as soon as they are read); every coefficient is read before any SH
+    // gradient is written below (gsh may alias shi).
+    float gdir[3] = {0, 0, 0};
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
-        gbk[k] = 0.0f;
-        if (k < g.shCount)
-            for (int c = 0; c < 3; ++c) gbk[k] += grgb[c] * shi[k * 3 + c];
+        if (k < g.shCount) {
+            float gbk = 0.0f;
+            for (int c = 0; c < 3; ++c) gbk += grgb[c] * shi[k * 3 + c];
+            float row[3];
+            shBasisGradRow(k, g.dir[0], g.dir[1], g.dir[2], row);
+            for (int j = 0; j < 3; ++j) gdir[j] += gbk * row[j];
+        }
     }
-    float gdir[3] = {0, 0, 0};
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
         if (k < g.shCount) {
@@ -408,9 +412,6 @@ __host__ __device__ inline void gaussAdjointView(const GaussProj& g, const CamPa
                 if constexpr (ACC) gsh[k * 3 + c] += grgb[c] * g.basis[k];
                 else gsh[k * 3 + c] = grgb[c] * g.basis[k];
             }
-            float row[3];
-            shBasisGradRow(k, g.dir[0], g.dir[1], g.dir[2], row);
-            for (int j = 0; j < 3; ++j) gdir[j] += gbk[k] * row[j];
         }
     }
     if constexpr (!ACC)
@@ -702,7 +703,8 @@ __device__ inline void gaussBackwardWave(int n, int views, const DiffParamsDev& 
             }
         }
         if (valid && P.screenGrads) P.screenGrads[vo + i] = make_float2(gs[0], gs[1]);
-        if (tt > 0 && cameraTransform(m, cams[v], g) && finishProjection<false, true>(m, logit, shRow, 1, cams[v], g)) {
+        if (tt > 0 && cameraTransform(m, cams[v], g) &&
+            finishProjection<false, true, false>(m, logit, shRow, 1, cams[v], g)) {
             float vm[3], vSig[3][3];
             gaussAdjointView<true>(g, cams[v], gs, shRow, vm, vSig, gsh);
             for (int j = 0; j < 3; ++j) gm[j] += vm[j];
@@ -809,9 +811,16 @@ __device__ inline void gaussBackwardWave(int n, int views, const DiffParamsDev& 
     if (valid) {
         for (int c = 0; c < 9; ++c) splatGrad[static_cast<size_t>(i) * 9 + c] = gs[c];
         GaussProj g;
-        if (tt > 0 && projectGaussian<false, true>(i, pos, logScale, quat, logit, row, 1, cam, g, act)) {
-            float gm[3], gls[3], gq[4], gl;
-            gaussAdjoint(g, cam, gs, row, gm, gls, gq, &gl, row);
+        if (tt > 0 && projectGaussian<false, true, false>(i, pos, logScale, quat, logit, row, 1, cam, g, act)) {
+            float gm[3], gls[3], gq[4], gl, gSigma[3][3];
+            const float opacity = g.opacity;
+            gaussAdjointView<false>(g, cam, gs, row, gm, gSigma, row);
+            // The rotation and scale matrices are formed again for the shape part, so that they
+            // need not stay in registers through the view-dependent part.
+            GaussProj h;
+            prepareGaussian<true>(quat[i], reinterpret_cast<const F3*>(logScale)[i], cam.splatScale, h, act);
+            h.opacity = opacity;
+            gaussAdjointShape(h, cam.splatScale, gSigma, gs[5], gls, gq, &gl);
             for (int j = 0; j < 3; ++j) gMeans[3 * i + j] = gm[j], gScales[3 * i + j] = gls[j];
             reinterpret_cast<float4*>(gQuats)[i] = make_float4(gq[0], gq[1], gq[2], gq[3]);
             gLogit[i] = gl;

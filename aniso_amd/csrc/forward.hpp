@@ -55,11 +55,14 @@ __host__ __device__ inline int shBasis(int degree, float x, float y, float z, fl
 
 // 0.5 + sum_k b_k sh_k per channel (before the clamp at zero; src/sh.cpp:53-55). Coefficient
 // j = k * 3 + channel sits at sh[j * stride]: stride 1 for one Gaussian's [coefficient][channel]
-// block, stride n for the planar device layout [coefficient * 3 + channel][n].
-template <bool PLANAR>
+// block, stride n for the planar device layout [coefficient * 3 + channel][n]. EAGER (SH in
+// global memory): all 48 coefficients of degree 3 are loaded before the first is used; without it
+// (SH in LDS, where a read's latency is short) each coefficient's three values are read as they
+// are used, so that the 48 do not occupy registers at once.
+template <bool PLANAR, bool EAGER = true>
 __host__ __device__ inline float3 shAccumulate(const float* __restrict__ sh, size_t stride, int count, const float b[16]) {
     float3 c = make_float3(0.5f, 0.5f, 0.5f);
-    if (count == 16) {
+    if (EAGER && count == 16) {
         float v[48];
         if constexpr (PLANAR) {
 #pragma unroll
@@ -165,14 +168,14 @@ __host__ __device__ inline void prepareGaussian(const float4 q4, const F3 sc, fl
         for (int c = 0; c < 3; ++c) g.Sigma[r][c] = g.M[r][0] * g.M[c][0] + g.M[r][1] * g.M[c][1] + g.M[r][2] * g.M[c][2];
 }
 
-template <bool PLANAR, bool RAW>
+template <bool PLANAR, bool RAW, bool EAGER_SH = true>
 __host__ __device__ inline bool finishProjection(const F3 m, float opacityIn, const float* __restrict__ shBase,
                                                  size_t shStride, const CamParams& cam, GaussProj& g);
 
 // Projects Gaussian i: cameraTransform, prepareGaussian, finishProjection. shBase points at this
 // Gaussian's coefficient 0 (red); coefficient * 3 + channel j sits at shBase[j * shStride]
 // (PLANAR: shStride = n; otherwise 1 and shBase 16-byte aligned).
-template <bool PLANAR, bool RAW>
+template <bool PLANAR, bool RAW, bool EAGER_SH = true>
 __host__ __device__ inline bool projectGaussian(int i, const float* __restrict__ pos, const float* __restrict__ scale,
                                                 const float4* __restrict__ rot, const float* __restrict__ opacity,
                                                 const float* __restrict__ shBase, size_t shStride, const CamParams& cam,
@@ -180,13 +183,13 @@ __host__ __device__ inline bool projectGaussian(int i, const float* __restrict__
     const F3 m = reinterpret_cast<const F3*>(pos)[i];
     if (!cameraTransform(m, cam, g)) return false;
     prepareGaussian<RAW>(rot[i], reinterpret_cast<const F3*>(scale)[i], cam.splatScale, g, act);
-    return finishProjection<PLANAR, RAW>(m, opacity[i], shBase, shStride, cam, g);
+    return finishProjection<PLANAR, RAW, EAGER_SH>(m, opacity[i], shBase, shStride, cam, g);
 }
 
 // The view-dependent rest: EWA 2D covariance with the clamped Jacobian and the 0.3 low-pass,
 // conic, radius, centre, screen cull, tile rect, SH colour, opacity (sigmoid of opacityIn when
 // RAW). Needs cameraTransform and prepareGaussian done on g.
-template <bool PLANAR, bool RAW>
+template <bool PLANAR, bool RAW, bool EAGER_SH>
 __host__ __device__ inline bool finishProjection(const F3 m, float opacityIn, const float* __restrict__ shBase,
                                                  size_t shStride, const CamParams& cam, GaussProj& g) {
     const float px = g.px, py = g.py, pz = g.pz;
@@ -240,7 +243,7 @@ __host__ __device__ inline bool finishProjection(const F3 m, float opacityIn, co
     g.dir[1] = g.dir[1] / g.dlen;
     g.dir[2] = g.dir[2] / g.dlen;
     g.shCount = shBasis(cam.shDegree, g.dir[0], g.dir[1], g.dir[2], g.basis);
-    g.rgbRaw = shAccumulate<PLANAR>(shBase, shStride, g.shCount, g.basis);
+    g.rgbRaw = shAccumulate<PLANAR, EAGER_SH>(shBase, shStride, g.shCount, g.basis);
     // std::max(0, v) semantics (NaN -> 0).
     g.rgb.x = (0.0f < g.rgbRaw.x) ? g.rgbRaw.x : 0.0f;
     g.rgb.y = (0.0f < g.rgbRaw.y) ? g.rgbRaw.y : 0.0f;
