@@ -111,9 +111,9 @@ constexpr int kPartialStride = 12;  // floats per pair record: u v A B C opacity
 //   alpha = o G (no gradient where the 0.99 clamp is active), G = exp(power),
 //   power = -(A dx^2 + C dy^2) / 2 - B dx dy, dx = px - u, dy = py - v.
 // Each wavefront visits only the splats whose culling test (the forward's) admits its 8 x 8
-// block, two at a time: pairSumStore reduces the two splats' per-pixel contributions over the
-// wavefront, and the wavefronts' sums are added in fixed order into one partial record per pair:
-// partial[slot * 12 + {u, v, A, B, C, opacity, r, g, b}].
+// block, two at a time: pairSumStore reduces the two splats' per-pixel moments over the
+// wavefront, and the wavefronts' sums are added in fixed order and turned into one partial
+// record per pair: partial[slot * 12 + {u, v, A, B, C, opacity, r, g, b}].
 template <int TW, int TH, int BATCH>
 __device__ inline void blendBackwardTile(int tileX, int tileY, uint2 range, const uint32_t* __restrict__ sortedGauss,
                                          const SplatRec* __restrict__ rec, const uint32_t* __restrict__ slotStart,
@@ -210,45 +210,43 @@ __device__ inline void blendBackwardTile(int tileX, int tileY, uint2 range, cons
         gg = gradImage[3 * p + 1];
         gb = gradImage[3 * p + 2];
     }
-    float ar = 0.0f, ag = 0.0f, ab = 0.0f;  // accum
+    float S = 0.0f;  // g . accum
+    int lastLocal = 0;  // positions of the current batch below it were blended by this pixel
     float Tcur = T;
-    // This pixel's gradient contributions from the splat with staged records (a, bb, c) at list
-    // position pos of the tile (zero if it does not blend there); steps T and accum back past it.
-    // True if it contributed. power and alpha are the blend's, bit for bit (same staged conic, same
-    // operations), so every blend/skip decision matches the forward's; selects replace branches.
-    auto visit = [&](const float4& a, const float4& bb, const float4& c, uint32_t pos, float (&d)[9]) -> bool {
+    // This pixel's contributions from the splat with staged records (a, bb, c) at position j of
+    // the batch (zero if it does not blend there), as the moments the pair record is formed from:
+    // d = (gPow dx, gPow dy, gPow dx dx, gPow dx dy, gPow dy dy, gPow, dL/drgb) with
+    // gPow = dL/dpower; steps T and g . accum back past the splat. True if it contributed.
+    // power and alpha are the blend's, bit for bit (same staged conic, same operations), so every
+    // blend/skip decision matches the forward's; selects and zero weights replace branches.
+    auto visit = [&](const float4& a, const float4& bb, const float4& c, int j, float (&d)[9]) -> bool {
         const float dx = px - a.x, dy = py - a.y;
-        const SplatExponent x = splatExponentParts(a.z, a.w, bb.x, bb.z, dx, dy);
-        const float raw = __builtin_amdgcn_exp2f(x.e);  // o G
-        const float G = raw * bb.w;                       // bb.w = 1 / o
+        const float e = splatExponent(a.z, a.w, bb.x, bb.z, dx, dy);
+        const float raw = __builtin_amdgcn_exp2f(e);  // o G
         const float alpha = fminf(0.99f, raw);
-        const bool use = pos < lastContrib && !(x.e > bb.z) && alpha >= 1.0f / 255.0f;
+        const bool use = j < lastLocal && !(e > bb.z) && alpha >= 1.0f / 255.0f;
         const float Ti = Tcur * __builtin_amdgcn_rcpf(1.0f - alpha);
-        const float w = use ? alpha * Ti : 0.0f;
+        // A zero weight leaves every sum below unchanged exactly (all terms are finite).
+        const float wa = use ? alpha : 0.0f;
+        const float w = wa * Ti;
         d[6] = gr * w;
         d[7] = gg * w;
         d[8] = gb * w;
-        const float er = c.x - ar, eg = c.y - ag, eb = c.z - ab;
-        const float gAlpha = Ti * __builtin_fmaf(gb, eb, __builtin_fmaf(gg, eg, gr * er));
-        // No gradient where the 0.99 clamp is active; gPow = 0 zeroes the conic and centre terms.
+        // dL/dalpha = T_i g . (rgb - accum), with g . accum carried as one running sum:
+        // accum <- accum + alpha (rgb - accum) gives g . accum <- S + alpha (g . rgb - S).
+        const float diff = __builtin_fmaf(gb, c.z, __builtin_fmaf(gg, c.y, gr * c.x)) - S;
+        const float gAlpha = Ti * diff;
+        // No gradient where the 0.99 clamp is active.
         const bool open = use && raw <= 0.99f;
         const float gPow = open ? gAlpha * raw : 0.0f;
-        d[5] = open ? gAlpha * G : 0.0f;
-        // With the staged qa = -log2(e) A/2, qb = -log2(e) B, qc = -log2(e) C/2 and
-        // t = qa dx + qb dy, r = qc dy: A dx + B dy = -(qa dx + t) ln 2 and
-        // C dy + B dx = -(qb dx + 2 r) ln 2.
-        const float gPowLn2 = gPow * (1.0f / kLog2e);
-        d[0] = -gPowLn2 * __builtin_fmaf(a.z, dx, x.t);
-        d[1] = -gPowLn2 * __builtin_fmaf(a.w, dx, x.r + x.r);
-        const float h = -0.5f * gPow;
-        const float hx = h * dx;
-        d[2] = hx * dx;
-        d[3] = (hx + hx) * dy;
-        d[4] = (h * dy) * dy;
-        // accum <- alpha rgb + (1 - alpha) accum, as accum + alpha (rgb - accum).
-        ar = use ? __builtin_fmaf(alpha, er, ar) : ar;
-        ag = use ? __builtin_fmaf(alpha, eg, ag) : ag;
-        ab = use ? __builtin_fmaf(alpha, eb, ab) : ab;
+        const float p = gPow * dx, q = gPow * dy;
+        d[0] = p;
+        d[1] = q;
+        d[2] = p * dx;
+        d[3] = p * dy;
+        d[4] = q * dy;
+        d[5] = gPow;
+        S = __builtin_fmaf(wa, diff, S);
         Tcur = use ? Ti : Tcur;
         return use;
     };
@@ -262,14 +260,14 @@ __device__ inline void blendBackwardTile(int tileX, int tileY, uint2 range, cons
         if (__ballot(lastContrib > static_cast<uint32_t>(b) * BATCH) != 0 && count > 0) {
             // Pairs (i, i - 1) back to front. Records are read at the top of each step: carried
             // across steps they cost more register moves than the LDS latency other waves hide.
-            const uint32_t base = static_cast<uint32_t>(b) * BATCH;
+            lastLocal = static_cast<int>(lastContrib) - b * BATCH;
             for (int i = count - 1; i >= 0; i -= 2) {
                 const int j1 = sList[wave][i];
                 const int j2 = i >= 1 ? static_cast<int>(sList[wave][i - 1]) : BATCH;
                 float d1[9], d2[9];
-                bool any = visit(s0[j1], s1[j1], s2[j1], base + j1, d1);
+                bool any = visit(s0[j1], s1[j1], s2[j1], j1, d1);
                 if (i >= 1) {
-                    any |= visit(s0[j2], s1[j2], s2[j2], base + j2, d2);
+                    any |= visit(s0[j2], s1[j2], s2[j2], j2, d2);
                 } else {
 #pragma unroll
                     for (int c = 0; c < 9; ++c) d2[c] = 0.0f;
@@ -278,20 +276,30 @@ __device__ inline void blendBackwardTile(int tileX, int tileY, uint2 range, cons
             }
         }
         __syncthreads();
-        // One partial per pair of this batch: the wavefronts' sums in fixed order.
+        // One partial per pair of this batch: the wavefronts' moment sums in fixed order, turned
+        // into the record's gradients. With the staged qa = -log2(e) A/2, qb = -log2(e) B,
+        // qc = -log2(e) C/2 and power = -(A dx^2 + C dy^2) / 2 - B dx dy:
+        //   dL/du = sum gPow (A dx + B dy) = -(2 qa M_x + qb M_y) ln 2,
+        //   dL/dv = sum gPow (B dx + C dy) = -(qb M_x + 2 qc M_y) ln 2,
+        //   dL/dA = -M_xx / 2, dL/dB = -M_xy, dL/dC = -M_yy / 2, dL/do = M_1 / o,
+        // M_x = sum gPow dx and so on.
         for (int k = tid; k < cnt; k += kThreads) {
-            float s[12];
+            float s[9];
 #pragma unroll
             for (int c = 0; c < 9; ++c) {
                 s[c] = 0.0f;
 #pragma unroll
                 for (int wv = 0; wv < kWaves; ++wv) s[c] += sGrad[wv][k][c];
             }
-            s[9] = s[10] = s[11] = 0.0f;
+            const float4 q = s0[k];
+            const float qc = s1[k].x, invO = s1[k].w;
+            constexpr float kLn2 = 1.0f / kLog2e;
+            const float gu = -kLn2 * __builtin_fmaf(q.z + q.z, s[0], q.w * s[1]);
+            const float gv = -kLn2 * __builtin_fmaf(q.w, s[0], (qc + qc) * s[1]);
             float4* out = reinterpret_cast<float4*>(partial + static_cast<size_t>(sSlot[k]) * kPartialStride);
-            out[0] = make_float4(s[0], s[1], s[2], s[3]);
-            out[1] = make_float4(s[4], s[5], s[6], s[7]);
-            out[2] = make_float4(s[8], s[9], s[10], s[11]);
+            out[0] = make_float4(gu, gv, -0.5f * s[2], -s[3]);
+            out[1] = make_float4(-0.5f * s[4], invO * s[5], s[6], s[7]);
+            out[2] = make_float4(s[8], 0.0f, 0.0f, 0.0f);
         }
     }
     // Pairs after the point where every pixel had stopped get zero partials.

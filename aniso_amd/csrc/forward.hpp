@@ -663,20 +663,10 @@ __device__ inline float stagedLogOpacity(float o) { return __builtin_amdgcn_logf
 // (qa dx + qb dy) dx + ((qc dy) dy + lo) in three fused multiply-adds and two products. Every
 // term of the power rounds relative to itself, so e > lo (the reference's power > 0 skip) holds
 // only where the true power is zero to within rounding, as in the reference's own evaluation.
-// The partial sums t = qa dx + qb dy and r = qc dy come back too: the blend's adjoint reuses them
-// for the derivatives of the power.
-struct SplatExponent {
-    float e, t, r;
-};
-
-__device__ inline SplatExponent splatExponentParts(float qa, float qb, float qc, float lo, float dx, float dy) {
+__device__ inline float splatExponent(float qa, float qb, float qc, float lo, float dx, float dy) {
     const float t = __builtin_fmaf(qa, dx, qb * dy);
     const float r = qc * dy;
-    return {__builtin_fmaf(t, dx, __builtin_fmaf(r, dy, lo)), t, r};
-}
-
-__device__ inline float splatExponent(float qa, float qb, float qc, float lo, float dx, float dy) {
-    return splatExponentParts(qa, qb, qc, lo, dx, dy).e;
+    return __builtin_fmaf(t, dx, __builtin_fmaf(r, dy, lo));
 }
 
 // Front-to-back blend, one workgroup per TW x TH tile, one lane per pixel; each wavefront owns
