@@ -247,13 +247,14 @@ def _device_index(t):
 class RasterizeViewsFunction(torch.autograd.Function):
     """Inputs: the five parameter tensors (raw, or with activated=True standard deviations and
     opacities in place of log-scales and logits), an optional `means2d` leaf ([V, N, 2], receives
-    dL/d(u, v) per view), the cameras, options, activated, optional stats list. Outputs: all views'
-    images concatenated (flat float32, view v's [H_v, W_v, 3] at its offset), the radii [V, N]
-    and the alpha images concatenated (flat, 1 - final transmittance; neither differentiable)."""
+    dL/d(u, v) per view), the cameras, options, activated, optional stats list, whether to form the
+    alpha images. Outputs: all views' images concatenated (flat float32, view v's [H_v, W_v, 3] at
+    its offset), the radii [V, N] and the alpha images concatenated (flat, 1 - final transmittance;
+    empty unless asked for; neither differentiable)."""
 
     @staticmethod
     def forward(ctx, means, log_scales, quats, opacity_logits, sh, means2d, cameras, options: RenderOptions,
-                activated: bool, stats):
+                activated: bool, stats, want_alpha: bool = True):
         n = means.shape[0]
         k = sh.shape[1]
         V = len(cameras)
@@ -269,17 +270,18 @@ class RasterizeViewsFunction(torch.autograd.Function):
         pixels = [c.width * c.height for c in cameras]
         offs = np.concatenate([[0], np.cumsum(pixels)]).astype(np.int64)
         flat = torch.empty(int(offs[-1]) * 3, dtype=torch.float32, device=means.device)
-        alpha = torch.empty(int(offs[-1]), dtype=torch.float32, device=means.device)
+        alpha = torch.empty(int(offs[-1]) if want_alpha else 0, dtype=torch.float32, device=means.device)
         cams = (_lib.Camera_ * V)(*[camera_c(c) for c in cameras])
         imgs = (ctypes.c_void_p * V)(*[flat.data_ptr() + 12 * int(o) for o in offs[:-1]])
-        alps = (ctypes.c_void_p * V)(*[alpha.data_ptr() + 4 * int(o) for o in offs[:-1]])
         st = (_lib.Stats_ * V)()
         fid = ctypes.c_int64()
         params = _params(means, log_scales, quats, opacity_logits, sh, activated)
         _lib.check(L.ans_diff_views_forward(handle, ctypes.byref(params), V, cams, ctypes.byref(options.c()), imgs,
                                             _stream(), ctypes.byref(fid), st))
         frame = _Frame(handle, fid.value)
-        _lib.check(L.ans_diff_views_alpha(handle, fid.value, alps, _stream()))
+        if want_alpha:
+            alps = (ctypes.c_void_p * V)(*[alpha.data_ptr() + 4 * int(o) for o in offs[:-1]])
+            _lib.check(L.ans_diff_views_alpha(handle, fid.value, alps, _stream()))
         radii = torch.empty((V, n), dtype=torch.int32, device=means.device)
         if means2d is not None:
             _lib.check(L.ans_diff_views_radii(handle, fid.value, radii.data_ptr(), _stream()))
@@ -315,7 +317,7 @@ class RasterizeViewsFunction(torch.autograd.Function):
             gptrs, ctypes.byref(_lib.Grads_(*[t.data_ptr() for t in grads])),
             grad_uv.data_ptr() if grad_uv is not None else None, _stream()))
         frame.release()
-        return (*grads, grad_uv, None, None, None, None)
+        return (*grads, grad_uv, None, None, None, None, None)
 
 
 def _split_views(flat, cameras, channels):
@@ -342,7 +344,7 @@ def rasterize_views(means, log_scales, quats, opacity_logits, sh, cameras, optio
         means2d = torch.zeros((len(cameras), means.shape[0], 2), dtype=torch.float32, device=means.device,
                               requires_grad=True)
     flat_images, radii, alpha = RasterizeViewsFunction.apply(means, log_scales, quats, opacity_logits, sh, means2d,
-                                                             list(cameras), opts, activated, stats)
+                                                             list(cameras), opts, activated, stats, info)
     images = flat_images if flat else _split_views(flat_images, cameras, 3)
     if not info:
         return images

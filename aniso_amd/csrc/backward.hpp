@@ -56,15 +56,27 @@ __device__ inline float waveSum(float v) {
     return addSwapped(r32[0], r32[1]);
 }
 
-// Sums over the 64 lanes of two splats' 9 gradient components at once (a reduce-scatter):
+// v = sel ? a : b in this lane, w = sel ? b : a, and v plus w from the DPP partner lane: where
+// sel is true in a lane and false in its partner (and the other way round), each lane ends up with
+// a pairwise sum of a or of b.
+template <int CTRL>
+__device__ inline float foldPair(float a, float b, bool sel) {
+    const float v = sel ? a : b, w = sel ? b : a;
+    return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(w), CTRL, 0xf, 0xf, false));
+}
+
+// Sums over the 64 lanes of two splats' 9 gradient components at once, a reduce-scatter: each
+// stage halves the lanes a value is spread over and the number of registers holding values.
 // permlane32_swap leaves lanes 0-31 holding splat 1's pairwise sums and lanes 32-63 splat 2's;
-// permlane16_swap pairs components (k, k + 5) so that rows 0..3 of register k hold
-// (splat 1, k), (splat 1, k + 5), (splat 2, k), (splat 2, k + 5); a row butterfly of DPP adds
-// (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror, row_mirror) completes each row.
-// Component 4 has no partner: its rows 1 and 3 repeat rows 0 and 2. The last lane of each row
-// then writes its value to out[(splat 1 or 2) row base + k (+ 5)], where the caller's row of 10
-// floats takes component 4's repeat in slot 9. The order of additions is fixed.
-__device__ inline void pairSumStore(float (&d1)[9], float (&d2)[9], float* rowBase) {
+// permlane16_swap pairs components (k, k + 5), k < 4, so that rows 0..3 of register k hold
+// (splat 1, k), (splat 1, k + 5), (splat 2, k), (splat 2, k + 5) (component 4 pairs with itself:
+// rows 0, 1 and 2, 3 repeat it); within each 16-lane row, row_ror:8 folds registers (0, 1) and
+// (2, 3), row_half_mirror the two results, quad_perm [2,3,0,1] that with component 4, and
+// quad_perm [1,0,3,2] completes every sum. Lane p of row r then holds, for p % 4 == 0, component
+// {0, 2, 1, 3}[p / 4] (+ 5 in rows 1 and 3), and for p % 4 == 2 component 4, of splat 1 (rows 0, 1)
+// or splat 2 (rows 2, 3); the lanes with p % 4 == 0, and p == 2 in rows 0 and 2, store their
+// sum at rowBase1 or rowBase2 + component. The order of additions is fixed.
+__device__ inline void pairSumStore(const float (&d1)[9], const float (&d2)[9], float* rowBase1, float* rowBase2) {
     float t[9];
 #pragma unroll
     for (int c = 0; c < 9; ++c) {
@@ -81,21 +93,21 @@ __device__ inline void pairSumStore(float (&d1)[9], float (&d2)[9], float* rowBa
         const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(t[4]), __float_as_uint(t[4]), false, false);
         u[4] = addSwapped(r[0], r[1]);
     }
-#pragma unroll
-    for (int k = 0; k < 5; ++k) u[k] = dppAdd<0xB1>(u[k]);
-#pragma unroll
-    for (int k = 0; k < 5; ++k) u[k] = dppAdd<0x4E>(u[k]);
-#pragma unroll
-    for (int k = 0; k < 5; ++k) u[k] = dppAdd<0x141>(u[k]);
-#pragma unroll
-    for (int k = 0; k < 5; ++k) u[k] = dppAdd<0x140>(u[k]);
-    // An empty use of each sum outside the branch below: otherwise the compiler moves the last
-    // DPP add into the branch and splits it into a register clear, a DPP move and an add.
-#pragma unroll
-    for (int k = 0; k < 5; ++k) asm volatile("" : : "v"(u[k]));
-    if ((__lane_id() & 15) == 15) {
-#pragma unroll
-        for (int k = 0; k < 5; ++k) rowBase[k] = u[k];
+    const int lane = __lane_id(), p = lane & 15;
+    const float v01 = foldPair<0x128>(u[0], u[1], p < 8);  // row_ror:8
+    const float v23 = foldPair<0x128>(u[2], u[3], p < 8);
+    const float v4 = dppAdd<0x128>(u[4]);
+    const float w = foldPair<0x141>(v01, v23, (p & 7) < 4);  // row_half_mirror
+    const float w4 = dppAdd<0x141>(v4);
+    const float z = dppAdd<0xB1>(foldPair<0x4E>(w, w4, (p & 3) < 2));  // quad_perm [2,3,0,1], [1,0,3,2]
+    // An empty use of the sum outside the branch below: otherwise the compiler moves the last DPP
+    // add into the branch and splits it into a register clear, a DPP move and an add.
+    asm volatile("" : : "v"(z));
+    const int row = lane >> 4;
+    const bool writer = (p & 3) == 0 || (p == 2 && (row & 1) == 0);
+    if (writer) {
+        const int k = (p & 3) ? 4 : ((p >> 2) == 1 ? 2 : (p >> 2) == 2 ? 1 : (p >> 2)) + (row & 1) * 5;
+        (row < 2 ? rowBase1 : rowBase2)[k] = z;
     }
 }
 
@@ -122,12 +134,20 @@ __device__ inline void blendBackwardTile(int tileX, int tileY, uint2 range, cons
     constexpr int kThreads = TW * TH;
     constexpr int kWaves = kThreads / kWave;
     constexpr int kWavesX = TW / 8;
-    __shared__ float4 s0[BATCH], s1[BATCH], s2[BATCH];
+    // Staged per splat in three arrays of 16-byte rows read at one byte offset (the visit lists
+    // hold offsets, so the walk computes no addresses): (u, v, qa, qb), (qc, log2 o, r, g),
+    // (b, 1 / o). Row BATCH is a splat that never blends (log2 o = -inf, zero colour): the
+    // partner of the last splat of an odd-length list, which every list has at its entry 0.
+    __shared__ float4 sRec[3][BATCH + 1];
     __shared__ uint8_t sMask[BATCH];
-    __shared__ uint8_t sList[kWaves][BATCH];
+    __shared__ uint16_t sList[kWaves][BATCH + 1];
     __shared__ uint32_t sSlot[BATCH];
-    __shared__ float sGrad[kWaves][BATCH + 1][10];  // row BATCH: the partner of an unpaired splat
+    __shared__ float sGrad[kWaves][BATCH + 1][9];  // row BATCH: the never-blending splat's zeros
     __shared__ uint32_t sMaxLast;
+    auto row = [&](int r, uint32_t off) -> const float4& {
+        return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(sRec[r]) + off);
+    };
+    constexpr uint16_t kNone = BATCH * 16;
 
     const int tid = threadIdx.x;
     const int wave = tid / kWave, lane = __lane_id();
@@ -152,9 +172,10 @@ __device__ inline void blendBackwardTile(int tileX, int tileY, uint2 range, cons
             const uint2 rp = make_uint2(meta.x, meta.y);
             const uint32_t first = slotStart[g];
             const float3 q = stagedConic(a.z, a.w, bb.x);  // the blend's staged conic and log2 o
-            s0[k] = make_float4(a.x, a.y, q.x, q.y);
-            s1[k] = make_float4(q.z, bb.y, stagedLogOpacity(bb.y), 1.0f / bb.y);
-            s2[k] = rec[g].col;
+            const float4 col = rec[g].col;
+            sRec[0][k] = make_float4(a.x, a.y, q.x, q.y);
+            sRec[1][k] = make_float4(q.z, stagedLogOpacity(bb.y), col.x, col.y);
+            sRec[2][k] = make_float4(col.z, 1.0f / bb.y, 0.0f, 0.0f);
             const uint32_t x0 = rp.x & 0xffffu, y0 = rp.x >> 16, w = (rp.y & 0xffffu) - x0 + 1;
             sSlot[k] = first + (static_cast<uint32_t>(tileY) - y0) * w + (static_cast<uint32_t>(tileX) - x0);
             uint32_t m = 0;
@@ -178,7 +199,7 @@ __device__ inline void blendBackwardTile(int tileX, int tileY, uint2 range, cons
             const int j = g0 + lane;
             const bool hit = j < cnt && ((sMask[j] >> wave) & 1u);
             const uint64_t bal = __ballot(hit);
-            if (hit) sList[wave][count + __popcll(bal & below)] = static_cast<uint8_t>(j);
+            if (hit) sList[wave][1 + count + __popcll(bal & below)] = static_cast<uint16_t>(j * 16);
             count += __popcll(bal);
         }
         waveSync();
@@ -193,7 +214,13 @@ __device__ inline void blendBackwardTile(int tileX, int tileY, uint2 range, cons
         T = __uint_as_float(st.x);
         lastContrib = st.y;
     }
-    if (tid == 0) sMaxLast = 0;
+    if (tid == 0) {
+        sMaxLast = 0;
+        sRec[0][BATCH] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        sRec[1][BATCH] = make_float4(0.0f, -__builtin_huge_valf(), 0.0f, 0.0f);
+        sRec[2][BATCH] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    if (lane == 0) sList[wave][0] = kNone;
     uint32_t m = lastContrib;
 #pragma unroll
     for (int d = 1; d < kWave; d <<= 1) m = max(m, static_cast<uint32_t>(__shfl_xor(static_cast<int>(m), d, kWave)));
@@ -211,20 +238,21 @@ __device__ inline void blendBackwardTile(int tileX, int tileY, uint2 range, cons
         gb = gradImage[3 * p + 2];
     }
     float S = 0.0f;  // g . accum
-    int lastLocal = 0;  // positions of the current batch below it were blended by this pixel
+    int lastLocal = 0;  // staged byte offsets below it are positions this pixel blended
     float Tcur = T;
-    // This pixel's contributions from the splat with staged records (a, bb, c) at position j of
-    // the batch (zero if it does not blend there), as the moments the pair record is formed from:
+    // This pixel's contributions from the splat staged at byte offset off (position off / 16 of
+    // the batch; zero if it does not blend there), as the moments the pair record is formed from:
     // d = (gPow dx, gPow dy, gPow dx dx, gPow dx dy, gPow dy dy, gPow, dL/drgb) with
     // gPow = dL/dpower; steps T and g . accum back past the splat. True if it contributed.
     // power and alpha are the blend's, bit for bit (same staged conic, same operations), so every
     // blend/skip decision matches the forward's; selects and zero weights replace branches.
-    auto visit = [&](const float4& a, const float4& bb, const float4& c, int j, float (&d)[9]) -> bool {
+    auto visit = [&](uint32_t off, float (&d)[9]) -> bool {
+        const float4 a = row(0, off), bb = row(1, off), c = row(2, off);
         const float dx = px - a.x, dy = py - a.y;
-        const float e = splatExponent(a.z, a.w, bb.x, bb.z, dx, dy);
+        const float e = splatExponent(a.z, a.w, bb.x, bb.y, dx, dy);
         const float raw = __builtin_amdgcn_exp2f(e);  // o G
         const float alpha = fminf(0.99f, raw);
-        const bool use = j < lastLocal && !(e > bb.z) && alpha >= 1.0f / 255.0f;
+        const bool use = static_cast<int>(off) < lastLocal && !(e > bb.y) && alpha >= 1.0f / 255.0f;
         const float Ti = Tcur * __builtin_amdgcn_rcpf(1.0f - alpha);
         // A zero weight leaves every sum below unchanged exactly (all terms are finite).
         const float wa = use ? alpha : 0.0f;
@@ -234,7 +262,7 @@ __device__ inline void blendBackwardTile(int tileX, int tileY, uint2 range, cons
         d[8] = gb * w;
         // dL/dalpha = T_i g . (rgb - accum), with g . accum carried as one running sum:
         // accum <- accum + alpha (rgb - accum) gives g . accum <- S + alpha (g . rgb - S).
-        const float diff = __builtin_fmaf(gb, c.z, __builtin_fmaf(gg, c.y, gr * c.x)) - S;
+        const float diff = __builtin_fmaf(gb, c.x, __builtin_fmaf(gg, bb.w, gr * bb.z)) - S;
         const float gAlpha = Ti * diff;
         // No gradient where the 0.99 clamp is active.
         const bool open = use && raw <= 0.99f;
@@ -250,29 +278,23 @@ __device__ inline void blendBackwardTile(int tileX, int tileY, uint2 range, cons
         Tcur = use ? Ti : Tcur;
         return use;
     };
-    const int rowHalf = (lane >> 4) & 1;  // rows 1 and 3 write components 5..9
-    const bool secondSplat = lane >= 32;  // rows 2 and 3 write the second splat
     for (int b = nbDone - 1; b >= 0; --b) {
         __syncthreads();
-        for (int k = tid; k < kWaves * (BATCH + 1) * 10; k += kThreads) (&sGrad[0][0][0])[k] = 0.0f;
+        for (int k = tid; k < kWaves * (BATCH + 1) * 9; k += kThreads) (&sGrad[0][0][0])[k] = 0.0f;
         int count;
         const int cnt = stage(b, count);
         if (__ballot(lastContrib > static_cast<uint32_t>(b) * BATCH) != 0 && count > 0) {
-            // Pairs (i, i - 1) back to front. Records are read at the top of each step: carried
-            // across steps they cost more register moves than the LDS latency other waves hide.
-            lastLocal = static_cast<int>(lastContrib) - b * BATCH;
-            for (int i = count - 1; i >= 0; i -= 2) {
-                const int j1 = sList[wave][i];
-                const int j2 = i >= 1 ? static_cast<int>(sList[wave][i - 1]) : BATCH;
+            // Pairs (i, i - 1) back to front over list entries count..1 (entry 0: the splat that
+            // never blends). Records are read at the top of each step: carried across steps they
+            // cost more register moves than the LDS latency other waves hide.
+            lastLocal = (static_cast<int>(lastContrib) - b * BATCH) * 16;
+            for (int i = count; i >= 1; i -= 2) {
+                const uint32_t o1 = sList[wave][i], o2 = sList[wave][i - 1];
                 float d1[9], d2[9];
-                bool any = visit(s0[j1], s1[j1], s2[j1], j1, d1);
-                if (i >= 1) {
-                    any |= visit(s0[j2], s1[j2], s2[j2], j2, d2);
-                } else {
-#pragma unroll
-                    for (int c = 0; c < 9; ++c) d2[c] = 0.0f;
-                }
-                if (__ballot(any) != 0) pairSumStore(d1, d2, &sGrad[wave][secondSplat ? j2 : j1][rowHalf * 5]);
+                const bool u1 = visit(o1, d1);
+                const bool u2 = visit(o2, d2);
+                if ((__builtin_amdgcn_ballot_w64(u1) | __builtin_amdgcn_ballot_w64(u2)) != 0)
+                    pairSumStore(d1, d2, &sGrad[wave][o1 >> 4][0], &sGrad[wave][o2 >> 4][0]);
             }
         }
         __syncthreads();
@@ -291,8 +313,8 @@ __device__ inline void blendBackwardTile(int tileX, int tileY, uint2 range, cons
 #pragma unroll
                 for (int wv = 0; wv < kWaves; ++wv) s[c] += sGrad[wv][k][c];
             }
-            const float4 q = s0[k];
-            const float qc = s1[k].x, invO = s1[k].w;
+            const float4 q = sRec[0][k];
+            const float qc = sRec[1][k].x, invO = sRec[2][k].y;
             constexpr float kLn2 = 1.0f / kLog2e;
             const float gu = -kLn2 * __builtin_fmaf(q.z + q.z, s[0], q.w * s[1]);
             const float gv = -kLn2 * __builtin_fmaf(q.w, s[0], (qc + qc) * s[1]);
