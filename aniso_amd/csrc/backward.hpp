@@ -567,6 +567,19 @@ __device__ inline int rowOf(int e, float rcpK3) {
     return static_cast<int>((static_cast<float>(e) + 0.5f) * rcpK3);
 }
 
+// One step of a segmented inclusive scan of x over the lanes (the steps of waveInclusive): a lane
+// adds the DPP source lane's partial sums when that lane has the same owner (owners are
+// non-decreasing over the lanes, so the lanes in between have it too).
+template <int CTRL, int ROWS = 0xf>
+__device__ inline void segmentStep(int owner, float (&x)[9]) {
+    const bool same = dppMove<CTRL, ROWS>(-2, owner) == owner;  // owners are at least -1
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+        const float y = __int_as_float(dppMove<CTRL, ROWS>(0, __float_as_int(x[c])));
+        x[c] = same ? x[c] + y : x[c];
+    }
+}
+
 // Record sums of one wavefront's 64 Gaussians (lane = Gaussian, slots [s0, s0 + tt)) over their
 // records [r0, r1), 64 records per chunk, one record per lane: the owner of each record comes
 // from head flags at the owners' first slots and a wavefront max-scan (as in the pair
@@ -603,17 +616,13 @@ __device__ inline void sumRecords(const float4* __restrict__ rec, uint32_t r0, u
         const int owner = max(carry, waveInclusiveMax(S.head[lane]));
         carry = __builtin_amdgcn_readlane(owner, kWave - 1);
         const bool has = c0 + lane < r1;
-#pragma unroll
-        for (int d = 1; d < kWave; d <<= 1) {
-            const int up = __shfl_up(owner, d, kWave);
-            const bool same = lane >= d && up == owner;
-#pragma unroll
-            for (int c = 0; c < 9; ++c) {
-                const float y = __shfl_up(x[c], d, kWave);
-                x[c] = same ? x[c] + y : x[c];
-            }
-        }
-        const int next = __shfl_down(owner, 1, kWave);
+        segmentStep<kDppRowShr1>(owner, x);
+        segmentStep<kDppRowShr2>(owner, x);
+        segmentStep<kDppRowShr4>(owner, x);
+        segmentStep<kDppRowShr8>(owner, x);
+        segmentStep<kDppRowBcast15, 0xa>(owner, x);
+        segmentStep<kDppRowBcast31, 0xc>(owner, x);
+        const int next = dppMove<kDppWaveShl1>(-2, owner);
         const bool end = has && (lane == kWave - 1 || c0 + lane + 1 >= r1 || next != owner);
         if (end)
 #pragma unroll

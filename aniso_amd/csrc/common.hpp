@@ -85,4 +85,39 @@ __device__ inline uint64_t lanesBelow() {
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
+// DPP lane moves within a 64-lane wavefront (GFX9 controls): row_shr:n gives lane l of each
+// 16-lane row lane l - n of the same row; row_bcast:15 gives every lane of rows 1 and 3 (row mask
+// 0xa) lane 15 of the row below; row_bcast:31 gives rows 2 and 3 (row mask 0xc) lane 31;
+// wave_shl:1 gives lane l lane l + 1. Lanes without a source, or in masked rows, get `old`.
+// These run on the vector ALU; lane shuffles through the LDS crossbar (ds_bpermute) cost a
+// round trip each.
+enum : int {
+    kDppRowShr1 = 0x111,
+    kDppRowShr2 = 0x112,
+    kDppRowShr4 = 0x114,
+    kDppRowShr8 = 0x118,
+    kDppWaveShl1 = 0x130,
+    kDppRowBcast15 = 0x142,
+    kDppRowBcast31 = 0x143,
+};
+
+template <int CTRL, int ROWS = 0xf>
+__device__ inline int dppMove(int old, int v) {
+    return __builtin_amdgcn_update_dpp(old, v, CTRL, ROWS, 0xf, false);
+}
+
+// Inclusive scan over the 64 lanes with an associative integer operation whose identity is
+// ident: steps of 1, 2, 4 and 8 lanes within each row, then row 0's total into row 1 and row 2's
+// into row 3, then rows 0-1's total into rows 2-3.
+template <typename Op>
+__device__ inline int waveInclusive(int v, int ident, Op op) {
+    v = op(v, dppMove<kDppRowShr1>(ident, v));
+    v = op(v, dppMove<kDppRowShr2>(ident, v));
+    v = op(v, dppMove<kDppRowShr4>(ident, v));
+    v = op(v, dppMove<kDppRowShr8>(ident, v));
+    v = op(v, dppMove<kDppRowBcast15, 0xa>(ident, v));
+    v = op(v, dppMove<kDppRowBcast31, 0xc>(ident, v));
+    return v;
+}
+
 }  // namespace ans

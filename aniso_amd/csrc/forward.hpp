@@ -457,13 +457,7 @@ constexpr int kScanBlock = 1024;  // k_scanPairs: one pass covers 4096 expansion
 
 // Inclusive max-scan across a 64-lane wavefront.
 __device__ inline int waveInclusiveMax(int v) {
-    const int lane = __lane_id();
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-        const int t = __shfl_up(v, d, kWave);
-        if (lane >= d) v = max(v, t);
-    }
-    return v;
+    return waveInclusive(v, INT_MIN, [](int a, int b) { return max(a, b); });
 }
 
 // Pair expansion in depth order: each wavefront takes 64 consecutive depth ranks and writes their

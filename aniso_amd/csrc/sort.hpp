@@ -30,13 +30,8 @@ constexpr int kTileItems = 8;   // 2048 keys per sort block (a single frame's ti
 
 // Inclusive scan of one value per lane across a 64-lane wavefront.
 __device__ inline uint32_t waveInclusiveScan(uint32_t v) {
-    const int lane = __lane_id();
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-        const uint32_t t = __shfl_up(v, d, kWave);
-        if (lane >= d) v += t;
-    }
-    return v;
+    return static_cast<uint32_t>(
+        waveInclusive(static_cast<int>(v), 0, [](int a, int b) { return static_cast<int>(static_cast<uint32_t>(a) + static_cast<uint32_t>(b)); }));
 }
 
 // Exclusive scan across a block of BLOCK threads; sTmp holds BLOCK / 64 words. Returns the
@@ -64,12 +59,12 @@ __device__ inline uint32_t blockExclusiveScan(uint32_t v, uint32_t* sTmp, uint32
 // Lanes of the wavefront whose digit equals this lane's (restricted to `valid` lanes).
 template <int MAXBITS>
 __device__ inline uint64_t matchDigit(uint32_t digit, int bits, bool valid) {
-    uint64_t peers = __ballot(valid);
+    uint64_t peers = __builtin_amdgcn_ballot_w64(valid);
 #pragma unroll
     for (int b = 0; b < MAXBITS; ++b) {
         if (b < bits) {
             const bool bit = (digit >> b) & 1u;
-            const uint64_t m = __ballot(bit);
+            const uint64_t m = __builtin_amdgcn_ballot_w64(bit);
             peers &= bit ? m : ~m;
         }
     }
