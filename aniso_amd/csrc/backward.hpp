@@ -386,9 +386,11 @@ __host__ __device__ inline void gaussAdjointView(const GaussProj& g, const CamPa
                                                  const float* shi, float gm[3], float gSigma[3][3], float* gsh) {
     const float gu = gs[0], gv = gs[1], gA = gs[2], gB = gs[3], gC = gs[4];
     const float px = g.px, py = g.py, pz = g.pz;
+    // Powers of 1 / z are formed once and multiplied (one division instead of eleven).
+    const float ipz = 1.0f / pz, ipz2 = ipz * ipz, ipz3 = ipz2 * ipz;
 
-    float gp0 = gu * cam.fx / pz, gp1 = gv * cam.fy / pz;
-    float gp2 = -gu * cam.fx * px / (pz * pz) - gv * cam.fy * py / (pz * pz);
+    float gp0 = gu * cam.fx * ipz, gp1 = gv * cam.fy * ipz;
+    float gp2 = -gu * cam.fx * px * ipz2 - gv * cam.fy * py * ipz2;
     // Conic A = c / det, B = -b / det, C = a / det of the 2D covariance (det = a c - b^2).
     const float idet = 1.0f / g.det, idet2 = idet * idet;
     const float ga = gA * (-g.c * g.c * idet2) + gB * (g.b * g.c * idet2) + gC * (-g.b * g.b * idet2);
@@ -411,12 +413,11 @@ __host__ __device__ inline void gaussAdjointView(const GaussProj& g, const CamPa
     float gJ[2][3];
     for (int r = 0; r < 2; ++r)
         for (int k = 0; k < 3; ++k) gJ[r][k] = gT[r][0] * cam.R[3 * k] + gT[r][1] * cam.R[3 * k + 1] + gT[r][2] * cam.R[3 * k + 2];
-    const float pz2 = pz * pz, pz3 = pz2 * pz;
-    gp2 += gJ[0][0] * (-cam.fx / pz2) + gJ[1][1] * (-cam.fy / pz2) + gJ[0][2] * (2.0f * cam.fx * g.tx / pz3) +
-           gJ[1][2] * (2.0f * cam.fy * g.ty / pz3);
-    const float gtx = gJ[0][2] * (-cam.fx / pz2), gty = gJ[1][2] * (-cam.fy / pz2);
-    if (!g.clampX) gp0 += gtx; else gp2 += gtx * (g.tx / pz);
-    if (!g.clampY) gp1 += gty; else gp2 += gty * (g.ty / pz);
+    gp2 += gJ[0][0] * (-cam.fx * ipz2) + gJ[1][1] * (-cam.fy * ipz2) + gJ[0][2] * (2.0f * cam.fx * g.tx * ipz3) +
+           gJ[1][2] * (2.0f * cam.fy * g.ty * ipz3);
+    const float gtx = gJ[0][2] * (-cam.fx * ipz2), gty = gJ[1][2] * (-cam.fy * ipz2);
+    if (!g.clampX) gp0 += gtx; else gp2 += gtx * (g.tx * ipz);
+    if (!g.clampY) gp1 += gty; else gp2 += gty * (g.ty * ipz);
     for (int j = 0; j < 3; ++j) gm[j] = cam.R[j] * gp0 + cam.R[3 + j] * gp1 + cam.R[6 + j] * gp2;
     // SH colour, clamped at zero: d sh_k = basis_k d rgb; d dir through the basis.
     const float grgb[3] = {g.rgbRaw.x > 0.0f ? gs[6] : 0.0f, g.rgbRaw.y > 0.0f ? gs[7] : 0.0f,
@@ -448,7 +449,8 @@ __host__ __device__ inline void gaussAdjointView(const GaussProj& g, const CamPa
         for (int j = g.shCount * 3; j < cam.shCoeffs * 3; ++j) gsh[j] = 0.0f;
     // dir = (m - eye) / |m - eye|.
     const float dd = gdir[0] * g.dir[0] + gdir[1] * g.dir[1] + gdir[2] * g.dir[2];
-    for (int j = 0; j < 3; ++j) gm[j] += (gdir[j] - dd * g.dir[j]) / g.dlen;
+    const float idlen = 1.0f / g.dlen;
+    for (int j = 0; j < 3; ++j) gm[j] += (gdir[j] - dd * g.dir[j]) * idlen;
 }
 
 // The view-independent part: from dL/dSigma and dL/dopacity (go) to the gradients of the

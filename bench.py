@@ -644,8 +644,8 @@ def bench_views_fwd_bwd(workload, steps, warmup):
     for c in cams:
         targets.append(tflat[o:o + c.height * c.width * 3].view(c.height, c.width, 3))
         o += c.height * c.width * 3
-    # The batched loss over the flat images: mean over views of each view's L1 mean, as one
-    # weighted sum (weight 1 / (V * 3 H W) for every value of a view).
+    # The batched loss over the flat images: mean over views of each view's L1 mean, as one dot
+    # product with the weights (1 / (V * 3 H W) for every value of a view).
     weight = torch.cat([torch.full((c.height * c.width * 3,), 1.0 / (V * 3 * c.height * c.width), device=dev)
                         for c in cams])
     stats = []
@@ -654,7 +654,7 @@ def bench_views_fwd_bwd(workload, steps, warmup):
         for p in params:
             p.grad = None
         flat = rasterize_views(*params, cams, stats=stats if not stats else None, flat=True)
-        loss = ((flat - tflat).abs() * weight).sum()
+        loss = torch.dot((flat - tflat).abs(), weight)
         loss.backward()
 
     def per_view():
