@@ -745,7 +745,7 @@ __device__ inline void gaussBackwardWave(int n, int views, const DiffParamsDev& 
         }
         if (valid && P.screenGrads) P.screenGrads[vo + i] = make_float2(gs[0], gs[1]);
         if (tt > 0 && cameraTransform(m, cams[v], g) &&
-            finishProjection<false, true, false>(m, logit, shRow, 1, cams[v], g)) {
+            finishProjection<false, true, false, true>(m, logit, shRow, 1, cams[v], g)) {
             float vm[3], vSig[3][3];
             gaussAdjointView<true>(g, cams[v], gs, shRow, vm, vSig, gsh);
             for (int j = 0; j < 3; ++j) gm[j] += vm[j];
@@ -852,9 +852,9 @@ __device__ inline void gaussBackwardWave(int n, int views, const DiffParamsDev& 
     if (valid) {
         for (int c = 0; c < 9; ++c) splatGrad[static_cast<size_t>(i) * 9 + c] = gs[c];
         GaussProj g;
-        if (tt > 0 && projectGaussian<false, true, false>(i, pos, logScale, quat, logit, row, 1, cam, g, act)) {
+        if (tt > 0 && projectGaussian<false, true, false, true>(i, pos, logScale, quat, logit, row, 1, cam, g, act)) {
             float gm[3], gls[3], gq[4], gl, gSigma[3][3];
-            const float opacity = g.opacity;
+            const float opacity = act ? logit[i] : 1.0f / (1.0f + expf(-logit[i]));  // as the forward's
             gaussAdjointView<false>(g, cam, gs, row, gm, gSigma, row);
             // The rotation and scale matrices are formed again for the shape part, so that they
             // need not stay in registers through the view-dependent part.

@@ -168,14 +168,25 @@ __host__ __device__ inline void prepareGaussian(const float4 q4, const F3 sc, fl
         for (int c = 0; c < 3; ++c) g.Sigma[r][c] = g.M[r][0] * g.M[c][0] + g.M[r][1] * g.M[c][1] + g.M[r][2] * g.M[c][2];
 }
 
-template <bool PLANAR, bool RAW, bool EAGER_SH = true>
+// Unit direction from the camera centre to the Gaussian, and the distance.
+__host__ __device__ inline void setViewDirection(const F3 m, const CamParams& cam, GaussProj& g) {
+    g.dir[0] = m.x - cam.eye[0];
+    g.dir[1] = m.y - cam.eye[1];
+    g.dir[2] = m.z - cam.eye[2];
+    g.dlen = sqrtf(g.dir[0] * g.dir[0] + g.dir[1] * g.dir[1] + g.dir[2] * g.dir[2]);
+    g.dir[0] = g.dir[0] / g.dlen;
+    g.dir[1] = g.dir[1] / g.dlen;
+    g.dir[2] = g.dir[2] / g.dlen;
+}
+
+template <bool PLANAR, bool RAW, bool EAGER_SH = true, bool ADJ = false>
 __host__ __device__ inline bool finishProjection(const F3 m, float opacityIn, const float* __restrict__ shBase,
                                                  size_t shStride, const CamParams& cam, GaussProj& g);
 
 // Projects Gaussian i: cameraTransform, prepareGaussian, finishProjection. shBase points at this
 // Gaussian's coefficient 0 (red); coefficient * 3 + channel j sits at shBase[j * shStride]
 // (PLANAR: shStride = n; otherwise 1 and shBase 16-byte aligned).
-template <bool PLANAR, bool RAW, bool EAGER_SH = true>
+template <bool PLANAR, bool RAW, bool EAGER_SH = true, bool ADJ = false>
 __host__ __device__ inline bool projectGaussian(int i, const float* __restrict__ pos, const float* __restrict__ scale,
                                                 const float4* __restrict__ rot, const float* __restrict__ opacity,
                                                 const float* __restrict__ shBase, size_t shStride, const CamParams& cam,
@@ -183,13 +194,16 @@ __host__ __device__ inline bool projectGaussian(int i, const float* __restrict__
     const F3 m = reinterpret_cast<const F3*>(pos)[i];
     if (!cameraTransform(m, cam, g)) return false;
     prepareGaussian<RAW>(rot[i], reinterpret_cast<const F3*>(scale)[i], cam.splatScale, g, act);
-    return finishProjection<PLANAR, RAW, EAGER_SH>(m, opacity[i], shBase, shStride, cam, g);
+    return finishProjection<PLANAR, RAW, EAGER_SH, ADJ>(m, opacity[i], shBase, shStride, cam, g);
 }
 
 // The view-dependent rest: EWA 2D covariance with the clamped Jacobian and the 0.3 low-pass,
 // conic, radius, centre, screen cull, tile rect, SH colour, opacity (sigmoid of opacityIn when
-// RAW). Needs cameraTransform and prepareGaussian done on g.
-template <bool PLANAR, bool RAW, bool EAGER_SH>
+// RAW). Needs cameraTransform and prepareGaussian done on g. ADJ (the adjoint of a Gaussian the
+// forward kept): only what the adjoint reads (the covariance and its Jacobian, the view
+// direction, SH basis and unclamped colour), with the same operations; the radius, centre,
+// screen cull, tile rect, conic and opacity are skipped.
+template <bool PLANAR, bool RAW, bool EAGER_SH, bool ADJ>
 __host__ __device__ inline bool finishProjection(const F3 m, float opacityIn, const float* __restrict__ shBase,
                                                  size_t shStride, const CamParams& cam, GaussProj& g) {
     const float px = g.px, py = g.py, pz = g.pz;
@@ -214,6 +228,12 @@ __host__ __device__ inline bool finishProjection(const F3 m, float opacityIn, co
     g.det = g.a * g.c - g.b * g.b;
     if (!(g.det > 0.0f)) return false;
 
+    if constexpr (ADJ) {
+        setViewDirection(m, cam, g);
+        g.shCount = shBasis(cam.shDegree, g.dir[0], g.dir[1], g.dir[2], g.basis);
+        g.rgbRaw = shAccumulate<PLANAR, EAGER_SH>(shBase, shStride, g.shCount, g.basis);
+        return true;
+    }
     const float mid = 0.5f * (g.a + g.c);
     const float disc = mid * mid - g.det;
     const float lambda = mid + sqrtf((0.1f < disc) ? disc : 0.1f);
@@ -235,13 +255,7 @@ __host__ __device__ inline bool finishProjection(const F3 m, float opacityIn, co
     g.x1 = min(cam.tilesX - 1, static_cast<int>(g.u + radius) >> cam.tileShiftX);
     g.y1 = min(cam.tilesY - 1, static_cast<int>(g.v + radius) >> cam.tileShiftY);
 
-    g.dir[0] = m.x - cam.eye[0];
-    g.dir[1] = m.y - cam.eye[1];
-    g.dir[2] = m.z - cam.eye[2];
-    g.dlen = sqrtf(g.dir[0] * g.dir[0] + g.dir[1] * g.dir[1] + g.dir[2] * g.dir[2]);
-    g.dir[0] = g.dir[0] / g.dlen;
-    g.dir[1] = g.dir[1] / g.dlen;
-    g.dir[2] = g.dir[2] / g.dlen;
+    setViewDirection(m, cam, g);
     g.shCount = shBasis(cam.shDegree, g.dir[0], g.dir[1], g.dir[2], g.basis);
     g.rgbRaw = shAccumulate<PLANAR, EAGER_SH>(shBase, shStride, g.shCount, g.basis);
     // std::max(0, v) semantics (NaN -> 0).
