@@ -243,10 +243,11 @@ __device__ inline void blendBackwardTile(int tileX, int tileY, uint2 range, cons
     // This pixel's contributions from the splat staged at byte offset off (position off / 16 of
     // the batch; zero if it does not blend there), as the moments the pair record is formed from:
     // d = (gPow dx, gPow dy, gPow dx dx, gPow dx dy, gPow dy dy, gPow, dL/drgb) with
-    // gPow = dL/dpower; steps T and g . accum back past the splat. True if it contributed.
+    // gPow = dL/dpower; steps T and g . accum back past the splat. Returns alpha if it
+    // contributed, else 0.
     // power and alpha are the blend's, bit for bit (same staged conic, same operations), so every
     // blend/skip decision matches the forward's; selects and zero weights replace branches.
-    auto visit = [&](uint32_t off, float (&d)[9]) -> bool {
+    auto visit = [&](uint32_t off, float (&d)[9]) -> float {
         const float4 a = row(0, off), bb = row(1, off), c = row(2, off);
         const float dx = px - a.x, dy = py - a.y;
         const float e = splatExponent(a.z, a.w, bb.x, bb.y, dx, dy);
@@ -276,7 +277,7 @@ __device__ inline void blendBackwardTile(int tileX, int tileY, uint2 range, cons
         d[5] = gPow;
         S = __builtin_fmaf(wa, diff, S);
         Tcur = use ? Ti : Tcur;
-        return use;
+        return wa;
     };
     for (int b = nbDone - 1; b >= 0; --b) {
         __syncthreads();
@@ -291,9 +292,11 @@ __device__ inline void blendBackwardTile(int tileX, int tileY, uint2 range, cons
             for (int i = count; i >= 1; i -= 2) {
                 const uint32_t o1 = sList[wave][i], o2 = sList[wave][i - 1];
                 float d1[9], d2[9];
-                const bool u1 = visit(o1, d1);
-                const bool u2 = visit(o2, d2);
-                if ((__builtin_amdgcn_ballot_w64(u1) | __builtin_amdgcn_ballot_w64(u2)) != 0)
+                const float w1 = visit(o1, d1);
+                const float w2 = visit(o2, d2);
+                // A contributing splat has alpha >= 1/255: the ballot of a fresh compare costs
+                // one instruction, that of the carried flag three.
+                if (__builtin_amdgcn_ballot_w64(w1 > 0.0f || w2 > 0.0f) != 0)
                     pairSumStore(d1, d2, &sGrad[wave][o1 >> 4][0], &sGrad[wave][o2 >> 4][0]);
             }
         }
