@@ -80,6 +80,9 @@ SIGNATURES = {
     "ans_activate": (None, [ctypes.c_int64, P, P, P, P, P, P]),
     "ans_render_views": (ctypes.c_int, [P, ctypes.c_int32, ctypes.POINTER(Camera_), ctypes.POINTER(Options_), P, P,
                                         ctypes.POINTER(Stats_)]),
+    "ans_render_views_async": (ctypes.c_int, [P, ctypes.c_int32, ctypes.POINTER(Camera_), ctypes.POINTER(Options_),
+                                              P, P]),
+    "ans_views_wait": (ctypes.c_int, [P, ctypes.POINTER(Stats_)]),
     "ans_views_autotune": (ctypes.c_int, [P, ctypes.c_int32, ctypes.POINTER(Camera_), ctypes.POINTER(Options_), P, P,
                                           ctypes.c_int32, ctypes.POINTER(Options_), ctypes.POINTER(ctypes.c_double)]),
     "ans_render_path": (ctypes.c_int, [P, ctypes.c_int64, ctypes.POINTER(Camera_), P, ctypes.POINTER(Options_),

@@ -73,55 +73,64 @@ void ViewBatchDeleter::operator()(ViewBatch* b) const { delete b; }
 
 namespace {
 
-template <class T>
+// Whether buffer b must grow to n elements; with APPLY it grows.
+template <bool APPLY, class T>
 bool grow(DevBuf<T>& b, size_t n) {
     if (n <= b.cap) return false;
-    b.reserve(n);
+    if (APPLY) b.reserve(n);
     return true;
+}
+
+// Whether any arena buffer must grow for the batch L; with APPLY they grow.
+template <bool APPLY>
+bool arenaGrowth(ViewBatch& B, const Layout& L) {
+    const size_t V = L.views, n = std::max<uint32_t>(L.n, 1), vn = V * n;
+    bool moved = false;
+    moved |= grow<APPLY>(B.table, TableLayout(L.views).bytes);
+    moved |= grow<APPLY>(B.ctl, V + 1);
+    moved |= grow<APPLY>(B.rec, vn);
+    moved |= grow<APPLY>(B.depthKey, vn);
+    moved |= grow<APPLY>(B.order, vn);
+    moved |= grow<APPLY>(B.kA, vn);
+    moved |= grow<APPLY>(B.vA, vn);
+    moved |= grow<APPLY>(B.kB, vn);
+    moved |= grow<APPLY>(B.vB, vn);
+    moved |= grow<APPLY>(B.ttByRank, vn);
+    moved |= grow<APPLY>(B.rectIdx, vn);
+    moved |= grow<APPLY>(B.rectByRank, vn);
+    const size_t histD = V * kRadix * size_t(L.blocksD), histP = V * kRadix * size_t(L.blocksP),
+                 histI = size_t(kRadix) * L.blocksI;
+    moved |= grow<APPLY>(B.hist, std::max({histD, histP, histI}));
+    moved |= grow<APPLY>(B.rowTotal, (V + 1) * kRadix);
+    moved |= grow<APPLY>(B.blockPairs, V * L.blocksDup);
+    moved |= grow<APPLY>(B.blockVisible, V * L.blocksDup);
+    const size_t vp = V * L.pairCap;
+    moved |= grow<APPLY>(B.tileA, vp);
+    moved |= grow<APPLY>(B.gaussA, vp);
+    moved |= grow<APPLY>(B.tileB, vp);
+    moved |= grow<APPLY>(B.gaussB, vp);
+    moved |= grow<APPLY>(B.ranges, size_t(L.items));
+    moved |= grow<APPLY>(B.itemKey, L.items);
+    moved |= grow<APPLY>(B.itemK2, L.items);
+    moved |= grow<APPLY>(B.itemV2, L.items);
+    moved |= grow<APPLY>(B.itemOrder, L.items);
+    if (L.diff) {
+        moved |= grow<APPLY>(B.tilesTouched, vn);
+        moved |= grow<APPLY>(B.pixState, std::max<uint64_t>(L.pixels, 1));
+    }
+    return moved;
 }
 
 }  // namespace
 
+bool arenaFits(ViewBatch& B, const Layout& L) { return !arenaGrowth<false>(B, L); }
+
 void reserveArena(ViewBatch& B, const Layout& L) {
-    const size_t V = L.views, n = std::max<uint32_t>(L.n, 1), vn = V * n;
-    bool moved = false;
-    moved |= grow(B.table, TableLayout(L.views).bytes);
-    moved |= grow(B.ctl, V + 1);
-    moved |= grow(B.rec, vn);
-    moved |= grow(B.depthKey, vn);
-    moved |= grow(B.order, vn);
-    moved |= grow(B.kA, vn);
-    moved |= grow(B.vA, vn);
-    moved |= grow(B.kB, vn);
-    moved |= grow(B.vB, vn);
-    moved |= grow(B.ttByRank, vn);
-    moved |= grow(B.rectIdx, vn);
-    moved |= grow(B.rectByRank, vn);
-    const size_t histD = V * kRadix * size_t(L.blocksD), histP = V * kRadix * size_t(L.blocksP),
-                 histI = size_t(kRadix) * L.blocksI;
-    moved |= grow(B.hist, std::max({histD, histP, histI}));
-    moved |= grow(B.rowTotal, (V + 1) * kRadix);
-    moved |= grow(B.blockPairs, V * L.blocksDup);
-    moved |= grow(B.blockVisible, V * L.blocksDup);
-    const size_t vp = V * L.pairCap;
-    moved |= grow(B.tileA, vp);
-    moved |= grow(B.gaussA, vp);
-    moved |= grow(B.tileB, vp);
-    moved |= grow(B.gaussB, vp);
-    moved |= grow(B.ranges, size_t(L.items));
-    moved |= grow(B.itemKey, L.items);
-    moved |= grow(B.itemK2, L.items);
-    moved |= grow(B.itemV2, L.items);
-    moved |= grow(B.itemOrder, L.items);
-    if (L.diff) {
-        moved |= grow(B.tilesTouched, vn);
-        moved |= grow(B.pixState, std::max<uint64_t>(L.pixels, 1));
-    }
-    if (moved) {
+    if (arenaGrowth<true>(B, L)) {
         B.dropGraphs();
         ++B.generation;
         // A fresh control array: zero it once (the per-call init leaves sticky alone).
-        hipCheck(hipMemset(B.ctl.p, 0, sizeof(FrameCtl) * (V + 1)), "hipMemset ctl");
+        hipCheck(hipMemset(B.ctl.p, 0, sizeof(FrameCtl) * (L.views + 1)), "hipMemset ctl");
     }
 }
 
@@ -408,12 +417,12 @@ ViewBatch& batchOf(ans_renderer* r) {
     return *r->batch;
 }
 
-void renderViews(ans_renderer* r, int32_t n_views, const ans_camera* cams, const ans_options* options,
-                 uint8_t* const* dev_rgb, ans_stats* stats, void* stream) {
+// Validates a batch's arguments and derives each view's constants.
+std::vector<CamParams> prepareViews(ans_renderer* r, int32_t n_views, const ans_camera* cams,
+                                    const ans_options& o, uint8_t* const* dev_rgb) {
     if (!r) throw std::invalid_argument("renderer is null");
     if (n_views < 1 || n_views > kMaxViews) throw std::invalid_argument("n_views must be in 1..256");
     if (!cams || !dev_rgb) throw std::invalid_argument("cameras and outputs must not be null");
-    const ans_options o = resolveOptions(options);
     std::vector<CamParams> cps(n_views);
     for (int v = 0; v < n_views; ++v) {
         checkCamera(&cams[v]);
@@ -421,7 +430,30 @@ void renderViews(ans_renderer* r, int32_t n_views, const ans_camera* cams, const
         cps[v] = makeCamParams(cams[v], o, r->scene.shDegree);
         if (cps[v].tilesX > 65535 || cps[v].tilesY > 65535) throw std::invalid_argument("too many tiles");
     }
+    return cps;
+}
+
+// Ends the ans_render_views_async batch in `slot`, if one is in flight; true if it overflowed.
+bool retireAsync(ViewBatch& B, int slot) {
+    if (!B.asyncSeq[slot]) return false;
+    B.asyncSeq[slot] = 0;
+    return finishBatch(B, B.asyncL[slot], B.asyncCps[slot], slot);
+}
+
+// Ends every ans_render_views_async batch in flight, oldest first, and keeps any overflow for the
+// next ans_views_wait.
+void drainAsync(ViewBatch& B) {
+    const int first = (B.asyncSeq[0] && (!B.asyncSeq[1] || B.asyncSeq[0] < B.asyncSeq[1])) ? 0 : 1;
+    B.asyncOverflow |= retireAsync(B, first);
+    B.asyncOverflow |= retireAsync(B, first ^ 1);
+}
+
+void renderViews(ans_renderer* r, int32_t n_views, const ans_camera* cams, const ans_options* options,
+                 uint8_t* const* dev_rgb, ans_stats* stats, void* stream) {
+    const ans_options o = resolveOptions(options);
+    const std::vector<CamParams> cps = prepareViews(r, n_views, cams, o, dev_rgb);
     ViewBatch& B = batchOf(r);
+    drainAsync(B);  // slot 0 and the arena are shared with asynchronous batches
     int retries = 0;
     while (true) {
         const Layout L = makeLayout(B, r->scene, n_views, cps, o);
@@ -447,6 +479,59 @@ void renderViews(ans_renderer* r, int32_t n_views, const ans_camera* cams, const
             st.reserved = 0;
         }
     }
+}
+
+// Enqueues one batch without waiting for it. Up to two batches are in flight, one per submission
+// slot (its pinned table and control-block copy): a third call first ends the oldest. All of them
+// share the arena, so a batch enqueued on another stream waits for the one in flight, and growth
+// (a larger batch shape, or pairs above the per-view capacity) waits for every batch in flight.
+void renderViewsAsync(ans_renderer* r, int32_t n_views, const ans_camera* cams, const ans_options* options,
+                      uint8_t* const* dev_rgb, void* stream) {
+    const ans_options o = resolveOptions(options);
+    std::vector<CamParams> cps = prepareViews(r, n_views, cams, o, dev_rgb);
+    ViewBatch& B = batchOf(r);
+    const int slot = !B.asyncSeq[0] ? 0 : !B.asyncSeq[1] ? 1 : (B.asyncSeq[0] < B.asyncSeq[1] ? 0 : 1);
+    B.asyncOverflow |= retireAsync(B, slot);
+    Layout L = makeLayout(B, r->scene, n_views, cps, o);
+    if (!arenaFits(B, L)) {
+        drainAsync(B);  // growth frees buffers the batch in flight reads
+        L = makeLayout(B, r->scene, n_views, cps, o);
+    }
+    reserveArena(B, L);
+    B.pairCap = L.pairCap;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int other = slot ^ 1;
+    if (B.asyncSeq[other]) hipCheck(hipStreamWaitEvent(st, B.ev[other][1], 0), "hipStreamWaitEvent");
+    BatchIO io;
+    io.outs = dev_rgb;
+    submitBatch(B, r->scene, L, cps, io, st, slot);
+    B.asyncSeq[slot] = B.asyncNext++;
+    B.asyncViews = n_views;
+    B.asyncL[slot] = L;
+    B.asyncCps[slot] = std::move(cps);
+}
+
+// Waits for every asynchronous batch; true if one of them overflowed since the last wait.
+bool waitViews(ans_renderer* r, ans_stats* stats) {
+    if (!r) throw std::invalid_argument("renderer is null");
+    ViewBatch& B = batchOf(r);
+    drainAsync(B);
+    const bool over = B.asyncOverflow;
+    B.asyncOverflow = false;
+    if (over) B.views = 0;  // a batch to render again leaves no lists to inspect
+    if (stats)
+        for (size_t v = 0; v < std::min(B.lastCtl.size(), static_cast<size_t>(B.asyncViews)); ++v) {
+            ans_stats& st = stats[v];
+            st.visible = B.lastCtl[v].visible;
+            st.tile_pairs = B.lastCtl[v].pairs;
+            st.project_ms = 0.0f;
+            st.sort_ms = 0.0f;
+            st.blend_ms = B.lastMs;
+            st.pair_capacity = static_cast<int64_t>(B.pairCap);
+            st.retries = 0;
+            st.reserved = 0;
+        }
+    return over;
 }
 
 // One chunk of a camera path: frames [first, first + count), their constants, and each frame's
@@ -512,6 +597,7 @@ void renderPath(ans_renderer* r, int64_t nFrames, const ans_camera* cams, const 
     }
 
     ViewBatch& B = batchOf(r);
+    drainAsync(B);  // the path's slots and the arena are shared with asynchronous batches
     if (!B.pathRender) {
         hipCheck(hipStreamCreateWithFlags(&B.pathRender, hipStreamNonBlocking), "hipStreamCreate");
         hipCheck(hipStreamCreateWithFlags(&B.pathCopy, hipStreamNonBlocking), "hipStreamCreate");
@@ -610,6 +696,18 @@ int ans_render_views(ans_renderer* r, int32_t n_views, const ans_camera* cams, c
         renderViews(r, n_views, cams, options, dev_rgb, stats, stream);
         return ANS_OK;
     });
+}
+
+int ans_render_views_async(ans_renderer* r, int32_t n_views, const ans_camera* cams, const ans_options* options,
+                           uint8_t* const* dev_rgb, void* stream) {
+    return guarded([&] {
+        renderViewsAsync(r, n_views, cams, options, dev_rgb, stream);
+        return ANS_OK;
+    });
+}
+
+int ans_views_wait(ans_renderer* r, ans_stats* stats) {
+    return guarded([&] { return waitViews(r, stats) ? ANS_ERR_CAPACITY : ANS_OK; });
 }
 
 int ans_views_autotune(ans_renderer* r, int32_t n_views, const ans_camera* cams, const ans_options* base,

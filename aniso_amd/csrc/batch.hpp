@@ -39,6 +39,20 @@ struct TableLayout {
     }
 };
 
+// Shapes of one batched call.
+struct Layout {
+    int views;
+    uint32_t n;
+    ans_options opts;
+    bool cull;
+    bool diff;        // differentiable: raw parameters, float images and pixel state
+    uint64_t pixels;  // all views' pixels (differentiable batches)
+    uint32_t maxTiles, totalTiles, items;
+    int tileBits, passes, digitBits;
+    uint32_t blocksD, blocksDup, blocksP, blocksI;
+    size_t pairCap;
+};
+
 struct ViewBatch {
     // Grow-only arena. Cached graphs bake its addresses, so growth of any buffer drops them.
     DevBuf<uint8_t> table;
@@ -77,6 +91,16 @@ struct ViewBatch {
         std::tuple<int, int64_t, int, int, int, bool, uint32_t, uint32_t, size_t, uint64_t, bool, uint64_t>;
     std::map<GraphKey, hipGraphExec_t> graphs;
 
+    // Batches of ans_render_views_async not yet waited for, per submission slot: the order they
+    // were submitted in (0: slot free), their shapes and cameras; and whether a batch finished
+    // since the last wait overflowed the arena.
+    uint64_t asyncSeq[2] = {};
+    uint64_t asyncNext = 1;
+    Layout asyncL[2] = {};
+    std::vector<CamParams> asyncCps[2];
+    bool asyncOverflow = false;
+    int asyncViews = 0;  // views of the last batch enqueued asynchronously
+
     // The last finished batch, for stats and dumps.
     int views = 0;
     uint32_t maxTiles = 0;
@@ -101,20 +125,6 @@ struct ViewBatch {
     void dropGraphs();
 };
 
-// Shapes of one batched call.
-struct Layout {
-    int views;
-    uint32_t n;
-    ans_options opts;
-    bool cull;
-    bool diff;        // differentiable: raw parameters, float images and pixel state
-    uint64_t pixels;  // all views' pixels (differentiable batches)
-    uint32_t maxTiles, totalTiles, items;
-    int tileBits, passes, digitBits;
-    uint32_t blocksD, blocksDup, blocksP, blocksI;
-    size_t pairCap;
-};
-
 // What one batched call writes and reads besides the scene: per view an RGB8 output (render) or a
 // float output (differentiable), and for differentiable calls the parameter arrays.
 struct BatchIO {
@@ -126,6 +136,8 @@ struct BatchIO {
 Layout makeLayout(const ViewBatch& B, const SceneDev& s, int views, const std::vector<CamParams>& cps,
                   const ans_options& o, bool diff = false);
 void reserveArena(ViewBatch& B, const Layout& L);
+// Whether the arena already holds the batch L (reserveArena would not move any buffer).
+bool arenaFits(ViewBatch& B, const Layout& L);
 // Enqueues the whole batch on st (kernel launches only, so it can be captured). Returns the
 // buffer holding the per-view sorted Gaussian lists.
 const uint32_t* enqueueBatch(ViewBatch& B, const SceneDev& s, const Layout& L, hipStream_t st);

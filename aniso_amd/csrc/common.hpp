@@ -88,7 +88,8 @@ __device__ inline uint64_t lanesBelow() {
 // DPP lane moves within a 64-lane wavefront (GFX9 controls): row_shr:n gives lane l of each
 // 16-lane row lane l - n of the same row; row_bcast:15 gives every lane of rows 1 and 3 (row mask
 // 0xa) lane 15 of the row below; row_bcast:31 gives rows 2 and 3 (row mask 0xc) lane 31;
-// wave_shl:1 gives lane l lane l + 1. Lanes without a source, or in masked rows, get `old`.
+// wave_shl:1 gives lane l lane l + 1, wave_shr:1 lane l - 1. Lanes without a source, or in masked
+// rows, get `old`.
 // These run on the vector ALU; lane shuffles through the LDS crossbar (ds_bpermute) cost a
 // round trip each.
 enum : int {
@@ -97,6 +98,7 @@ enum : int {
     kDppRowShr4 = 0x114,
     kDppRowShr8 = 0x118,
     kDppWaveShl1 = 0x130,
+    kDppWaveShr1 = 0x138,
     kDppRowBcast15 = 0x142,
     kDppRowBcast31 = 0x143,
 };

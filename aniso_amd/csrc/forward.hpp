@@ -605,7 +605,9 @@ __device__ inline void writeRectPairs(uint32_t first, uint32_t g, uint32_t rectL
 }
 
 // Start and end of each tile's run in the sorted pairs (gpu_render.cu:164-177); ranges were
-// cleared to [0, 0) by the preprocess. Grid-stride over groups of four pairs.
+// cleared to [0, 0) by the preprocess. Grid-stride, one pair per lane per step: each lane loads its
+// own pair (coalesced across the wavefront) and takes its neighbours' from the adjacent lanes by
+// DPP; the wavefront's first and last lanes load the pairs beyond its ends.
 [[maybe_unused]] static __global__ __launch_bounds__(256) void k_tileRanges(const uint32_t* __restrict__ sortedTile,
                                                            const FrameCtl* __restrict__ ctl,
                                                            uint2* __restrict__ ranges, uint32_t pairStride,
@@ -615,21 +617,20 @@ __device__ inline void writeRectPairs(uint32_t first, uint32_t g, uint32_t rectL
     ranges += static_cast<size_t>(blockIdx.y) * tileStride;
     if (ctl->overflow) return;
     const uint32_t total = ctl->pairs;
-    for (uint32_t p0 = (blockIdx.x * 256 + threadIdx.x) * 4; p0 < total; p0 += gridDim.x * 256 * 4) {
-        // Pairs p0 .. p0 + 3 and their neighbours, all loads issued before any comparison.
-        uint32_t t[6];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            const uint32_t p = p0 + k - 1;  // p0 - 1 wraps for p0 = 0 and is not read
-            t[k] = (k > 0 || p0 > 0) && p < total ? sortedTile[p] : 0xffffffffu;
-        }
-#pragma unroll
-        for (int k = 1; k <= 4; ++k) {
-            const uint32_t p = p0 + k - 1;
-            if (p < total) {
-                if (p == 0 || t[k - 1] != t[k]) ranges[t[k]].x = p;
-                if (p == total - 1 || t[k + 1] != t[k]) ranges[t[k]].y = p + 1;
-            }
+    const uint32_t lane = __lane_id();
+    constexpr uint32_t kNone = 0xffffffffu;
+    // Wavefront-uniform loop (the DPP moves need every lane): it runs while the wavefront's first
+    // pair exists.
+    for (uint32_t p = blockIdx.x * 256 + threadIdx.x; p - lane < total; p += gridDim.x * 256) {
+        const uint32_t t = p < total ? sortedTile[p] : kNone;
+        uint32_t before = kNone, after = kNone;
+        if (lane == 0 && p > 0) before = sortedTile[p - 1];
+        if (lane == kWave - 1 && p + 1 < total) after = sortedTile[p + 1];
+        const uint32_t prev = static_cast<uint32_t>(dppMove<kDppWaveShr1>(static_cast<int>(before), static_cast<int>(t)));
+        const uint32_t next = static_cast<uint32_t>(dppMove<kDppWaveShl1>(static_cast<int>(after), static_cast<int>(t)));
+        if (p < total) {
+            if (p == 0 || prev != t) ranges[t].x = p;
+            if (p == total - 1 || next != t) ranges[t].y = p + 1;
         }
     }
 }

@@ -186,6 +186,35 @@ class GpuRenderer:
             for s_ in st:
                 stats.append(RenderStats(**{f.name: getattr(s_, f.name) for f in dataclasses.fields(RenderStats)}))
 
+    def render_views_async(self, cameras, outputs, options: RenderOptions | None = None):
+        """Enqueues one batched pass (ans_render_views_async) on torch's current stream and returns;
+        views_wait() waits for every batch enqueued so. The output buffers must stay alive until
+        then."""
+        k = len(cameras)
+        if len(outputs) != k:
+            raise ValueError("one output per camera")
+        cams = (_lib.Camera_ * k)(*[camera_c(c) for c in cameras])
+        ptrs = (ctypes.c_void_p * k)(*[o if isinstance(o, int) else o.data_ptr() for o in outputs])
+        opts = (options or RenderOptions()).c()
+        _lib.check(_lib.load().ans_render_views_async(self._h, k, cams, ctypes.byref(opts), ptrs, _torch_stream()))
+        self._async_views = k
+
+    def views_wait(self, stats: list | None = None) -> bool:
+        """Waits for every batch of render_views_async. Returns False if a view of any of them had
+        more pairs than the pair arena held (ANS_ERR_CAPACITY: the arena has grown; render those
+        batches again), True otherwise. Fills `stats` (a list) with the last batch's RenderStats."""
+        n_views = getattr(self, "_async_views", 0)
+        st = (_lib.Stats_ * max(1, n_views))()
+        rc = _lib.load().ans_views_wait(self._h, st if n_views > 0 else None)
+        if rc == 3:  # ANS_ERR_CAPACITY
+            return False
+        _lib.check(rc)
+        if stats is not None and n_views > 0:
+            stats.clear()
+            for s_ in st:
+                stats.append(RenderStats(**{f.name: getattr(s_, f.name) for f in dataclasses.fields(RenderStats)}))
+        return True
+
     def autotune(self, camera: Camera, frame, base: RenderOptions | None = None, repeats: int = 3):
         """(best RenderOptions, device ms per frame) over tile shapes x LDS batch depths for single
         frames of `camera`, rendered into the device buffer `frame` (ans_render_autotune)."""

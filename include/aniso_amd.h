@@ -166,6 +166,22 @@ int ans_frame_dump(ans_renderer* r, uint32_t* splat_bits, int32_t* rect, uint64_
 int ans_render_views(ans_renderer* r, int32_t n_views, const ans_camera* cams, const ans_options* options,
                      uint8_t* const* dev_rgb, void* stream, ans_stats* stats);
 
+/* ans_render_views without the wait, for callers that render batch after batch (a training or
+ * serving loop): the batch is enqueued on `stream` and the call returns. Two batches may be in
+ * flight; a third call first waits for the oldest. A batch waits on the device for the batch in
+ * flight (they share the pair arena); a batch that needs a larger arena waits on the host for
+ * every batch in flight first. Pairs above the arena are not rendered again here: see
+ * ans_views_wait. The async counterpart of ans_render_views as ans_render_async is of ans_render
+ * (the reference renders one view per synchronous GpuRenderer::render, gpu_render.h:30). */
+int ans_render_views_async(ans_renderer* r, int32_t n_views, const ans_camera* cams, const ans_options* options,
+                           uint8_t* const* dev_rgb, void* stream);
+
+/* Waits for every batch of ans_render_views_async. Returns ANS_ERR_CAPACITY if a view of any of
+ * them had more pairs than the arena held: the arena has then been grown and the caller renders
+ * those batches again. stats: NULL or one entry per view of the last batch enqueued by
+ * ans_render_views_async (as ans_render_views fills them, retries = 0). */
+int ans_views_wait(ans_renderer* r, ans_stats* stats);
+
 /* Tries every tile shape (8x8, 16x16, 32x16) and LDS batch depth (64, 128, 256) on single frames
  * of this camera (sh_degree, splat_scale and flags from base), each rendered into the DEVICE
  * buffer dev_rgb and waited for, and returns the fastest options in *best and their device time

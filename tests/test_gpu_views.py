@@ -152,3 +152,58 @@ def test_autotune_picks_a_supported_config(r):
     print(f"autotune: tile {best.tile} batch {best.batch_depth}: {ms:.3f} ms", flush=True)
     assert best.tile in TILES and best.batch_depth in (64, 128, 256) and ms > 0
     check_batch_against_single(r, cams, best, "autotuned")
+
+
+def test_async_batches_equal_sync_batches(r):
+    """ans_render_views_async: three batches in a row (two in flight at once), one of another
+    shape and one on a second stream; after ans_views_wait each batch's images equal the
+    synchronous ans_render_views of the same cameras."""
+    opts = RenderOptions(tile=(16, 16), batch_depth=128)
+    batches = [small_cameras(11), small_cameras(12), small_cameras(13, count=3), small_cameras(14)]
+    print(f"case async: {len(batches)} batches", flush=True)
+    outs = [device_outputs(c) for c in batches]
+    side = torch.cuda.Stream()
+    for k, cams in enumerate(batches):
+        if k == 3:
+            with torch.cuda.stream(side):
+                r.render_views_async(cams, outs[k], opts)
+        else:
+            r.render_views_async(cams, outs[k], opts)
+    stats = []
+    assert r.views_wait(stats)
+    torch.cuda.synchronize()
+    assert len(stats) == len(batches[-1])
+    for k, cams in enumerate(batches):
+        ref = device_outputs(cams)
+        ref_stats = []
+        r.render_views(cams, ref, opts, ref_stats)
+        torch.cuda.synchronize()
+        for v in range(len(cams)):
+            assert torch.equal(outs[k][v], ref[v]), f"batch {k} view {v} differs"
+        if k == len(batches) - 1:
+            assert [s.tile_pairs for s in stats] == [s.tile_pairs for s in ref_stats]
+
+
+def test_async_batch_overflow_reported_then_renders(gpu):
+    """An asynchronous batch whose pairs exceed the first arena: ans_views_wait reports
+    ANS_ERR_CAPACITY with the arena grown, and the same batch enqueued again renders equal to
+    the synchronous batch."""
+    sc = scenes.synthetic_scene(30_000, 640, 360, seed=9)
+    sc.log_scales[:] += 2.0  # about 2.1M pairs in one view at 8x8 tiles
+    cams = small_cameras(7, count=3)
+    opts = RenderOptions(tile=(8, 8))
+    print("case async overflow", flush=True)
+    with GpuRenderer(sc) as rr:
+        outs = device_outputs(cams)
+        rr.render_views_async(cams, outs, opts)
+        assert not rr.views_wait()
+        rr.render_views_async(cams, outs, opts)
+        stats = []
+        assert rr.views_wait(stats)
+        assert max(s.tile_pairs for s in stats) > 2 ** 20
+        assert all(s.pair_capacity >= s.tile_pairs for s in stats)
+        ref = device_outputs(cams)
+        rr.render_views(cams, ref, opts)
+        torch.cuda.synchronize()
+        for v in range(len(cams)):
+            assert torch.equal(outs[v], ref[v]), f"view {v} differs"
