@@ -30,7 +30,7 @@ struct CamParams {
 // the blend, its adjoint and the pair expansion each touch a single line.
 struct alignas(64) SplatRec {
     float4 geo;   // u, v (pixels), conic A, B
-    float4 geo2;  // conic C, opacity, alpha-region half-extents x, y
+    float4 geo2;  // conic C, opacity, 0, 0 (the blend culls from the conic and opacity)
     float4 col;   // r, g, b, depth
     uint4 meta;   // tile rect (x0 | y0 << 16, x1 | y1 << 16), tiles touched (0: culled), radius (px)
 };

@@ -297,19 +297,10 @@ __device__ inline void storeCulled(size_t o, SplatRec* __restrict__ rec, uint32_
 __device__ inline void storeSplat(const GaussProj& g, size_t o, SplatRec* __restrict__ rec,
                                   uint32_t* __restrict__ tilesTouched, uint32_t* __restrict__ depthKey,
                                   uint2* __restrict__ rect) {
-    // Region where op * exp(-q/2) >= 1/255: q <= 2 ln(255 op); half-extents sqrt(k a), sqrt(k c)
-    // of that ellipse, widened by a 5% + 0.1 margin far above the FP32 rounding of the blend's
-    // exponent.
-    float ex = -1e30f, ey = -1e30f;
-    if (g.opacity >= 1.0f / 255.0f) {
-        const float k = fmaxf(2.0f * logf(255.0f * g.opacity), 0.0f) * 1.05f + 0.1f;
-        ex = sqrtf(k * g.a) + 0.01f;
-        ey = sqrtf(k * g.c) + 0.01f;
-    }
     const uint32_t tt = static_cast<uint32_t>((g.x1 - g.x0 + 1) * (g.y1 - g.y0 + 1));
     SplatRec& r = rec[o];
     r.geo = make_float4(g.u, g.v, g.A, g.B);
-    r.geo2 = make_float4(g.C, g.opacity, ex, ey);
+    r.geo2 = make_float4(g.C, g.opacity, 0.0f, 0.0f);
     r.col = make_float4(g.rgb.x, g.rgb.y, g.rgb.z, g.depth);
     const uint2 rp = make_uint2(static_cast<uint32_t>(g.x0) | (static_cast<uint32_t>(g.y0) << 16),
                                 static_cast<uint32_t>(g.x1) | (static_cast<uint32_t>(g.y1) << 16));
@@ -320,10 +311,8 @@ __device__ inline void storeSplat(const GaussProj& g, size_t o, SplatRec* __rest
 }
 
 // Per-Gaussian preprocess, one lane per Gaussian. Writes the splat record
-// {u, v, A, B | C, opacity, ext_x, ext_y | r, g, b, depth}, the packed tile rect, the tile count and
-// the depth sort key (0xffffffff for culled Gaussians). ext_x/y are conservative half-extents of
-// the region where alpha can reach 1/255, used by the blend to skip work that cannot change a
-// pixel.
+// {u, v, A, B | C, opacity, 0, 0 | r, g, b, depth | rect, tiles, radius}, the packed tile rect, the
+// tile count and the depth sort key (0xffffffff for culled Gaussians).
 template <bool PLANAR, bool RAW>
 __global__ __launch_bounds__(256) void k_preprocess(int n, const float* __restrict__ pos,
                                                     const float* __restrict__ scale, const float4* __restrict__ rot,

@@ -80,9 +80,12 @@ namespace ans {
     for (int j = 0; j < 48; ++j) shr[j] = j < K3 ? src[j] : 0.0f;
     GaussProj g;
     prepareGaussian<true>(P.rot[i], reinterpret_cast<const F3*>(P.scale)[i], cams[0].splatScale, g, P.act != 0);
+    // The opacity is the same in every view: activated once here (as the single-view
+    // finishProjection activates it), then passed on as an activated value.
+    const float opAct = P.act ? op : 1.0f / (1.0f + expf(-op));
     for (int v = 0; v < views; ++v) {
         const size_t o = static_cast<size_t>(v) * n + i;
-        if (!cameraTransform(m, cams[v], g) || !finishProjection<false, true>(m, op, shr, 1, cams[v], g)) {
+        if (!cameraTransform(m, cams[v], g) || !finishProjection<false, false>(m, opAct, shr, 1, cams[v], g)) {
             storeCulled(o, rec, tilesTouched, depthKey, rect);
             continue;
         }
