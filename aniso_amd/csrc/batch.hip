@@ -194,12 +194,15 @@ const uint32_t* enqueueBatch(ViewBatch& B, const SceneDev& s, const Layout& L, h
         uint32_t* kout[4] = {B.kA.p, B.kB.p, B.kA.p, nullptr};
         uint32_t* vout[4] = {B.vA.p, B.vB.p, B.vA.p, B.order.p};
         const dim3 gD(L.blocksD, V);
+        // The first pass drops the culled Gaussians (key 0xffffffff: a quarter of the
+        // Gaussian-views of a typical batch), so the later passes sort only the visible ones.
         for (int pass = 0; pass < 4; ++pass) {
-            k_radixUpsweep<kPairItems><<<gD, kSortBlock, 0, st>>>(kin, false, n, n, 8 * pass, 8, B.hist.p, L.blocksD,
+            const int from = pass == 0 ? kCountDrop : kCountKept;
+            k_radixUpsweep<kPairItems><<<gD, kSortBlock, 0, st>>>(kin, from, n, n, 8 * pass, 8, B.hist.p, L.blocksD,
                                                                   ctl, n, histD);
             k_scanRows<<<dim3(kRadix, V), kSortBlock, 0, st>>>(B.hist.p, L.blocksD, L.blocksD, B.rowTotal.p, ctl,
-                                                               false, n, kSortBlock * kPairItems, histD);
-            k_radixDownsweep<kPairItems><<<gD, kSortBlock, 0, st>>>(kin, vin, kout[pass], vout[pass], false, n, n,
+                                                               from, n, kSortBlock * kPairItems, histD);
+            k_radixDownsweep<kPairItems><<<gD, kSortBlock, 0, st>>>(kin, vin, kout[pass], vout[pass], from, n, n,
                                                                      8 * pass, 8, B.hist.p, B.rowTotal.p, L.blocksD,
                                                                      ctl, n, histD);
             kin = kout[pass];
@@ -208,7 +211,7 @@ const uint32_t* enqueueBatch(ViewBatch& B, const SceneDev& s, const Layout& L, h
         // Pairs of each view in depth order, into the view's region of the pair arena.
         const dim3 gDup(L.blocksDup, V);
         k_pairTotals<<<gDup, kDupBlock, 0, st>>>(B.order.p, B.rectIdx.p, n, B.ttByRank.p, B.rectByRank.p, B.blockPairs.p,
-                                                 B.blockVisible.p, n, L.blocksDup, nullptr);
+                                                 B.blockVisible.p, n, L.blocksDup, nullptr, ctl);
         k_scanPairs<<<dim3(1, V), kScanBlock, 0, st>>>(B.blockPairs.p, B.blockVisible.p, L.blocksDup, cap, ctl,
                                                       L.blocksDup, L.diff ? B.mirrorDev : nullptr, 1u);
         k_expandPairs<<<gDup, kDupBlock, 0, st>>>(B.order.p, B.ttByRank.p, B.rectByRank.p, n, 0, B.blockPairs.p,

@@ -44,7 +44,8 @@ struct FrameCtl {
     uint32_t overflow;    // 1 if pairs exceeded the pair buffers' capacity
     uint32_t sticky;      // overflow of any frame since the host last cleared it (not cleared by init)
     uint32_t finished;    // blend workgroups that have ended (stage timing)
-    uint32_t pad[2];
+    uint32_t sortCount;   // keys kept by a depth-sort pass that dropped culled Gaussians
+    uint32_t pad;
 };
 
 // A frame's pair count as the pair-count scan publishes it to pinned host memory: the fields,

@@ -360,7 +360,8 @@ constexpr uint32_t kBigPairs = 32;
 // Pairs and visible Gaussians per block of 256 depth ranks. Also gathers each rank's tile rect
 // (8 bytes of the compact rect array) into depth order with its tile count (ttByRank,
 // rectByRank; rect left unwritten for culled ranks), so that the expansion reads them with
-// coalesced loads.
+// coalesced loads. keptCtl (or null): the depth sort dropped the culled Gaussians and kept
+// keptCtl[view].sortCount ranks; the ranks after them count as culled.
 [[maybe_unused]] static __global__ __launch_bounds__(kDupBlock) void k_pairTotals(const uint32_t* __restrict__ sortedIdx,
                                                                  const uint2* __restrict__ rect, uint32_t n,
                                                                  uint32_t* __restrict__ ttByRank,
@@ -368,7 +369,8 @@ constexpr uint32_t kBigPairs = 32;
                                                                  uint32_t* __restrict__ blockPairs,
                                                                  uint32_t* __restrict__ blockVisible,
                                                                  uint32_t gaussStride, uint32_t blockStride,
-                                                                 FrameStamps* __restrict__ stamps) {
+                                                                 FrameStamps* __restrict__ stamps,
+                                                                 const FrameCtl* __restrict__ keptCtl) {
     __shared__ uint32_t sTmp[kDupBlock / kWave];
     if (stamps && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) stamps->t[1] = wall_clock64();
     const size_t go = static_cast<size_t>(blockIdx.y) * gaussStride;
@@ -379,12 +381,15 @@ constexpr uint32_t kBigPairs = 32;
     blockPairs += static_cast<size_t>(blockIdx.y) * blockStride;
     blockVisible += static_cast<size_t>(blockIdx.y) * blockStride;
     const uint32_t r = blockIdx.x * kDupTile + threadIdx.x;
+    const uint32_t kept = keptCtl ? min(n, keptCtl[blockIdx.y].sortCount) : n;
     uint32_t c = 0;
-    if (r < n) {
+    if (r < kept) {
         const uint2 rp = rect[sortedIdx[r]];
         c = rectTileCount(rp);
         ttByRank[r] = c;
         if (c) rectByRank[r] = rp;
+    } else if (r < n) {
+        ttByRank[r] = 0;
     }
     uint32_t total;
     blockExclusiveScan<kDupBlock>(c, sTmp, &total);

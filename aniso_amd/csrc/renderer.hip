@@ -191,7 +191,7 @@ void enqueueCountKernels(const FrameBuffers& b, const SceneDev& s, const CamPara
         }
         const uint32_t blocksDup = (n + kDupTile - 1) / kDupTile;
         k_pairTotals<<<blocksDup, kDupBlock, 0, stream>>>(b.order, b.rectIdx, n, b.ttByRank, b.rectByRank, b.blockPairs,
-                                                          b.blockVisible, 0, 0, S);
+                                                          b.blockVisible, 0, 0, S, nullptr);
         k_scanPairs<<<1, kScanBlock, 0, stream>>>(b.blockPairs, b.blockVisible, blocksDup,
                                                  static_cast<uint32_t>(b.pairCap), b.ctl, 0, mirror, seq);
     }

@@ -71,12 +71,23 @@ __device__ inline uint64_t matchDigit(uint32_t digit, int bits, bool valid) {
     return peers;
 }
 
+// Where a sort pass takes its item count (countFrom; a bool argument means host or pairs):
+//   kCountHost    nHost;
+//   kCountPairs   the frame's pair count (ctl->pairs, at most cap), or skip the pass if the pair
+//                 buffers overflowed;
+//   kCountKept    ctl->sortCount (at most nHost): the keys an earlier kCountDrop pass kept;
+//   kCountDrop    nHost, dropping keys equal to 0xffffffff (culled Gaussians: they would sort
+//                 last) — the pass writes only the kept keys, in order, and publishes their
+//                 number in ctl->sortCount.
+enum : int { kCountHost = 0, kCountPairs = 1, kCountKept = 2, kCountDrop = 3 };
+
 // The number of items to sort, or -1 if this sort is skipped (the pair buffers overflowed). Only
-// a sort of pairs (countFromCtl) reads the control block: a sort with a host-given count runs
-// before anything of its frame can set the overflow flag, and a load it waited for would delay
-// every other load of the workgroup by a round trip.
-__device__ inline int64_t itemCount(const FrameCtl* ctl, bool countFromCtl, uint32_t nHost, uint32_t cap) {
-    if (!countFromCtl) return nHost;
+// sorts that need it read the control block: a sort with a host-given count runs before anything
+// of its frame can set the overflow flag, and a load it waited for would delay every other load
+// of the workgroup by a round trip.
+__device__ inline int64_t itemCount(const FrameCtl* ctl, int countFrom, uint32_t nHost, uint32_t cap) {
+    if (countFrom == kCountHost || countFrom == kCountDrop) return nHost;
+    if (countFrom == kCountKept) return min(ctl->sortCount, nHost);
     if (ctl->overflow) return -1;
     return min(ctl->pairs, cap);
 }
@@ -84,7 +95,7 @@ __device__ inline int64_t itemCount(const FrameCtl* ctl, bool countFromCtl, uint
 // Per-block histogram of bits [shift, shift + bits) into hist[bin * stride + block], for the
 // blocks that hold items.
 template <int ITEMS>
-__global__ __launch_bounds__(kSortBlock) void k_radixUpsweep(const uint32_t* __restrict__ keys, bool countFromCtl,
+__global__ __launch_bounds__(kSortBlock) void k_radixUpsweep(const uint32_t* __restrict__ keys, int countFrom,
                                                              uint32_t nHost, uint32_t cap, int shift, int bits,
                                                              uint32_t* __restrict__ hist, uint32_t stride,
                                                              const FrameCtl* __restrict__ ctl, uint32_t itemStride,
@@ -98,9 +109,10 @@ __global__ __launch_bounds__(kSortBlock) void k_radixUpsweep(const uint32_t* __r
     __shared__ uint32_t h[kCopies * kRow];
     const uint32_t view = blockIdx.y;
     ctl += view;
-    const int64_t count = itemCount(ctl, countFromCtl, nHost, cap);
+    const int64_t count = itemCount(ctl, countFrom, nHost, cap);
     if (count < 0) return;
     const uint32_t n = static_cast<uint32_t>(count);
+    const bool drop = countFrom == kCountDrop;
     keys += static_cast<size_t>(view) * itemStride;
     hist += static_cast<size_t>(view) * histStride;
     const uint32_t base = blockIdx.x * (kSortBlock * ITEMS);
@@ -119,7 +131,8 @@ __global__ __launch_bounds__(kSortBlock) void k_radixUpsweep(const uint32_t* __r
         uint32_t* mine = h + (__lane_id() % kCopies) * kRow;
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j)
-            if (base + j * kSortBlock + tid < n) atomicAdd(&mine[(key[j] >> shift) & mask], 1u);
+            if (base + j * kSortBlock + tid < n && !(drop && key[j] == 0xffffffffu))
+                atomicAdd(&mine[(key[j] >> shift) & mask], 1u);
     }
     __syncthreads();
     for (int i = tid; i < radix; i += kSortBlock) {
@@ -134,17 +147,18 @@ __global__ __launch_bounds__(kSortBlock) void k_radixUpsweep(const uint32_t* __r
 // (blockIdx.x = bin); the row total goes to rowTotal[bin]. tileItems: items per sort block.
 [[maybe_unused]] static __global__ __launch_bounds__(kSortBlock) void k_scanRows(uint32_t* __restrict__ hist, uint32_t stride,
                                                                 uint32_t len, uint32_t* __restrict__ rowTotal,
-                                                                const FrameCtl* __restrict__ ctl, bool countFromCtl,
+                                                                const FrameCtl* __restrict__ ctl, int countFrom,
                                                                 uint32_t cap, uint32_t tileItems,
                                                                 uint32_t histStride) {
     __shared__ uint32_t sTmp[kSortWaves];
     const uint32_t view = blockIdx.y;
     ctl += view;
-    const int64_t count = itemCount(ctl, countFromCtl, 0, cap);
+    const int64_t count = itemCount(ctl, countFrom, cap, cap);
     if (count < 0) return;
     hist += static_cast<size_t>(view) * histStride;
     rowTotal += view * kRadix;
-    if (countFromCtl) len = max(1u, min(len, (static_cast<uint32_t>(count) + tileItems - 1) / tileItems));
+    if (countFrom == kCountPairs || countFrom == kCountKept)
+        len = max(1u, min(len, (static_cast<uint32_t>(count) + tileItems - 1) / tileItems));
     constexpr int kPer = 8;
     uint32_t* row = hist + static_cast<size_t>(blockIdx.x) * stride;
     uint32_t carry = 0;
@@ -175,9 +189,9 @@ __global__ __launch_bounds__(kSortBlock) void k_radixUpsweep(const uint32_t* __r
 template <int ITEMS>
 __global__ __launch_bounds__(kSortBlock) void k_radixDownsweep(
     const uint32_t* __restrict__ keysIn, const uint32_t* __restrict__ valsIn, uint32_t* __restrict__ keysOut,
-    uint32_t* __restrict__ valsOut, bool countFromCtl, uint32_t nHost, uint32_t cap, int shift, int bits,
+    uint32_t* __restrict__ valsOut, int countFrom, uint32_t nHost, uint32_t cap, int shift, int bits,
     const uint32_t* __restrict__ hist, const uint32_t* __restrict__ rowTotal, uint32_t stride,
-    const FrameCtl* __restrict__ ctl, uint32_t itemStride, uint32_t histStride) {
+    FrameCtl* __restrict__ ctl, uint32_t itemStride, uint32_t histStride) {
     constexpr int kTile = kSortBlock * ITEMS;
     __shared__ uint32_t sWaveHist[kSortWaves][kRadix];
     __shared__ uint32_t sBinStart[kRadix];
@@ -198,9 +212,10 @@ __global__ __launch_bounds__(kSortBlock) void k_radixDownsweep(
         rowPrefix = hist[static_cast<size_t>(view) * histStride + static_cast<size_t>(tid) * stride + blockIdx.x];
     }
     ctl += view;
-    const int64_t count = itemCount(ctl, countFromCtl, nHost, cap);
+    const int64_t count = itemCount(ctl, countFrom, nHost, cap);
     if (count < 0) return;
     const uint32_t n = static_cast<uint32_t>(count);
+    const bool drop = countFrom == kCountDrop;
     const uint32_t base = blockIdx.x * kTile;
     if (base >= n) return;
     {
@@ -229,7 +244,7 @@ __global__ __launch_bounds__(kSortBlock) void k_radixDownsweep(
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
         const uint32_t idx = base + (wave * ITEMS + j) * kWave + lane;
-        const bool valid = idx < n;
+        const bool valid = idx < n && !(drop && key[j] == 0xffffffffu);
         const uint32_t digit = (key[j] >> shift) & mask;
         const uint64_t peers = matchDigit<8>(digit, bits, valid);
         uint32_t r = 0;
@@ -258,12 +273,13 @@ __global__ __launch_bounds__(kSortBlock) void k_radixDownsweep(
     const uint32_t digitStart = blockExclusiveScan<kSortBlock>(digitTotal, sScan, &allTotal);
     sBinStart[b] = binStart;
     if (live) sDst[b] = digitStart + rowPrefix - binStart;
+    if (drop && blockIdx.x == 0 && tid == 0) ctl[0].sortCount = allTotal;  // ctl is this view's
     __syncthreads();
 
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
         const uint32_t idx = base + (wave * ITEMS + j) * kWave + lane;
-        if (idx < n) {
+        if (idx < n && !(drop && key[j] == 0xffffffffu)) {
             const uint32_t digit = (key[j] >> shift) & mask;
             const uint32_t pos = sBinStart[digit] + sWaveHist[wave][digit] + rank[j];
             sKeys[pos] = key[j];
@@ -271,7 +287,7 @@ __global__ __launch_bounds__(kSortBlock) void k_radixDownsweep(
         }
     }
     __syncthreads();
-    const uint32_t inBlock = min(static_cast<uint32_t>(kTile), n - base);
+    const uint32_t inBlock = blockTotal;  // the block's items, less any dropped
     for (uint32_t i = tid; i < inBlock; i += kSortBlock) {
         const uint32_t k = sKeys[i];
         const uint32_t dst = sDst[(k >> shift) & mask] + i;
