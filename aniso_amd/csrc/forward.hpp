@@ -599,9 +599,10 @@ __device__ inline void writeRectPairs(uint32_t first, uint32_t g, uint32_t rectL
 }
 
 // Start and end of each tile's run in the sorted pairs (gpu_render.cu:164-177); ranges were
-// cleared to [0, 0) by the preprocess. Grid-stride, one pair per lane per step: each lane loads its
-// own pair (coalesced across the wavefront) and takes its neighbours' from the adjacent lanes by
-// DPP; the wavefront's first and last lanes load the pairs beyond its ends.
+// cleared to [0, 0) by the preprocess. Grid-stride over runs of 256 pairs per wavefront: lane l
+// loads pairs l, l + 64, l + 128, l + 192 of the run (each load coalesced across the wavefront)
+// and takes its neighbours' tiles from the adjacent lanes by DPP; the run's inner edges come from
+// lane 0 or 63 of the neighbouring load (readlane), its outer edges from one load each.
 [[maybe_unused]] static __global__ __launch_bounds__(256) void k_tileRanges(const uint32_t* __restrict__ sortedTile,
                                                            const FrameCtl* __restrict__ ctl,
                                                            uint2* __restrict__ ranges, uint32_t pairStride,
@@ -613,18 +614,29 @@ __device__ inline void writeRectPairs(uint32_t first, uint32_t g, uint32_t rectL
     const uint32_t total = ctl->pairs;
     const uint32_t lane = __lane_id();
     constexpr uint32_t kNone = 0xffffffffu;
-    // Wavefront-uniform loop (the DPP moves need every lane): it runs while the wavefront's first
-    // pair exists.
-    for (uint32_t p = blockIdx.x * 256 + threadIdx.x; p - lane < total; p += gridDim.x * 256) {
-        const uint32_t t = p < total ? sortedTile[p] : kNone;
-        uint32_t before = kNone, after = kNone;
-        if (lane == 0 && p > 0) before = sortedTile[p - 1];
-        if (lane == kWave - 1 && p + 1 < total) after = sortedTile[p + 1];
-        const uint32_t prev = static_cast<uint32_t>(dppMove<kDppWaveShr1>(static_cast<int>(before), static_cast<int>(t)));
-        const uint32_t next = static_cast<uint32_t>(dppMove<kDppWaveShl1>(static_cast<int>(after), static_cast<int>(t)));
-        if (p < total) {
-            if (p == 0 || prev != t) ranges[t].x = p;
-            if (p == total - 1 || next != t) ranges[t].y = p + 1;
+    constexpr uint32_t kRun = 4 * kWave;
+    // Wavefront-uniform loop (the DPP moves need every lane).
+    for (uint32_t base = (blockIdx.x * 4 + threadIdx.x / kWave) * kRun; base < total;
+         base += gridDim.x * 4 * kRun) {
+        uint32_t t[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t p = base + k * kWave + lane;
+            t[k] = p < total ? sortedTile[p] : kNone;
+        }
+        const uint32_t before = (lane == 0 && base > 0) ? sortedTile[base - 1] : kNone;
+        const uint32_t after = (lane == kWave - 1 && base + kRun < total) ? sortedTile[base + kRun] : kNone;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t pOld = k == 0 ? before : static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(t[k - 1]), kWave - 1));
+            const uint32_t nOld = k == 3 ? after : static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(t[k + 1]), 0));
+            const uint32_t prev = static_cast<uint32_t>(dppMove<kDppWaveShr1>(static_cast<int>(pOld), static_cast<int>(t[k])));
+            const uint32_t next = static_cast<uint32_t>(dppMove<kDppWaveShl1>(static_cast<int>(nOld), static_cast<int>(t[k])));
+            const uint32_t p = base + k * kWave + lane;
+            if (p < total) {
+                if (p == 0 || prev != t[k]) ranges[t[k]].x = p;
+                if (p == total - 1 || next != t[k]) ranges[t[k]].y = p + 1;
+            }
         }
     }
 }
