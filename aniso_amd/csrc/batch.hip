@@ -269,6 +269,10 @@ const uint32_t* enqueueBatch(ViewBatch& B, const SceneDev& s, const Layout& L, h
     return sortedGauss;
 }
 
+uint32_t tileKey(const ans_options& o) {
+    return static_cast<uint32_t>(o.tile_w) << 16 | static_cast<uint32_t>(o.tile_h);
+}
+
 Layout makeLayout(const ViewBatch& B, const SceneDev& s, int views, const std::vector<CamParams>& cps,
                   const ans_options& o, bool diff) {
     Layout L{};
@@ -294,7 +298,8 @@ Layout makeLayout(const ViewBatch& B, const SceneDev& s, int views, const std::v
     L.digitBits = (L.tileBits + L.passes - 1) / L.passes;
     L.blocksD = std::max<uint32_t>(1, (L.n + kSortBlock * kPairItems - 1) / (kSortBlock * kPairItems));
     L.blocksDup = std::max<uint32_t>(1, (L.n + kDupTile - 1) / kDupTile);
-    L.pairCap = B.pairCap ? B.pairCap : std::max<size_t>(size_t(L.n) * 8, size_t(1) << 20);
+    const auto cap = B.capByTile.find(tileKey(o));
+    L.pairCap = cap != B.capByTile.end() ? cap->second : std::max<size_t>(size_t(L.n) * 8, size_t(1) << 20);
     L.blocksP = static_cast<uint32_t>((L.pairCap + kSortBlock * kPairItems - 1) / (kSortBlock * kPairItems));
     L.blocksI = std::max<uint32_t>(1, (L.items + kSortBlock * kDepthItems - 1) / (kSortBlock * kDepthItems));
     return L;
@@ -372,7 +377,8 @@ bool finishBatch(ViewBatch& B, const Layout& L, const std::vector<CamParams>& cp
     }
     if (over) {
         if (most >= (1u << 30) - 1) throw std::runtime_error("pair count of a view exceeds 2^30");
-        B.pairCap = std::max(B.pairCap, static_cast<size_t>(most) + most / 4 + 1024);
+        B.pairCap = std::max(L.pairCap, static_cast<size_t>(most) + most / 4 + 1024);
+        B.capByTile[tileKey(L.opts)] = B.pairCap;
     }
     return over;
 }

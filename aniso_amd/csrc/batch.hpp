@@ -82,7 +82,11 @@ struct ViewBatch {
     uint8_t* tableHost[2] = {};
     FrameCtl* ctlHost[2] = {};
     const uint32_t* sortedSlot[2] = {};
-    size_t pairCap = 0;       // pairs per view
+    size_t pairCap = 0;       // pairs per view of the last batch
+    // Pairs per view by tile shape (tile_w << 16 | tile_h): a batch's capacity follows what its
+    // own tile shape has needed, so that the sort grids of a 16x16 batch are not sized for the
+    // pairs an 8x8 batch of the same views once had (the autotuner renders both).
+    std::map<uint32_t, size_t> capByTile;
     uint64_t generation = 0;  // bumped when the arena moves
 
     // views, n, tile w, tile h, batch depth, cull, max tiles, total tiles, pair capacity,
