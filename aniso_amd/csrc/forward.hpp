@@ -291,23 +291,48 @@ __device__ inline void storeCulled(size_t o, SplatRec* __restrict__ rec, uint32_
     depthKey[o] = 0xffffffffu;
 }
 
-// Writes a visible Gaussian's splat record (with its tile rect and count), tile count (unless
-// tilesTouched is null), packed tile rect (a compact copy of the record's, which the pair count
-// gathers in depth order) and depth key at index o.
-__device__ inline void storeSplat(const GaussProj& g, size_t o, SplatRec* __restrict__ rec,
-                                  uint32_t* __restrict__ tilesTouched, uint32_t* __restrict__ depthKey,
-                                  uint2* __restrict__ rect) {
+// A visible Gaussian's splat record (with its tile rect and count) and its packed tile rect.
+__device__ inline SplatRec splatRecord(const GaussProj& g, uint2& rp) {
     const uint32_t tt = static_cast<uint32_t>((g.x1 - g.x0 + 1) * (g.y1 - g.y0 + 1));
-    SplatRec& r = rec[o];
+    rp = make_uint2(static_cast<uint32_t>(g.x0) | (static_cast<uint32_t>(g.y0) << 16),
+                    static_cast<uint32_t>(g.x1) | (static_cast<uint32_t>(g.y1) << 16));
+    SplatRec r;
     r.geo = make_float4(g.u, g.v, g.A, g.B);
     r.geo2 = make_float4(g.C, g.opacity, 0.0f, 0.0f);
     r.col = make_float4(g.rgb.x, g.rgb.y, g.rgb.z, g.depth);
-    const uint2 rp = make_uint2(static_cast<uint32_t>(g.x0) | (static_cast<uint32_t>(g.y0) << 16),
-                                static_cast<uint32_t>(g.x1) | (static_cast<uint32_t>(g.y1) << 16));
     r.meta = make_uint4(rp.x, rp.y, tt, static_cast<uint32_t>(g.radius));
+    return r;
+}
+
+// Writes a visible Gaussian's splat record, tile count (unless tilesTouched is null), packed tile
+// rect (a compact copy of the record's, which the pair count gathers in depth order) and depth key
+// at index o.
+__device__ inline void storeSplat(const GaussProj& g, size_t o, SplatRec* __restrict__ rec,
+                                  uint32_t* __restrict__ tilesTouched, uint32_t* __restrict__ depthKey,
+                                  uint2* __restrict__ rect) {
+    uint2 rp;
+    const SplatRec r = splatRecord(g, rp);
+    rec[o] = r;
     rect[o] = rp;
-    if (tilesTouched) tilesTouched[o] = tt;
+    if (tilesTouched) tilesTouched[o] = r.meta.z;
     depthKey[o] = __float_as_uint(g.depth);
+}
+
+// Writes the records staged in sRec (one per lane of a 256-lane workgroup) for Gaussians
+// [first, first + 256) below n: lane l writes 16-byte words l, l + 256, l + 512, l + 768 of the
+// run, so each store of a wavefront covers one contiguous kilobyte. A culled Gaussian's record
+// (tile count 0, staged as zeros) gets only its meta word, as storeCulled writes it. Called by
+// every lane after the staging and a barrier.
+__device__ inline void writeStagedRecords(const SplatRec* sRec, SplatRec* __restrict__ rec, uint32_t first,
+                                          uint32_t n) {
+    const uint32_t words = first < n ? 4u * min(256u, n - first) : 0u;
+    const float4* src = reinterpret_cast<const float4*>(sRec);
+    float4* dst = reinterpret_cast<float4*>(rec + first);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t w = k * 256 + threadIdx.x;
+        if (w < words && ((w & 3u) == 3u || sRec[w >> 2].meta.z != 0u)) dst[w] = src[w];
+    }
 }
 
 // Per-Gaussian preprocess, one lane per Gaussian. Writes the splat record
@@ -336,15 +361,28 @@ __global__ __launch_bounds__(256) void k_preprocess(int n, const float* __restri
         ctl->finished = 0;
     }
     if (static_cast<uint32_t>(i) < tiles) ranges[i] = make_uint2(0, 0);
-    if (i >= n) return;
-    GaussProj g;
-    const float* shBase = PLANAR ? sh + i : sh + static_cast<size_t>(i) * cam.shCoeffs * 3;
-    if (!projectGaussian<PLANAR, RAW>(i, pos, scale, rot, opacity, shBase, PLANAR ? static_cast<size_t>(n) : 1, cam,
-                                      g, act)) {
-        storeCulled(i, rec, tilesTouched, depthKey, rect);
-        return;
+    // The workgroup's 256 records are staged in LDS and written out as 16 contiguous kilobytes
+    // (writeStagedRecords), instead of four 16-byte stores per lane 64 bytes apart.
+    __shared__ SplatRec sRec[256];
+    SplatRec r{};
+    if (i < n) {
+        GaussProj g;
+        const float* shBase = PLANAR ? sh + i : sh + static_cast<size_t>(i) * cam.shCoeffs * 3;
+        uint2 rp = kEmptyRect;
+        uint32_t key = 0xffffffffu;
+        if (projectGaussian<PLANAR, RAW>(i, pos, scale, rot, opacity, shBase, PLANAR ? static_cast<size_t>(n) : 1,
+                                         cam, g, act)) {
+            r = splatRecord(g, rp);
+            key = __float_as_uint(g.depth);
+        }
+        rect[i] = rp;
+        if (tilesTouched) tilesTouched[i] = r.meta.z;
+        depthKey[i] = key;
     }
-    storeSplat(g, i, rec, tilesTouched, depthKey, rect);
+    sRec[threadIdx.x] = r;
+    __syncthreads();
+    // The grid covers max(n, tiles) lanes (the range clear): workgroups past n write no records.
+    writeStagedRecords(sRec, rec, blockIdx.x * 256u, static_cast<uint32_t>(n));
 }
 
 constexpr int kDupBlock = 256;                       // 4 wavefronts x 64 depth ranks
